@@ -1,0 +1,386 @@
+/*
+ * lsdg.h — C-ABI of the MI355X (gfx950) LSD-SLAM per-pixel front end.
+ *
+ * The reference wrapper (apl-ocean-engineering/lsd-slam-pangolin-gui) links
+ * the LSD-SLAM core as the external fips module `lsdslam`
+ * (fips.yml:2-4, lib/CMakeLists.txt:16, tools/CMakeLists.txt:6) and reaches
+ * the hot path only through SlamSystem::nextImage
+ * (lib/App/InputThread.cpp:71).  Inside that core the hot path is the C++
+ * classes SE3Tracker, Sim3Tracker, TrackingReference, Frame and DepthMap.
+ * They have no FFI of their own; the entry points below are what such an FFI
+ * binds, one per public method, with plain pointers and sizes.  The C++ shim
+ * in lsd-slam-pangolin-gui_amd/shim/ wraps them back into those classes.
+ *
+ * Conventions
+ *   - images and per-pixel maps are row-major, idx = x + y*w;
+ *   - SE3 poses are double[7]  = (qx, qy, qz, qw, tx, ty, tz), unit quaternion;
+ *   - Sim3 poses are double[8] = (qx, qy, qz, qw, tx, ty, tz, scale);
+ *   - twists are (v1, v2, v3, w1, w2, w3[, sigma]); updates are T <- exp(d) * T;
+ *   - pyramid level l has size (w >> l) x (h >> l), fx_l = fx / 2^l,
+ *     cx_l = (cx + 0.5) / 2^l - 0.5 (likewise fy, cy);
+ *   - inverse depth <= 0 or variance <= 0 marks a pixel without depth, as in the
+ *     keyframe arrays the wrapper publishes
+ *     (lib/Pangolin_IOWrapper/PangolinOutputIOWrapper.cpp:77-89).
+ *
+ * Errors are returned, never thrown.  A tracker that diverges returns
+ * LSDG_DIVERGED and writes the identity pose, as the reference trackers
+ * return SE3() / Sim3() with `diverged` set.
+ *
+ * Threading: one lsdg_ctx per host thread (each owns one HIP stream).  Frames,
+ * references and depth maps belong to the context that created them.
+ */
+#ifndef LSDG_H
+#define LSDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSDG_PYRAMID_LEVELS 5
+
+typedef int32_t lsdg_status;
+#define LSDG_OK 0
+#define LSDG_DIVERGED 1
+#define LSDG_EINVAL (-1)
+#define LSDG_EDEVICE (-2)
+#define LSDG_ENOMEM (-3)
+#define LSDG_ESTATE (-4)
+
+typedef struct lsdg_ctx_s* lsdg_ctx;
+typedef struct lsdg_frame_s* lsdg_frame;
+typedef struct lsdg_ref_s* lsdg_ref;
+typedef struct lsdg_depthmap_s* lsdg_depthmap;
+typedef struct lsdg_batch_s* lsdg_batch;
+
+/* Pyramid storage precision of (I, gx, gy).  Residuals, weights and sums are
+ * always computed in FP32; normal equations are merged and solved in FP64. */
+#define LSDG_FP32 0
+#define LSDG_FP16 1
+#define LSDG_BF16 2
+
+typedef struct lsdg_camera {
+  int32_t w, h;
+  float fx, fy, cx, cy;
+} lsdg_camera;
+
+/* Tuning constants of the LSD-SLAM core (settings.h, DenseDepthTrackerSettings
+ * and the Conf() singleton the wrapper configures at tools/LSD.cpp:97-100).
+ * lsdg_params_default() fills the upstream defaults. */
+typedef struct lsdg_params {
+  /* tracker (per pyramid level) */
+  float lambda_initial[LSDG_PYRAMID_LEVELS];
+  float lambda_success_fac[LSDG_PYRAMID_LEVELS];
+  float lambda_fail_fac[LSDG_PYRAMID_LEVELS];
+  float step_size_min[LSDG_PYRAMID_LEVELS];
+  float convergence_eps[LSDG_PYRAMID_LEVELS];
+  int32_t max_its_per_lvl[LSDG_PYRAMID_LEVELS];
+  float var_weight;
+  float huber_d;
+  float camera_pixel_noise2;
+  int32_t se3_min_level;          /* finest tracked level (1) */
+  int32_t se3_max_level;          /* one past the coarsest tracked level (5) */
+  float min_goodperall_pixel;
+  float min_goodperall_pixel_absmin;
+  float min_goodpergoodbad_pixel;
+  float max_diff_constant;
+  float max_diff_grad_mult;
+  int32_t use_affine_lightning_estimation;
+  int32_t use_esm_tracking;       /* Sim3: average frame and rotated reference gradients */
+  /* depth map */
+  float min_abs_grad_create;
+  float min_abs_grad_decrease;
+  float min_epl_length_squared;
+  float min_epl_grad_squared;
+  float min_epl_angle_squared;
+  float gradient_sample_dist;
+  float sample_point_to_border;
+  float max_epl_length_crop;
+  float min_epl_length_crop;
+  float max_error_stereo;
+  float min_distance_error_stereo;
+  float stereo_epl_var_fac;
+  float min_depth;
+  float max_var;
+  float var_random_init_initial;
+  float var_gt_init_initial;
+  float succ_var_inc_fac;
+  float fail_var_inc_fac;
+  float diff_fac_observe;
+  float diff_fac_prop_merge;
+  float diff_fac_smoothing;
+  float depth_smoothing_factor;
+  float division_eps;
+  int32_t validity_counter_inc;
+  int32_t validity_counter_dec;
+  int32_t validity_counter_initial_observe;
+  int32_t validity_counter_max;
+  int32_t validity_counter_max_variable;
+  int32_t val_sum_min_for_create;
+  int32_t val_sum_min_for_keep;
+  int32_t val_sum_min_for_unblacklist;
+  int32_t min_blacklist;
+  int32_t allow_negative_idepths;
+  int32_t use_subpixel_stereo;
+  int32_t stereo_subpixel_oob_guard; /* skip the sub-pixel step when the best match has no pre/post sample */
+  int32_t stereo_max_steps;          /* safety cap on epipolar search steps (>= crop + 16) */
+} lsdg_params;
+
+/* Per-tracking statistics: the public result fields of SE3Tracker / Sim3Tracker. */
+typedef struct lsdg_track_stats {
+  float last_residual;          /* SE3: final lastErr; Sim3: mean */
+  float last_depth_residual;    /* Sim3 meanD */
+  float last_photometric_residual; /* Sim3 meanP */
+  float last_mean_res;          /* SE3 lastMeanRes */
+  float point_usage;
+  float affine_a, affine_b;
+  int32_t last_good_count;
+  int32_t last_bad_count;
+  int32_t diverged;
+  int32_t tracking_was_good;
+  int32_t num_residual_calls[LSDG_PYRAMID_LEVELS];
+  int32_t num_warp_update_calls[LSDG_PYRAMID_LEVELS];
+  double hessian[49];           /* Sim3: lastSim3Hessian (row-major 7x7) */
+} lsdg_track_stats;
+
+/* One fused evaluation of the tracker at a fixed pose and level: the sums of
+ * calcResidualAndBuffers + calcWeightsAndResidual + calculateWarpUpdate
+ * (SE3) or calcSim3Buffers + calcSim3WeightsAndResidual + calcSim3LGS (Sim3).
+ * Normal equations are NOT divided by n. */
+typedef struct lsdg_eval {
+  double A[49];        /* SE3: top-left 6x6 of a 7-stride matrix; Sim3: full 7x7 */
+  double b[7];
+  double error;        /* sum W r^2 (SE3); Sim3: sumResP + sumResD */
+  double sum_res_p, sum_res_d;
+  double num_terms_p, num_terms_d;
+  double n_constraints;
+  int32_t warped_count;      /* buf_warped_size */
+  int32_t good_count, bad_count;
+  double sum_res_unweighted; /* sum over good points of r^2 */
+  double sum_signed_res;     /* sum over good points of r */
+  double usage;              /* sum min(1, z / z') */
+  double sxx, syy, sx, sy, sw; /* affine brightness statistics */
+} lsdg_eval;
+
+/* Per-pixel depth hypotheses (DepthMapPixelHypothesis), structure of arrays.
+ * Any NULL member is skipped on read and left unchanged on write. */
+typedef struct lsdg_hypotheses {
+  uint8_t* is_valid;
+  int32_t* blacklisted;
+  float* next_stereo_frame_min_id;
+  int32_t* validity_counter;
+  float* idepth;
+  float* idepth_var;
+  float* idepth_smoothed;
+  float* idepth_var_smoothed;
+  uint8_t* observe_code;  /* outcome of the last observeDepth per pixel (LSDG_OBS_*) */
+} lsdg_hypotheses;
+
+/* observeDepth outcome codes (one per pixel, from the last lsdg_depth_update). */
+#define LSDG_OBS_NONE 0            /* outside [3,w-3)x[3,h-3), low gradient or blacklisted */
+#define LSDG_OBS_LOWGRAD_INVALIDATE 1
+#define LSDG_OBS_SKIP_ALREADY_GOOD 2
+#define LSDG_OBS_SKIP_NOT_GOOD_TRACKED 3
+#define LSDG_OBS_EPL_FAIL 4
+#define LSDG_OBS_CREATE_VAR_TOO_BIG 5
+#define LSDG_OBS_CREATED 6
+#define LSDG_OBS_INCONSISTENT 7
+#define LSDG_OBS_UPDATED 8
+#define LSDG_OBS_STEREO_FAIL_BASE 10  /* 10 - code for doLineStereo codes -1..-4 (11..14) */
+
+/* What lsdg_frame_read returns. */
+#define LSDG_WHAT_IMAGE 0           /* float[w_l*h_l] */
+#define LSDG_WHAT_GRADIENTS 1       /* float[4*w_l*h_l]: gx, gy, I, 0 (FP32 view) */
+#define LSDG_WHAT_MAXGRAD 2         /* float[w_l*h_l] */
+#define LSDG_WHAT_IDEPTH 3          /* float[w_l*h_l] */
+#define LSDG_WHAT_IDEPTH_VAR 4      /* float[w_l*h_l] */
+#define LSDG_WHAT_REF_PIXEL_WAS_GOOD 5 /* uint8[w_1*h_1] (level ignored) */
+
+/* ---------------------------------------------------------------- context */
+
+/* Fills the upstream defaults (settings.cpp / settings.h). */
+void lsdg_params_default(lsdg_params* p);
+
+/* One context per host thread: device, stream and the parameters that
+ * settings.h / Conf() hold globally in the reference (tools/LSD.cpp:97-100). */
+lsdg_status lsdg_ctx_create(int32_t device, const lsdg_params* params, lsdg_ctx* out);
+lsdg_status lsdg_ctx_destroy(lsdg_ctx ctx);
+lsdg_status lsdg_ctx_set_params(lsdg_ctx ctx, const lsdg_params* params);
+lsdg_status lsdg_ctx_synchronize(lsdg_ctx ctx);
+/* Last error message of this context (static storage, valid until next call). */
+const char* lsdg_ctx_last_error(lsdg_ctx ctx);
+
+/* ------------------------------------------------------------------ frames */
+
+/* Frame(id, width, height, K, timestamp, image): uploads the level-0 image
+ * (float, w*h) from host memory and builds image, gradient and
+ * max-gradient pyramids on the device (Frame::buildImage / buildGradients /
+ * buildMaxGradients).  The wrapper's input is an undistorted CV_8UC1 image
+ * (lib/App/InputThread.cpp:59-65); lsdg_frame_create_u8 takes that form. */
+lsdg_status lsdg_frame_create(lsdg_ctx ctx, int32_t id, const lsdg_camera* cam,
+                              const float* image, int32_t precision, lsdg_frame* out);
+lsdg_status lsdg_frame_create_u8(lsdg_ctx ctx, int32_t id, const lsdg_camera* cam,
+                                 const uint8_t* image, int32_t stride, int32_t precision,
+                                 lsdg_frame* out);
+lsdg_status lsdg_frame_destroy(lsdg_frame f);
+/* Rebuilds every pyramid level from the resident level-0 image (the per-frame
+ * work a new frame costs the tracker). */
+lsdg_status lsdg_frame_build_pyramid(lsdg_frame f);
+/* Frame::setDepth from plain arrays (idepth <= 0 or var <= 0: no depth), then
+ * the inverse-variance depth pyramid (Frame::buildIDepthAndIDepthVar). */
+lsdg_status lsdg_frame_set_depth(lsdg_frame f, const float* idepth, const float* idepth_var);
+lsdg_status lsdg_frame_read(lsdg_frame f, int32_t what, int32_t level, void* out, size_t bytes);
+/* Frame pose fields used by mapping: trackingParent, thisToParent_raw,
+ * initialTrackedResidual and refPixelWasGood (NULL keeps the current mask;
+ * a fresh frame's mask is all true).  The reference writes these at the end
+ * of SE3Tracker::trackFrame; lsdg_se3_track does the same. */
+lsdg_status lsdg_frame_set_tracking(lsdg_frame f, lsdg_frame parent, const double this_to_parent[8],
+                                    float initial_tracked_residual, const uint8_t* ref_pixel_was_good);
+lsdg_status lsdg_frame_get_tracking(lsdg_frame f, int32_t* parent_id, double this_to_parent[8],
+                                    float* initial_tracked_residual);
+/* Frame::getScaledCamToWorld, used by updateKeyframe when a reference frame was
+ * tracked on another keyframe. */
+lsdg_status lsdg_frame_set_cam_to_world(lsdg_frame f, const double cam_to_world[8]);
+/* Keyframe counters numFramesTrackedOnThis / numMappedOnThis. */
+lsdg_status lsdg_frame_counters(lsdg_frame f, int32_t* num_frames_tracked_on_this,
+                                int32_t* num_mapped_on_this);
+lsdg_status lsdg_frame_set_counters(lsdg_frame f, int32_t num_frames_tracked_on_this,
+                                    int32_t num_mapped_on_this);
+
+/* ------------------------------------------------------- tracking reference */
+
+/* TrackingReference::importFrame(keyframe). */
+lsdg_status lsdg_ref_create(lsdg_ctx ctx, lsdg_frame keyframe, lsdg_ref* out);
+lsdg_status lsdg_ref_destroy(lsdg_ref r);
+/* TrackingReference::makePointCloud(level): stream-compacts the keyframe's
+ * pixels with depth into points (x, y, z, I, var, idx).  Idempotent. */
+lsdg_status lsdg_ref_make_point_cloud(lsdg_ref r, int32_t level);
+lsdg_status lsdg_ref_num_points(lsdg_ref r, int32_t level, int32_t* n);
+/* Reads points as float[8] records: x, y, z, I, var, gx, gy, idx (as float). */
+lsdg_status lsdg_ref_read_points(lsdg_ref r, int32_t level, float* out, int32_t max_points);
+
+/* --------------------------------------------------------------- trackers */
+
+/* SE3Tracker::trackFrame(TrackingReference*, Frame*, const SE3& frameToReference_initialEstimate).
+ * Returns frameToReference.  Writes the frame's tracking fields (trackingParent,
+ * thisToParent_raw, initialTrackedResidual, refPixelWasGood) as the reference does. */
+lsdg_status lsdg_se3_track(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame,
+                           const double frame_to_ref_init[7], double frame_to_ref_out[7],
+                           lsdg_track_stats* stats);
+
+/* Sim3Tracker::trackFrameSim3(TrackingReference*, Frame*, const Sim3&, int startLevel, int finalLevel).
+ * The frame must carry depth (lsdg_frame_set_depth). */
+lsdg_status lsdg_sim3_track(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame,
+                            const double frame_to_ref_init[8], int32_t start_level,
+                            int32_t final_level, double frame_to_ref_out[8],
+                            lsdg_track_stats* stats);
+
+/* One fused evaluation at referenceToFrame (SE3: double[7], scale ignored;
+ * Sim3: double[8]).  is_good (optional, uint8[w_1*h_1]) receives the
+ * refPixelWasGood writes of that evaluation (only when level == se3_min_level). */
+lsdg_status lsdg_se3_evaluate(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, int32_t level,
+                              const double ref_to_frame[7], float affine_a, float affine_b,
+                              lsdg_eval* out, uint8_t* is_good);
+lsdg_status lsdg_sim3_evaluate(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, int32_t level,
+                               const double ref_to_frame[8], lsdg_eval* out);
+
+/* Adjoint of the Huber-weighted photometric energy at referenceToFrame, with
+ * the depth-variance weights w_p and the in-image mask frozen at that pose:
+ *   E = sum_i rho(sqrt(w_p,i) r_i),  rho Huber at huber_d/2,
+ *   grad_xi[6]     = dE/dxi for T <- exp(xi) T,
+ *   grad_idepth[i] = dE/d(idepth of point i), point order of lsdg_ref_read_points
+ *                    (0 for points outside the image).
+ * No counterpart in the reference; it is the reduction behind LGS6::b. */
+lsdg_status lsdg_energy_grad(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, int32_t level,
+                             const double ref_to_frame[7], double* energy, double grad_xi[6],
+                             float* grad_idepth, int32_t max_points);
+
+/* ------------------------------------------------------------- depth map */
+
+lsdg_status lsdg_depthmap_create(lsdg_ctx ctx, const lsdg_camera* cam, lsdg_depthmap* out);
+lsdg_status lsdg_depthmap_destroy(lsdg_depthmap dm);
+/* DepthMap::reset / isValid. */
+lsdg_status lsdg_depth_reset(lsdg_depthmap dm);
+int32_t lsdg_depth_is_valid(lsdg_depthmap dm);
+/* DepthMap::initializeFromGTDepth(Frame*): the keyframe's idepth becomes the
+ * hypotheses (var VAR_GT_INIT_INITIAL, validity 20). */
+lsdg_status lsdg_depth_initialize_from_gt(lsdg_depthmap dm, lsdg_frame keyframe);
+/* Prior hypotheses from arrays (idepth, var, validity) on the keyframe, then
+ * setDepth.  Pixels with var <= 0 or non-finite idepth are invalid. */
+lsdg_status lsdg_depth_set_prior(lsdg_depthmap dm, lsdg_frame keyframe, const float* idepth,
+                                 const float* idepth_var, int32_t validity);
+/* DepthMap::updateKeyframe(std::deque<std::shared_ptr<Frame>> referenceFrames),
+ * frames ordered oldest first. */
+lsdg_status lsdg_depth_update(lsdg_depthmap dm, const lsdg_frame* refs, int32_t n);
+/* DepthMap::createKeyFrame(Frame* new_keyframe). */
+lsdg_status lsdg_depth_create_keyframe(lsdg_depthmap dm, lsdg_frame new_keyframe);
+/* DepthMap::finalizeKeyFrame(): fill holes, regularize, setDepth. */
+lsdg_status lsdg_depth_finalize_keyframe(lsdg_depthmap dm);
+lsdg_status lsdg_depth_read(lsdg_depthmap dm, lsdg_hypotheses* out);
+lsdg_status lsdg_depth_write(lsdg_depthmap dm, const lsdg_hypotheses* in);
+/* Keyframe the map currently belongs to (activeKeyFrame) and its id. */
+lsdg_status lsdg_depth_active_keyframe(lsdg_depthmap dm, int32_t* id);
+
+/* ------------------------------------------------------------ measurement */
+
+/* Per-kernel device time, recorded with a hipEvent pair around every launch
+ * on the context's stream, and the algorithmic HBM bytes each launch is
+ * charged (tracker evaluation at level l: N_l * 21 + P_l * b_px, b_px = 12
+ * for FP32 and 6 for FP16/BF16 pyramids; Sim3 adds N_l * 9 + P_l * 8).
+ * No counterpart in the reference, whose core only keeps gettimeofday
+ * averages (msTrackFrame, msObserve, ...).  Off by default. */
+#define LSDG_KERNEL_EVAL 0        /* fused SE3 / Sim3 evaluation (plain path) */
+#define LSDG_KERNEL_PYRAMID 1     /* frame pyramid build */
+#define LSDG_KERNEL_DEPTH 2       /* depth-map kernels of one update */
+#define LSDG_KERNEL_BATCH_EVAL 3  /* batched evaluation sweep */
+#define LSDG_KERNEL_BATCH_SOLVE 4 /* batched LM step (merge, solve, accept) */
+#define LSDG_KERNEL_KINDS 8
+
+typedef struct lsdg_kernel_stats {
+  int64_t launches;
+  double ms;     /* summed device time */
+  double bytes;  /* summed algorithmic bytes */
+} lsdg_kernel_stats;
+
+/* enable != 0 resets the statistics and starts recording. */
+lsdg_status lsdg_ctx_profile(lsdg_ctx ctx, int32_t enable);
+lsdg_status lsdg_ctx_kernel_stats(lsdg_ctx ctx, int32_t kernel, lsdg_kernel_stats* out);
+
+/* ------------------------------------------------------- batched tracking */
+
+#define LSDG_JOB_SE3 0
+#define LSDG_JOB_SIM3 1
+
+/* One frame-vs-keyframe alignment of a batch. */
+typedef struct lsdg_job {
+  lsdg_ref ref;
+  lsdg_frame frame;
+  double frame_to_ref_init[8];
+  int32_t kind;         /* LSDG_JOB_SE3 / LSDG_JOB_SIM3 */
+  int32_t start_level;  /* Sim3 only; SE3 uses se3_max_level-1 .. se3_min_level */
+  int32_t final_level;
+  int32_t rebuild_pyramid; /* rebuild the frame pyramid inside the batch */
+} lsdg_job;
+
+typedef struct lsdg_job_result {
+  double frame_to_ref[8];
+  lsdg_track_stats stats;
+  lsdg_status status;
+} lsdg_job_result;
+
+/* Plans a batch: many alignments of uneven size run in one persistent launch
+ * per LM step, the steps captured in a hipGraph.  The plan may be run many
+ * times (frames may change content, not shape). */
+lsdg_status lsdg_batch_create(lsdg_ctx ctx, const lsdg_job* jobs, int32_t n, lsdg_batch* out);
+lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results);
+lsdg_status lsdg_batch_destroy(lsdg_batch b);
+/* Convenience: create, run once, destroy. */
+lsdg_status lsdg_batch_track(lsdg_ctx ctx, const lsdg_job* jobs, int32_t n, lsdg_job_result* results);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSDG_H */

@@ -1,0 +1,34 @@
+// Helpers shared by the C-ABI translation units.
+#pragma once
+#include "lsdg_internal.h"
+
+namespace lsdg {
+
+// Records `what` (and the HIP error) as the context's last error; returns st.
+lsdg_status fail(lsdg_ctx c, lsdg_status st, const char* what, hipError_t e = hipSuccess);
+lsdg_status frame_depth_from_level0(lsdg_frame f);
+lsdg_status frame_ensure_was_good(lsdg_frame f);
+lsdg_status ref_points(lsdg_ref r, int level);
+Xf pose_xf(const Pose& T, bool scaled);
+TrackKParams track_kparams(const lsdg_params& P, bool sim3);
+void esm_roll(const Pose& T, float* r4);
+double eval_bytes(int n, lsdg_frame f, int lvl, bool sim3);
+
+// Kernel timing: begin() records the start event (if profiling is on) and
+// returns a handle for end(), which records the stop event and the bytes.
+struct ProfMark {
+  int kind = -1;
+  hipEvent_t a = nullptr;
+};
+ProfMark prof_begin(lsdg_ctx c, int kind);
+void prof_end(lsdg_ctx c, ProfMark& m, double bytes);
+// Synchronises the stream and folds finished event pairs into the totals.
+lsdg_status prof_resolve(lsdg_ctx c);
+
+}  // namespace lsdg
+
+#define CK(ctx, expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) return ::lsdg::fail((ctx), LSDG_EDEVICE, #expr, e_); \
+  } while (0)

@@ -1,0 +1,154 @@
+// Internal object layouts of liblsdg.so: contexts, frames (device pyramids),
+// tracking references (device point clouds), depth maps (device hypothesis
+// buffers) and the kernel launchers the C-ABI layer calls.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lsdg.h"
+#include "lsdg_lie.h"
+
+namespace lsdg {
+
+constexpr int PYR = LSDG_PYRAMID_LEVELS;
+
+// Number of FP64 sums one fused tracker evaluation produces.
+//  SE3 layout: [0,21) A upper triangle (row-major), [21,27) b, 27 err,
+//  28 sumResUnweighted(good), 29 sumSignedRes(good), 30 usage,
+//  31..35 sxx syy sx sy sw, 36 warped, 37 good, 38 bad.
+//  Sim3 layout: [0,28) A upper triangle of 7x7, [28,35) b, 35 sumResP,
+//  36 sumResD, 37 numTermsD, 38 warped, 39 usage, 40..44 sxx syy sx sy sw,
+//  45 errP (= sumResP, sum W_p r_p^2 from the LGS pass), 46 errD.
+constexpr int NS_SE3 = 40;
+constexpr int NS_SIM3 = 48;
+constexpr int NS_MAX = 48;
+
+enum Se3Slot { S3_A = 0, S3_B = 21, S3_ERR = 27, S3_RESUNW = 28, S3_SIGNED = 29, S3_USAGE = 30, S3_SXX = 31,
+               S3_SYY = 32, S3_SX = 33, S3_SY = 34, S3_SW = 35, S3_WARPED = 36, S3_GOOD = 37, S3_BAD = 38 };
+enum Sim3Slot { SM_A = 0, SM_B = 28, SM_RESP = 35, SM_RESD = 36, SM_NTD = 37, SM_WARPED = 38, SM_USAGE = 39,
+                SM_SXX = 40, SM_SYY = 41, SM_SX = 42, SM_SY = 43, SM_SW = 44 };
+
+struct LevelGeom {
+  int w, h;
+  float fx, fy, cx, cy;
+};
+
+// Rigid (or scaled) transform in FP32 for per-point work.
+struct Xf {
+  float R[9];
+  float t[3];
+};
+
+struct TrackKParams {
+  float huber;        // SE3: huber_d / 2 ; Sim3: huber_d
+  float var_weight;
+  float noise2;       // cameraPixelNoise2
+  float max_diff_constant;
+  float max_diff_grad_mult;
+  float aff_a, aff_b;
+  int use_esm;
+  float roll[4];      // Sim3 ESM roll (xRoll0, xRoll1, yRoll0, yRoll1)
+};
+
+struct PointsLevel {
+  const float4* posI;  // x, y, z, I
+  const float* var;
+  const float2* grad;  // keyframe gx, gy (Sim3 ESM)
+  const int* idx;      // pixel index in the keyframe level
+  int n;
+};
+
+}  // namespace lsdg
+
+// ------------------------------------------------------------ C-ABI objects
+struct lsdg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  lsdg_params P;
+  std::string err;
+  // reduction scratch for the plain (one-alignment-at-a-time) path
+  double* d_partials = nullptr;
+  int partials_blocks = 0;
+  unsigned* d_counter = nullptr;
+  double* h_result = nullptr;  // pinned, host-mapped
+  double* d_result = nullptr;  // device view of h_result
+  int num_cus = 256;
+  // per-kernel timing (lsdg_ctx_profile)
+  struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  bool prof_on = false;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_events;
+  lsdg_kernel_stats prof[LSDG_KERNEL_KINDS];
+};
+
+struct lsdg_frame_s {
+  lsdg_ctx ctx = nullptr;
+  int id = 0;
+  int w[lsdg::PYR], h[lsdg::PYR];
+  float fx[lsdg::PYR], fy[lsdg::PYR], cx[lsdg::PYR], cy[lsdg::PYR];
+  float fxi[lsdg::PYR], fyi[lsdg::PYR], cxi[lsdg::PYR], cyi[lsdg::PYR];
+  int precision = LSDG_FP32;
+  // one device allocation holds every pyramid
+  void* d_pool = nullptr;
+  float* d_image[lsdg::PYR];
+  float4* d_grad[lsdg::PYR];
+  uint2* d_gradh[lsdg::PYR];  // packed FP16/BF16 (gx, gy, I, 0); null for FP32 frames
+  float* d_maxg[lsdg::PYR];
+  float* d_idepth[lsdg::PYR];
+  float* d_ivar[lsdg::PYR];
+  uint8_t* d_input_u8 = nullptr;  // staging for u8 uploads (w*h)
+  uint8_t* d_wasGood = nullptr;   // refPixelWasGood (level 1)
+  bool wasGoodAllocated = false;
+  bool hasDepth = false;
+  // pose fields
+  int parentId = -1;
+  lsdg_frame parent = nullptr;
+  lsdg::Pose thisToParent = lsdg::pose_identity();
+  float initialTrackedResidual = 0.f;
+  lsdg::Pose camToWorld = lsdg::pose_identity();
+  int numFramesTrackedOnThis = 0, numMappedOnThis = 0;
+  // bumped when depth changes (invalidates reference point clouds)
+  int depthVersion = 0;
+  // stereo preparation (prepareForStereoWith), FP32 values as the reference caches them
+  float K_otherToThis_R[9], otherToThis_t[3], K_otherToThis_t[3], thisToOther_t[3], R_row[3][3];
+};
+
+struct lsdg_ref_s {
+  lsdg_ctx ctx = nullptr;
+  lsdg_frame kf = nullptr;
+  int depthVersion = -1;
+  int num[lsdg::PYR] = {-1, -1, -1, -1, -1};
+  int cap[lsdg::PYR] = {0, 0, 0, 0, 0};
+  void* d_pool[lsdg::PYR] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  lsdg::PointsLevel lv[lsdg::PYR];
+  int* d_blockcounts = nullptr;
+  int blockcounts_cap = 0;
+};
+
+// ---------------------------------------------------------- launchers
+namespace lsdg {
+
+// pyramid.hip
+hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s);
+hipError_t launch_pyr_grads(lsdg_frame f, hipStream_t s);
+hipError_t launch_pyr_depth(lsdg_frame f, hipStream_t s);
+hipError_t launch_depth_normalize(float* idepth, float* ivar, int n, hipStream_t s);
+hipError_t launch_point_cloud(lsdg_ref r, int level, hipStream_t s);  // fills r->lv[level], r->num[level]
+
+// track.hip
+hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                           const TrackKParams& kp, uint8_t* d_isgood, double* d_out);
+hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                            const TrackKParams& kp, double* d_out);
+hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                              const TrackKParams& kp, float* d_grad_d, double* d_out);
+
+}  // namespace lsdg

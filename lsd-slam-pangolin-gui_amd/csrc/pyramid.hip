@@ -1,0 +1,447 @@
+// Frame pyramids and TrackingReference point clouds on gfx950.
+//
+//  k_pyr_images  Frame::buildImage for all levels in one launch: a block owns a
+//                32x32 level-0 tile and reduces it through levels 1-4 in LDS
+//                (2x2 box, ((TL + TR) + BL) + BR, times 0.25).
+//  k_pyr_grads   Frame::buildGradients + buildMaxGradients for every level in
+//                one launch: one thread per pixel writes (gx, gy, I, 0), the
+//                optional FP16/BF16 packed copy and the 3x3 max of |grad I|
+//                (recomputed from the image; identical to the reference's
+//                two-pass vertical-then-horizontal max on its linear index).
+//  k_pyr_depth   Frame::buildIDepthAndIDepthVar for levels 1-4 in one launch.
+//  k_pc_*        TrackingReference::makePointCloud as a stream compaction
+//                (count, scan, write) in row-major pixel order.
+#include "lsdg_internal.h"
+
+namespace lsdg {
+
+struct PyrImgArgs {
+  float* img[PYR];
+  int w[PYR], h[PYR];
+};
+
+__global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, const uint8_t* __restrict__ u8) {
+  __shared__ float L1[16][17];
+  __shared__ float L2[8][9];
+  __shared__ float L3[4][5];
+  const int t = threadIdx.x;
+  const int bx = blockIdx.x, by = blockIdx.y;
+  const int w0 = p.w[0], h0 = p.h[0];
+  if (u8) {
+    for (int k = t; k < 1024; k += 256) {
+      int x = bx * 32 + (k & 31), y = by * 32 + (k >> 5);
+      if (x < w0 && y < h0) p.img[0][x + y * w0] = (float)u8[x + y * w0];
+    }
+  }
+  // level 1
+  {
+    int lx = t & 15, ly = t >> 4;
+    int x = bx * 16 + lx, y = by * 16 + ly;
+    float v = 0.f;
+    if (x < p.w[1] && y < p.h[1]) {
+      int s = 2 * x + 2 * y * w0;
+      float a, b, c, d;
+      if (u8) {
+        a = (float)u8[s]; b = (float)u8[s + 1]; c = (float)u8[s + w0]; d = (float)u8[s + w0 + 1];
+      } else {
+        const float* I = p.img[0];
+        a = I[s]; b = I[s + 1]; c = I[s + w0]; d = I[s + w0 + 1];
+      }
+      v = (a + b + c + d) * 0.25f;
+      p.img[1][x + y * p.w[1]] = v;
+    }
+    L1[ly][lx] = v;
+  }
+  __syncthreads();
+  if (t < 64) {
+    int lx = t & 7, ly = t >> 3;
+    int x = bx * 8 + lx, y = by * 8 + ly;
+    float v = 0.f;
+    if (x < p.w[2] && y < p.h[2]) {
+      v = (L1[2 * ly][2 * lx] + L1[2 * ly][2 * lx + 1] + L1[2 * ly + 1][2 * lx] + L1[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[2][x + y * p.w[2]] = v;
+    }
+    L2[ly][lx] = v;
+  }
+  __syncthreads();
+  if (t < 16) {
+    int lx = t & 3, ly = t >> 2;
+    int x = bx * 4 + lx, y = by * 4 + ly;
+    float v = 0.f;
+    if (x < p.w[3] && y < p.h[3]) {
+      v = (L2[2 * ly][2 * lx] + L2[2 * ly][2 * lx + 1] + L2[2 * ly + 1][2 * lx] + L2[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[3][x + y * p.w[3]] = v;
+    }
+    L3[ly][lx] = v;
+  }
+  __syncthreads();
+  if (t < 4) {
+    int lx = t & 1, ly = t >> 1;
+    int x = bx * 2 + lx, y = by * 2 + ly;
+    if (x < p.w[4] && y < p.h[4]) {
+      float v = (L3[2 * ly][2 * lx] + L3[2 * ly][2 * lx + 1] + L3[2 * ly + 1][2 * lx] + L3[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[4][x + y * p.w[4]] = v;
+    }
+  }
+}
+
+struct PyrGradArgs {
+  const float* img[PYR];
+  float4* grad[PYR];
+  uint2* gradh[PYR];
+  float* maxg[PYR];
+  int w[PYR], h[PYR];
+  int blk0[PYR + 1];
+  int prec;
+};
+
+__device__ inline float absgrad_at(const float* __restrict__ I, int j, int w, int h) {
+  if (j >= w && j < w * (h - 1)) {
+    float gx = 0.5f * (I[j + 1] - I[j - 1]);
+    float gy = 0.5f * (I[j + w] - I[j - w]);
+    return sqrtf(gx * gx + gy * gy);
+  }
+  return 0.f;
+}
+__device__ inline float vmax_at(const float* __restrict__ I, int j, int w, int h) {
+  if (j >= w + 1 && j < w * (h - 1) - 1) {
+    float g1 = absgrad_at(I, j - w, w, h), g2 = absgrad_at(I, j, w, h);
+    if (g1 < g2) g1 = g2;
+    float g3 = absgrad_at(I, j + w, w, h);
+    return (g1 < g3) ? g3 : g1;
+  }
+  return 0.f;
+}
+
+__device__ inline unsigned short f32_to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ inline uint2 pack_grad(float4 g, int prec) {
+  unsigned short a, b, c;
+  if (prec == LSDG_BF16) {
+    a = f32_to_bf16(g.x); b = f32_to_bf16(g.y); c = f32_to_bf16(g.z);
+  } else {
+    a = __half_as_ushort(__float2half_rn(g.x));
+    b = __half_as_ushort(__float2half_rn(g.y));
+    c = __half_as_ushort(__float2half_rn(g.z));
+  }
+  return make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c);
+}
+
+__global__ __launch_bounds__(256) void k_pyr_grads(PyrGradArgs p) {
+  int b = blockIdx.x;
+  int l = 0;
+  while (l < PYR - 1 && b >= p.blk0[l + 1]) l++;
+  int idx = (b - p.blk0[l]) * 256 + threadIdx.x;
+  int w = p.w[l], h = p.h[l];
+  if (idx >= w * h) return;
+  const float* I = p.img[l];
+  float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (idx >= w && idx < w * (h - 1)) {
+    g.x = 0.5f * (I[idx + 1] - I[idx - 1]);
+    g.y = 0.5f * (I[idx + w] - I[idx - w]);
+    g.z = I[idx];
+  }
+  p.grad[l][idx] = g;
+  if (p.gradh[l]) p.gradh[l][idx] = pack_grad(g, p.prec);
+  float m;
+  if (idx >= w + 1 && idx < w * (h - 1) - 1) {
+    float g1 = vmax_at(I, idx - 1, w, h), g2 = vmax_at(I, idx, w, h);
+    if (g1 < g2) g1 = g2;
+    float g3 = vmax_at(I, idx + 1, w, h);
+    m = (g1 < g3) ? g3 : g1;
+  } else {
+    m = absgrad_at(I, idx, w, h);
+  }
+  p.maxg[l][idx] = m;
+}
+
+struct PyrDepthArgs {
+  float* id[PYR];
+  float* var[PYR];
+  int w[PYR], h[PYR];
+};
+
+// Inverse-variance fusion of 4 children (order: (0,0), (1,0), (0,1), (1,1)).
+__device__ inline void fuse4(const float* ida, const float* va, float& oid, float& ov) {
+  float isum = 0.f, ivsum = 0.f;
+  int num = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float v = va[k];
+    if (v > 0.f) {
+      float iv = 1.0f / v;
+      ivsum += iv;
+      isum += iv * ida[k];
+      num++;
+    }
+  }
+  if (num > 0) {
+    float depth = ivsum / isum;
+    oid = 1.0f / depth;
+    ov = (float)num / ivsum;
+  } else {
+    oid = -1.f;
+    ov = -1.f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pyr_depth(PyrDepthArgs p) {
+  __shared__ float I1[16][17], V1[16][17], I2[8][9], V2[8][9], I3[4][5], V3[4][5];
+  const int t = threadIdx.x, bx = blockIdx.x, by = blockIdx.y;
+  const int w0 = p.w[0];
+  {
+    int lx = t & 15, ly = t >> 4, x = bx * 16 + lx, y = by * 16 + ly;
+    float oi = -1.f, ov = -1.f;
+    if (x < p.w[1] && y < p.h[1]) {
+      int s = 2 * x + 2 * y * w0;
+      float ia[4] = {p.id[0][s], p.id[0][s + 1], p.id[0][s + w0], p.id[0][s + w0 + 1]};
+      float va[4] = {p.var[0][s], p.var[0][s + 1], p.var[0][s + w0], p.var[0][s + w0 + 1]};
+      fuse4(ia, va, oi, ov);
+      p.id[1][x + y * p.w[1]] = oi;
+      p.var[1][x + y * p.w[1]] = ov;
+    }
+    I1[ly][lx] = oi;
+    V1[ly][lx] = ov;
+  }
+  __syncthreads();
+  if (t < 64) {
+    int lx = t & 7, ly = t >> 3, x = bx * 8 + lx, y = by * 8 + ly;
+    float oi = -1.f, ov = -1.f;
+    if (x < p.w[2] && y < p.h[2]) {
+      float ia[4] = {I1[2 * ly][2 * lx], I1[2 * ly][2 * lx + 1], I1[2 * ly + 1][2 * lx], I1[2 * ly + 1][2 * lx + 1]};
+      float va[4] = {V1[2 * ly][2 * lx], V1[2 * ly][2 * lx + 1], V1[2 * ly + 1][2 * lx], V1[2 * ly + 1][2 * lx + 1]};
+      fuse4(ia, va, oi, ov);
+      p.id[2][x + y * p.w[2]] = oi;
+      p.var[2][x + y * p.w[2]] = ov;
+    }
+    I2[ly][lx] = oi;
+    V2[ly][lx] = ov;
+  }
+  __syncthreads();
+  if (t < 16) {
+    int lx = t & 3, ly = t >> 2, x = bx * 4 + lx, y = by * 4 + ly;
+    float oi = -1.f, ov = -1.f;
+    if (x < p.w[3] && y < p.h[3]) {
+      float ia[4] = {I2[2 * ly][2 * lx], I2[2 * ly][2 * lx + 1], I2[2 * ly + 1][2 * lx], I2[2 * ly + 1][2 * lx + 1]};
+      float va[4] = {V2[2 * ly][2 * lx], V2[2 * ly][2 * lx + 1], V2[2 * ly + 1][2 * lx], V2[2 * ly + 1][2 * lx + 1]};
+      fuse4(ia, va, oi, ov);
+      p.id[3][x + y * p.w[3]] = oi;
+      p.var[3][x + y * p.w[3]] = ov;
+    }
+    I3[ly][lx] = oi;
+    V3[ly][lx] = ov;
+  }
+  __syncthreads();
+  if (t < 4) {
+    int lx = t & 1, ly = t >> 1, x = bx * 2 + lx, y = by * 2 + ly;
+    if (x < p.w[4] && y < p.h[4]) {
+      float ia[4] = {I3[2 * ly][2 * lx], I3[2 * ly][2 * lx + 1], I3[2 * ly + 1][2 * lx], I3[2 * ly + 1][2 * lx + 1]};
+      float va[4] = {V3[2 * ly][2 * lx], V3[2 * ly][2 * lx + 1], V3[2 * ly + 1][2 * lx], V3[2 * ly + 1][2 * lx + 1]};
+      float oi, ov;
+      fuse4(ia, va, oi, ov);
+      p.id[4][x + y * p.w[4]] = oi;
+      p.var[4][x + y * p.w[4]] = ov;
+    }
+  }
+}
+
+// Pixels without a usable depth become (-1, -1).
+__global__ void k_depth_normalize(float* __restrict__ id, float* __restrict__ var, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a = id[i], v = var[i];
+  bool ok = isfinite(a) && v > 0.f && a > 0.f;
+  if (!ok) {
+    id[i] = -1.f;
+    var[i] = -1.f;
+  }
+}
+
+// ---------------------------------------------------------- point cloud
+__device__ inline bool pc_valid(const float* id, const float* var, int idx, int w, int h) {
+  int y = idx / w, x = idx - y * w;
+  if (x < 1 || x >= w - 1 || y < 1 || y >= h - 1) return false;
+  return !(var[idx] <= 0.f || id[idx] == 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_pc_count(const float* __restrict__ id, const float* __restrict__ var,
+                                                  int w, int h, int* __restrict__ counts) {
+  int idx = blockIdx.x * 256 + threadIdx.x;
+  bool v = idx < w * h && pc_valid(id, var, idx, w, h);
+  int c = __syncthreads_count(v);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// Exclusive scan of n block counts in one 1024-thread block; total at counts[n].
+__global__ __launch_bounds__(1024) void k_pc_scan(int* counts, int n) {
+  __shared__ int sh[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < n ? counts[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int a = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (i < n) counts[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[n] = carry;
+}
+
+struct PcWriteArgs {
+  const float* id;
+  const float* var;
+  const float* img;
+  const float4* grad;
+  int w, h;
+  float fxi, fyi, cxi, cyi;
+  float4* posI;
+  float* pvar;
+  float2* pgrad;
+  int* pidx;
+};
+
+__global__ __launch_bounds__(256) void k_pc_write(PcWriteArgs a, const int* __restrict__ offsets) {
+  __shared__ int wave_count[4];
+  int idx = blockIdx.x * 256 + threadIdx.x;
+  bool v = idx < a.w * a.h && pc_valid(a.id, a.var, idx, a.w, a.h);
+  unsigned long long m = __ballot(v);
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int rank = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_count[wv] = __popcll(m);
+  __syncthreads();
+  int base = offsets[blockIdx.x];
+  for (int k = 0; k < wv; ++k) base += wave_count[k];
+  if (!v) return;
+  int o = base + rank;
+  int y = idx / a.w, x = idx - y * a.w;
+  float inv = 1.0f / a.id[idx];
+  float px = inv * (a.fxi * (float)x + a.cxi);
+  float py = inv * (a.fyi * (float)y + a.cyi);
+  float pz = inv * 1.0f;
+  a.posI[o] = make_float4(px, py, pz, a.img[idx]);
+  a.pvar[o] = a.var[idx];
+  float4 g = a.grad[idx];
+  a.pgrad[o] = make_float2(g.x, g.y);
+  a.pidx[o] = idx;
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s) {
+  PyrImgArgs p;
+  for (int l = 0; l < PYR; ++l) {
+    p.img[l] = f->d_image[l];
+    p.w[l] = f->w[l];
+    p.h[l] = f->h[l];
+  }
+  dim3 grid((f->w[0] + 31) / 32, (f->h[0] + 31) / 32);
+  hipLaunchKernelGGL(k_pyr_images, grid, dim3(256), 0, s, p, d_u8);
+  return hipGetLastError();
+}
+
+hipError_t launch_pyr_grads(lsdg_frame f, hipStream_t s) {
+  PyrGradArgs p;
+  int blk = 0;
+  for (int l = 0; l < PYR; ++l) {
+    p.img[l] = f->d_image[l];
+    p.grad[l] = f->d_grad[l];
+    p.gradh[l] = f->d_gradh[l];
+    p.maxg[l] = f->d_maxg[l];
+    p.w[l] = f->w[l];
+    p.h[l] = f->h[l];
+    p.blk0[l] = blk;
+    blk += (f->w[l] * f->h[l] + 255) / 256;
+  }
+  p.blk0[PYR] = blk;
+  p.prec = f->precision;
+  hipLaunchKernelGGL(k_pyr_grads, dim3(blk), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_pyr_depth(lsdg_frame f, hipStream_t s) {
+  PyrDepthArgs p;
+  for (int l = 0; l < PYR; ++l) {
+    p.id[l] = f->d_idepth[l];
+    p.var[l] = f->d_ivar[l];
+    p.w[l] = f->w[l];
+    p.h[l] = f->h[l];
+  }
+  dim3 grid((f->w[0] + 31) / 32, (f->h[0] + 31) / 32);
+  hipLaunchKernelGGL(k_pyr_depth, grid, dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_depth_normalize(float* id, float* var, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_depth_normalize, dim3((n + 255) / 256), dim3(256), 0, s, id, var, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_cloud(lsdg_ref r, int l, hipStream_t s) {
+  lsdg_frame f = r->kf;
+  int w = f->w[l], h = f->h[l];
+  int nb = (w * h + 255) / 256;
+  hipError_t e;
+  if (r->blockcounts_cap < nb + 1) {
+    if (r->d_blockcounts) (void)hipFree(r->d_blockcounts);
+    r->d_blockcounts = nullptr;
+    e = hipMalloc(&r->d_blockcounts, sizeof(int) * (nb + 1));
+    if (e != hipSuccess) return e;
+    r->blockcounts_cap = nb + 1;
+  }
+  hipLaunchKernelGGL(k_pc_count, dim3(nb), dim3(256), 0, s, f->d_idepth[l], f->d_ivar[l], w, h, r->d_blockcounts);
+  hipLaunchKernelGGL(k_pc_scan, dim3(1), dim3(1024), 0, s, r->d_blockcounts, nb);
+  int total = 0;
+  e = hipMemcpyAsync(&total, r->d_blockcounts + nb, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  if (total > r->cap[l]) {
+    if (r->d_pool[l]) (void)hipFree(r->d_pool[l]);
+    r->d_pool[l] = nullptr;
+    size_t bytes = (size_t)total * (sizeof(float4) + sizeof(float) + sizeof(float2) + sizeof(int)) + 256;
+    e = hipMalloc(&r->d_pool[l], bytes);
+    if (e != hipSuccess) return e;
+    r->cap[l] = total;
+  }
+  char* base = (char*)r->d_pool[l];
+  PointsLevel& pl = r->lv[l];
+  pl.posI = (const float4*)base;
+  pl.grad = (const float2*)(base + (size_t)total * sizeof(float4));
+  pl.var = (const float*)(base + (size_t)total * (sizeof(float4) + sizeof(float2)));
+  pl.idx = (const int*)(base + (size_t)total * (sizeof(float4) + sizeof(float2) + sizeof(float)));
+  pl.n = total;
+  r->num[l] = total;
+  if (total == 0) return hipSuccess;
+  PcWriteArgs a;
+  a.id = f->d_idepth[l];
+  a.var = f->d_ivar[l];
+  a.img = f->d_image[l];
+  a.grad = f->d_grad[l];
+  a.w = w;
+  a.h = h;
+  a.fxi = f->fxi[l];
+  a.fyi = f->fyi[l];
+  a.cxi = f->cxi[l];
+  a.cyi = f->cyi[l];
+  a.posI = (float4*)pl.posI;
+  a.pvar = (float*)pl.var;
+  a.pgrad = (float2*)pl.grad;
+  a.pidx = (int*)pl.idx;
+  hipLaunchKernelGGL(k_pc_write, dim3(nb), dim3(256), 0, s, a, r->d_blockcounts);
+  return hipGetLastError();
+}
+
+}  // namespace lsdg

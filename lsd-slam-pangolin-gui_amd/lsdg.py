@@ -1,0 +1,479 @@
+"""ctypes mirror of include/lsdg.h and a thin Python façade over liblsdg.so.
+
+Structure layouts here must match the C header field for field (a CPU test
+checks sizes against the library).  The façade raises if the HIP library is
+missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PYR = 5
+
+OK, DIVERGED, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, -1, -2, -3, -4
+FP32, FP16, BF16 = 0, 1, 2
+WHAT_IMAGE, WHAT_GRADIENTS, WHAT_MAXGRAD, WHAT_IDEPTH, WHAT_IDEPTH_VAR, WHAT_REF_PIXEL_WAS_GOOD = range(6)
+JOB_SE3, JOB_SIM3 = 0, 1
+
+OBS = dict(NONE=0, LOWGRAD_INVALIDATE=1, SKIP_ALREADY_GOOD=2, SKIP_NOT_GOOD_TRACKED=3, EPL_FAIL=4,
+           CREATE_VAR_TOO_BIG=5, CREATED=6, INCONSISTENT=7, UPDATED=8, STEREO_FAIL_BASE=10)
+
+
+class Camera(C.Structure):
+    _fields_ = [("w", C.c_int32), ("h", C.c_int32), ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float),
+                ("cy", C.c_float)]
+
+    @staticmethod
+    def of(cam):
+        return Camera(cam["w"], cam["h"], cam["fx"], cam["fy"], cam["cx"], cam["cy"])
+
+
+_PARAM_FIELDS = [
+    ("lambda_initial", C.c_float * PYR), ("lambda_success_fac", C.c_float * PYR),
+    ("lambda_fail_fac", C.c_float * PYR), ("step_size_min", C.c_float * PYR),
+    ("convergence_eps", C.c_float * PYR), ("max_its_per_lvl", C.c_int32 * PYR),
+    ("var_weight", C.c_float), ("huber_d", C.c_float), ("camera_pixel_noise2", C.c_float),
+    ("se3_min_level", C.c_int32), ("se3_max_level", C.c_int32),
+    ("min_goodperall_pixel", C.c_float), ("min_goodperall_pixel_absmin", C.c_float),
+    ("min_goodpergoodbad_pixel", C.c_float), ("max_diff_constant", C.c_float),
+    ("max_diff_grad_mult", C.c_float), ("use_affine_lightning_estimation", C.c_int32),
+    ("use_esm_tracking", C.c_int32),
+    ("min_abs_grad_create", C.c_float), ("min_abs_grad_decrease", C.c_float),
+    ("min_epl_length_squared", C.c_float), ("min_epl_grad_squared", C.c_float),
+    ("min_epl_angle_squared", C.c_float), ("gradient_sample_dist", C.c_float),
+    ("sample_point_to_border", C.c_float), ("max_epl_length_crop", C.c_float),
+    ("min_epl_length_crop", C.c_float), ("max_error_stereo", C.c_float),
+    ("min_distance_error_stereo", C.c_float), ("stereo_epl_var_fac", C.c_float),
+    ("min_depth", C.c_float), ("max_var", C.c_float), ("var_random_init_initial", C.c_float),
+    ("var_gt_init_initial", C.c_float), ("succ_var_inc_fac", C.c_float), ("fail_var_inc_fac", C.c_float),
+    ("diff_fac_observe", C.c_float), ("diff_fac_prop_merge", C.c_float), ("diff_fac_smoothing", C.c_float),
+    ("depth_smoothing_factor", C.c_float), ("division_eps", C.c_float),
+    ("validity_counter_inc", C.c_int32), ("validity_counter_dec", C.c_int32),
+    ("validity_counter_initial_observe", C.c_int32), ("validity_counter_max", C.c_int32),
+    ("validity_counter_max_variable", C.c_int32), ("val_sum_min_for_create", C.c_int32),
+    ("val_sum_min_for_keep", C.c_int32), ("val_sum_min_for_unblacklist", C.c_int32),
+    ("min_blacklist", C.c_int32), ("allow_negative_idepths", C.c_int32), ("use_subpixel_stereo", C.c_int32),
+    ("stereo_subpixel_oob_guard", C.c_int32), ("stereo_max_steps", C.c_int32),
+]
+
+
+class Params(C.Structure):
+    _fields_ = _PARAM_FIELDS
+
+    def as_dict(self):
+        d = {}
+        for name, _ in _PARAM_FIELDS:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+class TrackStats(C.Structure):
+    _fields_ = [("last_residual", C.c_float), ("last_depth_residual", C.c_float),
+                ("last_photometric_residual", C.c_float), ("last_mean_res", C.c_float),
+                ("point_usage", C.c_float), ("affine_a", C.c_float), ("affine_b", C.c_float),
+                ("last_good_count", C.c_int32), ("last_bad_count", C.c_int32), ("diverged", C.c_int32),
+                ("tracking_was_good", C.c_int32), ("num_residual_calls", C.c_int32 * PYR),
+                ("num_warp_update_calls", C.c_int32 * PYR), ("hessian", C.c_double * 49)]
+
+
+class Eval(C.Structure):
+    _fields_ = [("A", C.c_double * 49), ("b", C.c_double * 7), ("error", C.c_double), ("sum_res_p", C.c_double),
+                ("sum_res_d", C.c_double), ("num_terms_p", C.c_double), ("num_terms_d", C.c_double),
+                ("n_constraints", C.c_double), ("warped_count", C.c_int32), ("good_count", C.c_int32),
+                ("bad_count", C.c_int32), ("sum_res_unweighted", C.c_double), ("sum_signed_res", C.c_double),
+                ("usage", C.c_double), ("sxx", C.c_double), ("syy", C.c_double), ("sx", C.c_double),
+                ("sy", C.c_double), ("sw", C.c_double)]
+
+    def A7(self):
+        return np.array(self.A[:], np.float64).reshape(7, 7)
+
+
+class Hypotheses(C.Structure):
+    _fields_ = [("is_valid", C.c_void_p), ("blacklisted", C.c_void_p), ("next_stereo_frame_min_id", C.c_void_p),
+                ("validity_counter", C.c_void_p), ("idepth", C.c_void_p), ("idepth_var", C.c_void_p),
+                ("idepth_smoothed", C.c_void_p), ("idepth_var_smoothed", C.c_void_p),
+                ("observe_code", C.c_void_p)]
+
+
+HYP_FIELDS = [("is_valid", np.uint8), ("blacklisted", np.int32), ("next_stereo_frame_min_id", np.float32),
+              ("validity_counter", np.int32), ("idepth", np.float32), ("idepth_var", np.float32),
+              ("idepth_smoothed", np.float32), ("idepth_var_smoothed", np.float32), ("observe_code", np.uint8)]
+
+
+def hyp_arrays(n):
+    arrs = {k: np.zeros(n, dt) for k, dt in HYP_FIELDS}
+    h = Hypotheses(*[arrs[k].ctypes.data for k, _ in HYP_FIELDS])
+    return h, arrs
+
+
+def hyp_struct(arrs):
+    """Hypotheses struct pointing at (possibly partial) numpy arrays of the right dtypes."""
+    vals = []
+    for k, dt in HYP_FIELDS:
+        a = arrs.get(k)
+        if a is not None:
+            assert a.dtype == dt and a.flags.c_contiguous
+        vals.append(None if a is None else a.ctypes.data)
+    return Hypotheses(*vals)
+
+
+KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE = range(5)
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
+class Job(C.Structure):
+    _fields_ = [("ref", C.c_void_p), ("frame", C.c_void_p), ("frame_to_ref_init", C.c_double * 8),
+                ("kind", C.c_int32), ("start_level", C.c_int32), ("final_level", C.c_int32),
+                ("rebuild_pyramid", C.c_int32)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("frame_to_ref", C.c_double * 8), ("stats", TrackStats), ("status", C.c_int32)]
+
+
+def pose7(p):
+    return (C.c_double * 7)(*[float(v) for v in list(p)[:7]])
+
+
+def pose8(p):
+    p = list(p)
+    if len(p) == 7:
+        p = p + [1.0]
+    return (C.c_double * 8)(*[float(v) for v in p])
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data
+
+
+_LIB = None
+
+
+def lib_path():
+    return os.path.join(HERE, "liblsdg.so")
+
+
+def load():
+    """Loads liblsdg.so (the HIP build).  Raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library {path} is not built; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp, i32, f32, dp, fp = C.c_void_p, C.c_int32, C.c_float, C.POINTER(C.c_double), C.c_void_p
+    sig = {
+        "lsdg_params_default": (None, [C.POINTER(Params)]),
+        "lsdg_ctx_create": (i32, [i32, C.POINTER(Params), C.POINTER(vp)]),
+        "lsdg_ctx_destroy": (i32, [vp]),
+        "lsdg_ctx_set_params": (i32, [vp, C.POINTER(Params)]),
+        "lsdg_ctx_synchronize": (i32, [vp]),
+        "lsdg_ctx_last_error": (C.c_char_p, [vp]),
+        "lsdg_frame_create": (i32, [vp, i32, C.POINTER(Camera), fp, i32, C.POINTER(vp)]),
+        "lsdg_frame_create_u8": (i32, [vp, i32, C.POINTER(Camera), fp, i32, i32, C.POINTER(vp)]),
+        "lsdg_frame_destroy": (i32, [vp]),
+        "lsdg_frame_build_pyramid": (i32, [vp]),
+        "lsdg_frame_set_depth": (i32, [vp, fp, fp]),
+        "lsdg_frame_read": (i32, [vp, i32, i32, fp, C.c_size_t]),
+        "lsdg_frame_set_tracking": (i32, [vp, vp, dp, f32, fp]),
+        "lsdg_frame_get_tracking": (i32, [vp, C.POINTER(i32), dp, C.POINTER(f32)]),
+        "lsdg_frame_set_cam_to_world": (i32, [vp, dp]),
+        "lsdg_frame_counters": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
+        "lsdg_frame_set_counters": (i32, [vp, i32, i32]),
+        "lsdg_ref_create": (i32, [vp, vp, C.POINTER(vp)]),
+        "lsdg_ref_destroy": (i32, [vp]),
+        "lsdg_ref_make_point_cloud": (i32, [vp, i32]),
+        "lsdg_ref_num_points": (i32, [vp, i32, C.POINTER(i32)]),
+        "lsdg_ref_read_points": (i32, [vp, i32, fp, i32]),
+        "lsdg_se3_track": (i32, [vp, vp, vp, dp, dp, C.POINTER(TrackStats)]),
+        "lsdg_sim3_track": (i32, [vp, vp, vp, dp, i32, i32, dp, C.POINTER(TrackStats)]),
+        "lsdg_se3_evaluate": (i32, [vp, vp, vp, i32, dp, f32, f32, C.POINTER(Eval), fp]),
+        "lsdg_sim3_evaluate": (i32, [vp, vp, vp, i32, dp, C.POINTER(Eval)]),
+        "lsdg_energy_grad": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
+        "lsdg_depthmap_create": (i32, [vp, C.POINTER(Camera), C.POINTER(vp)]),
+        "lsdg_depthmap_destroy": (i32, [vp]),
+        "lsdg_depth_reset": (i32, [vp]),
+        "lsdg_depth_is_valid": (i32, [vp]),
+        "lsdg_depth_initialize_from_gt": (i32, [vp, vp]),
+        "lsdg_depth_set_prior": (i32, [vp, vp, fp, fp, i32]),
+        "lsdg_depth_update": (i32, [vp, C.POINTER(vp), i32]),
+        "lsdg_depth_create_keyframe": (i32, [vp, vp]),
+        "lsdg_depth_finalize_keyframe": (i32, [vp]),
+        "lsdg_depth_read": (i32, [vp, C.POINTER(Hypotheses)]),
+        "lsdg_depth_write": (i32, [vp, C.POINTER(Hypotheses)]),
+        "lsdg_depth_active_keyframe": (i32, [vp, C.POINTER(i32)]),
+        "lsdg_ctx_profile": (i32, [vp, i32]),
+        "lsdg_ctx_kernel_stats": (i32, [vp, i32, C.POINTER(KernelStats)]),
+        "lsdg_batch_create": (i32, [vp, C.POINTER(Job), i32, C.POINTER(vp)]),
+        "lsdg_batch_run": (i32, [vp, C.POINTER(JobResult)]),
+        "lsdg_batch_destroy": (i32, [vp]),
+        "lsdg_batch_track": (i32, [vp, C.POINTER(Job), i32, C.POINTER(JobResult)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def default_params():
+    p = Params()
+    load().lsdg_params_default(C.byref(p))
+    return p
+
+
+class LsdgError(RuntimeError):
+    pass
+
+
+class Context:
+    """One lsdg_ctx (device + HIP stream + parameters)."""
+
+    def __init__(self, device=0, params=None):
+        self.L = load()
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        self._check(self.L.lsdg_ctx_create(device, C.byref(self.params), C.byref(h)), "lsdg_ctx_create", None)
+        self.h = h
+
+    def _check(self, st, what, ctx=None):
+        if st < 0:
+            msg = self.L.lsdg_ctx_last_error(ctx) if ctx is not None else b""
+            raise LsdgError(f"{what} failed with status {st}: {msg.decode() if msg else ''}")
+        return st
+
+    def check(self, st, what):
+        return self._check(st, what, self.h)
+
+    def set_params(self, params):
+        self.params = params
+        self.check(self.L.lsdg_ctx_set_params(self.h, C.byref(params)), "lsdg_ctx_set_params")
+
+    def sync(self):
+        self.check(self.L.lsdg_ctx_synchronize(self.h), "lsdg_ctx_synchronize")
+
+    def profile(self, enable):
+        self.check(self.L.lsdg_ctx_profile(self.h, int(enable)), "lsdg_ctx_profile")
+
+    def kernel_stats(self, kind):
+        st = KernelStats()
+        self.check(self.L.lsdg_ctx_kernel_stats(self.h, kind, C.byref(st)), "lsdg_ctx_kernel_stats")
+        return dict(launches=st.launches, ms=st.ms, bytes=st.bytes)
+
+    def close(self):
+        if self.h:
+            self.L.lsdg_ctx_destroy(self.h)
+            self.h = None
+
+    def frame(self, fid, cam, image, precision=FP32):
+        h = C.c_void_p()
+        cc = Camera.of(cam)
+        if image.dtype == np.uint8:
+            img = np.ascontiguousarray(image)
+            self.check(self.L.lsdg_frame_create_u8(self.h, fid, C.byref(cc), img.ctypes.data, cam["w"], precision,
+                                                   C.byref(h)), "lsdg_frame_create_u8")
+        else:
+            img, ptr = _f32(image)
+            self.check(self.L.lsdg_frame_create(self.h, fid, C.byref(cc), ptr, precision, C.byref(h)),
+                       "lsdg_frame_create")
+        return Frame(self, h, fid, cam)
+
+    def ref(self, keyframe):
+        h = C.c_void_p()
+        self.check(self.L.lsdg_ref_create(self.h, keyframe.h, C.byref(h)), "lsdg_ref_create")
+        return Ref(self, h, keyframe)
+
+    def se3_track(self, ref, frame, init7):
+        out = (C.c_double * 7)()
+        st = TrackStats()
+        status = self.check(self.L.lsdg_se3_track(self.h, ref.h, frame.h, pose7(init7), out, C.byref(st)),
+                            "lsdg_se3_track")
+        return np.array(out[:]), st, status
+
+    def sim3_track(self, ref, frame, init8, start_level, final_level):
+        out = (C.c_double * 8)()
+        st = TrackStats()
+        status = self.check(self.L.lsdg_sim3_track(self.h, ref.h, frame.h, pose8(init8), start_level, final_level,
+                                                   out, C.byref(st)), "lsdg_sim3_track")
+        return np.array(out[:]), st, status
+
+    def se3_evaluate(self, ref, frame, level, pose, a=1.0, b=0.0, want_mask=False):
+        ev = Eval()
+        mask = None
+        if want_mask:
+            mask = np.full((frame.cam["h"] >> 1) * (frame.cam["w"] >> 1), 1, np.uint8)
+        self.check(self.L.lsdg_se3_evaluate(self.h, ref.h, frame.h, level, pose7(pose), a, b, C.byref(ev),
+                                            None if mask is None else mask.ctypes.data), "lsdg_se3_evaluate")
+        return ev, mask
+
+    def sim3_evaluate(self, ref, frame, level, pose8_):
+        ev = Eval()
+        self.check(self.L.lsdg_sim3_evaluate(self.h, ref.h, frame.h, level, pose8(pose8_), C.byref(ev)),
+                   "lsdg_sim3_evaluate")
+        return ev
+
+    def energy_grad(self, ref, frame, level, pose):
+        n = ref.num_points(level)
+        e = C.c_double()
+        g = (C.c_double * 6)()
+        gd = np.zeros(n, np.float32)
+        self.check(self.L.lsdg_energy_grad(self.h, ref.h, frame.h, level, pose7(pose), C.byref(e), g,
+                                           gd.ctypes.data, n), "lsdg_energy_grad")
+        return e.value, np.array(g[:]), gd
+
+    def depthmap(self, cam):
+        h = C.c_void_p()
+        cc = Camera.of(cam)
+        self.check(self.L.lsdg_depthmap_create(self.h, C.byref(cc), C.byref(h)), "lsdg_depthmap_create")
+        return DepthMapH(self, h, cam)
+
+    def batch(self, jobs):
+        arr = (Job * len(jobs))(*jobs)
+        h = C.c_void_p()
+        self.check(self.L.lsdg_batch_create(self.h, arr, len(jobs), C.byref(h)), "lsdg_batch_create")
+        return Batch(self, h, len(jobs))
+
+
+class Frame:
+    def __init__(self, ctx, h, fid, cam):
+        self.ctx, self.h, self.id, self.cam = ctx, h, fid, cam
+
+    def size(self, level):
+        return self.cam["w"] >> level, self.cam["h"] >> level
+
+    def read(self, what, level=0):
+        w, h = self.size(level)
+        if what == WHAT_GRADIENTS:
+            out = np.zeros((h, w, 4), np.float32)
+        elif what == WHAT_REF_PIXEL_WAS_GOOD:
+            w, h = self.size(1)
+            out = np.zeros((h, w), np.uint8)
+        else:
+            out = np.zeros((h, w), np.float32)
+        self.ctx.check(self.ctx.L.lsdg_frame_read(self.h, what, level, out.ctypes.data, out.nbytes),
+                       "lsdg_frame_read")
+        return out
+
+    def build_pyramid(self):
+        self.ctx.check(self.ctx.L.lsdg_frame_build_pyramid(self.h), "lsdg_frame_build_pyramid")
+
+    def set_depth(self, idepth, var):
+        a, pa = _f32(idepth)
+        b, pb = _f32(var)
+        self.ctx.check(self.ctx.L.lsdg_frame_set_depth(self.h, pa, pb), "lsdg_frame_set_depth")
+
+    def set_tracking(self, parent, this_to_parent8, initial_tracked_residual=0.0, was_good=None):
+        wg = None
+        if was_good is not None:
+            wg = np.ascontiguousarray(was_good, np.uint8)
+        self.ctx.check(self.ctx.L.lsdg_frame_set_tracking(self.h, None if parent is None else parent.h,
+                                                          pose8(this_to_parent8), initial_tracked_residual,
+                                                          None if wg is None else wg.ctypes.data),
+                       "lsdg_frame_set_tracking")
+
+    def get_tracking(self):
+        pid = C.c_int32()
+        tp = (C.c_double * 8)()
+        itr = C.c_float()
+        self.ctx.check(self.ctx.L.lsdg_frame_get_tracking(self.h, C.byref(pid), tp, C.byref(itr)),
+                       "lsdg_frame_get_tracking")
+        return pid.value, np.array(tp[:]), itr.value
+
+    def set_cam_to_world(self, c2w8):
+        self.ctx.check(self.ctx.L.lsdg_frame_set_cam_to_world(self.h, pose8(c2w8)), "lsdg_frame_set_cam_to_world")
+
+    def counters(self):
+        a, b = C.c_int32(), C.c_int32()
+        self.ctx.check(self.ctx.L.lsdg_frame_counters(self.h, C.byref(a), C.byref(b)), "lsdg_frame_counters")
+        return a.value, b.value
+
+    def set_counters(self, tracked, mapped):
+        self.ctx.check(self.ctx.L.lsdg_frame_set_counters(self.h, tracked, mapped), "lsdg_frame_set_counters")
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.lsdg_frame_destroy(self.h)
+            self.h = None
+
+
+class Ref:
+    def __init__(self, ctx, h, kf):
+        self.ctx, self.h, self.kf = ctx, h, kf
+
+    def num_points(self, level):
+        n = C.c_int32()
+        self.ctx.check(self.ctx.L.lsdg_ref_num_points(self.h, level, C.byref(n)), "lsdg_ref_num_points")
+        return n.value
+
+    def points(self, level):
+        n = self.num_points(level)
+        out = np.zeros((n, 8), np.float32)
+        self.ctx.check(self.ctx.L.lsdg_ref_read_points(self.h, level, out.ctypes.data, n), "lsdg_ref_read_points")
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.lsdg_ref_destroy(self.h)
+            self.h = None
+
+
+class DepthMapH:
+    def __init__(self, ctx, h, cam):
+        self.ctx, self.h, self.cam = ctx, h, cam
+
+    def set_prior(self, kf, idepth, var, validity):
+        a, pa = _f32(idepth)
+        b, pb = _f32(var)
+        self.ctx.check(self.ctx.L.lsdg_depth_set_prior(self.h, kf.h, pa, pb, validity), "lsdg_depth_set_prior")
+
+    def init_gt(self, kf):
+        self.ctx.check(self.ctx.L.lsdg_depth_initialize_from_gt(self.h, kf.h), "lsdg_depth_initialize_from_gt")
+
+    def update(self, refs):
+        arr = (C.c_void_p * len(refs))(*[r.h for r in refs])
+        self.ctx.check(self.ctx.L.lsdg_depth_update(self.h, arr, len(refs)), "lsdg_depth_update")
+
+    def create_keyframe(self, nkf):
+        self.ctx.check(self.ctx.L.lsdg_depth_create_keyframe(self.h, nkf.h), "lsdg_depth_create_keyframe")
+
+    def finalize(self):
+        self.ctx.check(self.ctx.L.lsdg_depth_finalize_keyframe(self.h), "lsdg_depth_finalize_keyframe")
+
+    def read(self):
+        n = self.cam["w"] * self.cam["h"]
+        h, arrs = hyp_arrays(n)
+        self.ctx.check(self.ctx.L.lsdg_depth_read(self.h, C.byref(h)), "lsdg_depth_read")
+        return arrs
+
+    def write(self, arrs):
+        h = hyp_struct(arrs)
+        self.ctx.check(self.ctx.L.lsdg_depth_write(self.h, C.byref(h)), "lsdg_depth_write")
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.lsdg_depthmap_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    def __init__(self, ctx, h, n):
+        self.ctx, self.h, self.n = ctx, h, n
+
+    def run(self):
+        res = (JobResult * self.n)()
+        self.ctx.check(self.ctx.L.lsdg_batch_run(self.h, res), "lsdg_batch_run")
+        return res
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.lsdg_batch_destroy(self.h)
+            self.h = None

@@ -1,0 +1,320 @@
+"""TEST INFRASTRUCTURE (the oracle): ctypes façade over oracle/liboracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker or the timed CPU baseline.  The API mirrors the
+product façade (lsdg.py) so parity tests run one sequence on both.
+prec 0 = FP32 restated reference path, prec 1 = FP64 oracle.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "lsd-slam-pangolin-gui_amd"))
+import lsdg  # noqa: E402  (struct layouts only)
+
+_LIBS = {}
+
+
+def load(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run `make -C oracle`")
+    L = C.CDLL(path)
+    vp, i32, f32, dp, fp = C.c_void_p, C.c_int, C.c_float, C.POINTER(C.c_double), C.c_void_p
+    P = C.POINTER(lsdg.Params)
+    sig = {
+        "orc_params_default": (None, [P]),
+        "orc_frame_new": (vp, [i32, i32, i32, i32, f32, f32, f32, f32, fp, P]),
+        "orc_frame_free": (None, [vp]),
+        "orc_frame_read_f32": (i32, [vp, i32, i32, fp]),
+        "orc_frame_read_f64": (i32, [vp, i32, i32, fp]),
+        "orc_frame_num_mappable": (None, [vp, C.POINTER(f32)]),
+        "orc_frame_set_depth": (None, [vp, fp, fp]),
+        "orc_frame_set_tracking": (None, [vp, vp, dp, f32, fp]),
+        "orc_frame_get_tracking": (None, [vp, C.POINTER(i32), dp, C.POINTER(f32)]),
+        "orc_frame_set_cam_to_world": (None, [vp, dp]),
+        "orc_frame_counters": (None, [vp, C.POINTER(i32), C.POINTER(i32), i32]),
+        "orc_ref_new": (vp, [vp]),
+        "orc_ref_free": (None, [vp]),
+        "orc_ref_num_points": (i32, [vp, i32]),
+        "orc_ref_read_points": (None, [vp, i32, fp]),
+        "orc_se3_track": (i32, [vp, vp, P, dp, dp, C.POINTER(lsdg.TrackStats)]),
+        "orc_se3_evaluate": (i32, [vp, vp, P, i32, dp, f32, f32, C.POINTER(lsdg.Eval), fp]),
+        "orc_sim3_track": (i32, [vp, vp, P, dp, i32, i32, dp, C.POINTER(lsdg.TrackStats)]),
+        "orc_sim3_evaluate": (i32, [vp, vp, P, i32, dp, C.POINTER(lsdg.Eval)]),
+        "orc_energy_grad": (i32, [vp, vp, P, i32, dp, dp, dp, fp]),
+        "orc_energy_frozen": (i32, [vp, vp, P, i32, dp, dp, fp, fp, fp, i32, dp]),
+        "orc_depth_new": (vp, [i32, i32, i32, f32, f32, f32, f32, P, i32]),
+        "orc_depth_free": (None, [vp]),
+        "orc_depth_set_threads": (None, [vp, i32]),
+        "orc_depth_set_prior": (None, [vp, vp, fp, fp, i32]),
+        "orc_depth_init_gt": (None, [vp, vp]),
+        "orc_depth_update": (None, [vp, C.POINTER(vp), i32]),
+        "orc_depth_create_keyframe": (None, [vp, vp]),
+        "orc_depth_finalize": (None, [vp]),
+        "orc_depth_step": (None, [vp, i32, i32]),
+        "orc_depth_read": (None, [vp, C.POINTER(lsdg.Hypotheses)]),
+        "orc_depth_read_f64": (None, [vp, fp, fp, fp, fp]),
+        "orc_depth_write": (None, [vp, C.POINTER(lsdg.Hypotheses)]),
+        "orc_se3_exp": (None, [dp, dp]),
+        "orc_sim3_exp": (None, [dp, dp]),
+        "orc_sim3_mul": (None, [dp, dp, dp]),
+        "orc_sim3_inv": (None, [dp, dp]),
+        "orc_ldlt6": (None, [dp, dp, dp]),
+        "orc_ldlt7": (None, [dp, dp, dp]),
+        "orc_interp_f32": (f32, [fp, f32, f32, i32]),
+    }
+    for n, (r, a) in sig.items():
+        fn = getattr(L, n)
+        fn.restype = r
+        fn.argtypes = a
+    _LIBS[name] = L
+    return L
+
+
+def default_params():
+    p = lsdg.Params()
+    load().orc_params_default(C.byref(p))
+    return p
+
+
+def _dp(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    def __init__(self, prec=0, params=None, fast=False):
+        self.L = load(fast)
+        self.prec = prec
+        self.params = params if params is not None else default_params()
+
+    def frame(self, fid, cam, image):
+        img = np.ascontiguousarray(image, np.float32)
+        h = self.L.orc_frame_new(self.prec, fid, cam["w"], cam["h"], cam["fx"], cam["fy"], cam["cx"], cam["cy"],
+                                 img.ctypes.data, C.byref(self.params))
+        return OFrame(self, h, fid, cam)
+
+    def ref(self, kf):
+        return ORef(self, self.L.orc_ref_new(kf.h), kf)
+
+    def se3_track(self, ref, frame, init7):
+        out = (C.c_double * 7)()
+        st = lsdg.TrackStats()
+        status = self.L.orc_se3_track(ref.h, frame.h, C.byref(self.params), lsdg.pose7(init7), out, C.byref(st))
+        return np.array(out[:]), st, status
+
+    def sim3_track(self, ref, frame, init8, start_level, final_level):
+        out = (C.c_double * 8)()
+        st = lsdg.TrackStats()
+        status = self.L.orc_sim3_track(ref.h, frame.h, C.byref(self.params), lsdg.pose8(init8), start_level,
+                                       final_level, out, C.byref(st))
+        return np.array(out[:]), st, status
+
+    def se3_evaluate(self, ref, frame, level, pose, a=1.0, b=0.0, want_mask=False):
+        ev = lsdg.Eval()
+        mask = None
+        if want_mask:
+            mask = np.full((frame.cam["h"] >> 1) * (frame.cam["w"] >> 1), 1, np.uint8)
+        self.L.orc_se3_evaluate(ref.h, frame.h, C.byref(self.params), level, lsdg.pose7(pose), a, b, C.byref(ev),
+                                None if mask is None else mask.ctypes.data)
+        return ev, mask
+
+    def sim3_evaluate(self, ref, frame, level, pose8_):
+        ev = lsdg.Eval()
+        self.L.orc_sim3_evaluate(ref.h, frame.h, C.byref(self.params), level, lsdg.pose8(pose8_), C.byref(ev))
+        return ev
+
+    def energy_grad(self, ref, frame, level, pose):
+        n = ref.num_points(level)
+        e = C.c_double()
+        g = (C.c_double * 6)()
+        gd = np.zeros(n, np.float64)
+        self.L.orc_energy_grad(ref.h, frame.h, C.byref(self.params), level, lsdg.pose7(pose), C.byref(e), g,
+                               gd.ctypes.data)
+        return e.value, np.array(g[:]), gd
+
+    def depthmap(self, cam, threads=1):
+        h = self.L.orc_depth_new(self.prec, cam["w"], cam["h"], cam["fx"], cam["fy"], cam["cx"], cam["cy"],
+                                 C.byref(self.params), threads)
+        return ODepth(self, h, cam)
+
+
+class OFrame:
+    def __init__(self, o, h, fid, cam):
+        self.o, self.h, self.id, self.cam = o, h, fid, cam
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.o.L.orc_frame_free(self.h)
+        except Exception:
+            pass
+
+    def size(self, level):
+        return self.cam["w"] >> level, self.cam["h"] >> level
+
+    def read(self, what, level=0, f64=False):
+        w, h = self.size(level)
+        dt = np.float64 if f64 else np.float32
+        if what == lsdg.WHAT_GRADIENTS:
+            out = np.zeros((h, w, 4), dt)
+        elif what == lsdg.WHAT_REF_PIXEL_WAS_GOOD:
+            w, h = self.size(1)
+            out = np.zeros((h, w), dt)
+        else:
+            out = np.zeros((h, w), dt)
+        fn = self.o.L.orc_frame_read_f64 if f64 else self.o.L.orc_frame_read_f32
+        fn(self.h, what, level, out.ctypes.data)
+        if what == lsdg.WHAT_REF_PIXEL_WAS_GOOD:
+            return out.astype(np.uint8)
+        return out
+
+    def num_mappable(self):
+        v = C.c_float()
+        self.o.L.orc_frame_num_mappable(self.h, C.byref(v))
+        return v.value
+
+    def set_depth(self, idepth, var):
+        a = np.ascontiguousarray(idepth, np.float32)
+        b = np.ascontiguousarray(var, np.float32)
+        self.o.L.orc_frame_set_depth(self.h, a.ctypes.data, b.ctypes.data)
+
+    def set_tracking(self, parent, this_to_parent8, initial_tracked_residual=0.0, was_good=None):
+        wg = None if was_good is None else np.ascontiguousarray(was_good, np.uint8)
+        self.o.L.orc_frame_set_tracking(self.h, None if parent is None else parent.h, lsdg.pose8(this_to_parent8),
+                                        initial_tracked_residual, None if wg is None else wg.ctypes.data)
+
+    def get_tracking(self):
+        pid = C.c_int()
+        tp = (C.c_double * 8)()
+        itr = C.c_float()
+        self.o.L.orc_frame_get_tracking(self.h, C.byref(pid), tp, C.byref(itr))
+        return pid.value, np.array(tp[:]), itr.value
+
+    def set_cam_to_world(self, c2w8):
+        self.o.L.orc_frame_set_cam_to_world(self.h, lsdg.pose8(c2w8))
+
+    def counters(self):
+        a, b = C.c_int(), C.c_int()
+        self.o.L.orc_frame_counters(self.h, C.byref(a), C.byref(b), 0)
+        return a.value, b.value
+
+    def set_counters(self, tracked, mapped):
+        a, b = C.c_int(tracked), C.c_int(mapped)
+        self.o.L.orc_frame_counters(self.h, C.byref(a), C.byref(b), 1)
+
+
+class ORef:
+    def __init__(self, o, h, kf):
+        self.o, self.h, self.kf = o, h, kf
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.o.L.orc_ref_free(self.h)
+        except Exception:
+            pass
+
+    def num_points(self, level):
+        return self.o.L.orc_ref_num_points(self.h, level)
+
+    def points(self, level):
+        n = self.num_points(level)
+        out = np.zeros((n, 8), np.float32)
+        self.o.L.orc_ref_read_points(self.h, level, out.ctypes.data)
+        return out
+
+
+class ODepth:
+    def __init__(self, o, h, cam):
+        self.o, self.h, self.cam = o, h, cam
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.o.L.orc_depth_free(self.h)
+        except Exception:
+            pass
+
+    def set_threads(self, n):
+        self.o.L.orc_depth_set_threads(self.h, n)
+
+    def set_prior(self, kf, idepth, var, validity):
+        a = np.ascontiguousarray(idepth, np.float32)
+        b = np.ascontiguousarray(var, np.float32)
+        self.o.L.orc_depth_set_prior(self.h, kf.h, a.ctypes.data, b.ctypes.data, validity)
+
+    def init_gt(self, kf):
+        self.o.L.orc_depth_init_gt(self.h, kf.h)
+
+    def update(self, refs):
+        arr = (C.c_void_p * len(refs))(*[r.h for r in refs])
+        self.o.L.orc_depth_update(self.h, arr, len(refs))
+
+    def create_keyframe(self, nkf):
+        self.o.L.orc_depth_create_keyframe(self.h, nkf.h)
+
+    def finalize(self):
+        self.o.L.orc_depth_finalize(self.h)
+
+    def step(self, step, arg=0):
+        self.o.L.orc_depth_step(self.h, step, arg)
+
+    def read(self):
+        n = self.cam["w"] * self.cam["h"]
+        h, arrs = lsdg.hyp_arrays(n)
+        self.o.L.orc_depth_read(self.h, C.byref(h))
+        return arrs
+
+    def read_f64(self):
+        n = self.cam["w"] * self.cam["h"]
+        out = [np.zeros(n, np.float64) for _ in range(4)]
+        self.o.L.orc_depth_read_f64(self.h, *[a.ctypes.data for a in out])
+        return dict(zip(["idepth", "idepth_var", "idepth_smoothed", "idepth_var_smoothed"], out))
+
+    def write(self, arrs):
+        h = lsdg.hyp_struct(arrs)
+        self.o.L.orc_depth_write(self.h, C.byref(h))
+
+
+# closed forms, for the oracle's own pinning tests
+def se3_exp(xi):
+    out = (C.c_double * 7)()
+    a, p = _dp(xi)
+    load().orc_se3_exp(p, out)
+    return np.array(out[:])
+
+
+def sim3_exp(xi):
+    out = (C.c_double * 8)()
+    a, p = _dp(xi)
+    load().orc_sim3_exp(p, out)
+    return np.array(out[:])
+
+
+def sim3_mul(a8, b8):
+    out = (C.c_double * 8)()
+    load().orc_sim3_mul(lsdg.pose8(a8), lsdg.pose8(b8), out)
+    return np.array(out[:])
+
+
+def sim3_inv(a8):
+    out = (C.c_double * 8)()
+    load().orc_sim3_inv(lsdg.pose8(a8), out)
+    return np.array(out[:])
+
+
+def ldlt(A, b):
+    n = len(b)
+    x = (C.c_double * n)()
+    Aa, pA = _dp(A)
+    ba, pb = _dp(b)
+    (load().orc_ldlt6 if n == 6 else load().orc_ldlt7)(pA, pb, x)
+    return np.array(x[:])
