@@ -81,6 +81,10 @@ static bool params_ok(const lsdg_params* p) {
   for (int l = 0; l < PYR; ++l)
     if (p->max_its_per_lvl[l] < 0) return false;
   if (p->stereo_max_steps < 1 || p->stereo_max_steps > 1 << 20) return false;
+  // epipolar samples reach 2 * gradient_sample_dist past the search interval,
+  // which starts sample_point_to_border pixels inside the image
+  if (!(p->gradient_sample_dist > 0.f) || !(2.f * p->gradient_sample_dist + 1.f < p->sample_point_to_border)) return false;
+  if (!(p->max_epl_length_crop > 0.f) || !(p->min_depth > 0.f)) return false;
   return true;
 }
 

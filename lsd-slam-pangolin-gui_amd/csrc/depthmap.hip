@@ -1,0 +1,934 @@
+// DepthMap (semi-dense inverse-depth filter) on gfx950, one thread per pixel.
+//
+//   k_observe      DepthMap::observeDepthRow -> observeDepthCreate / Update ->
+//                  makeAndCheckEPL -> doLineStereo, in place (each pixel only
+//                  touches its own hypothesis); writes a per-pixel outcome code.
+//   k_fill_holes   regularizeDepthMapFillHoles; the reference's integral image
+//                  of validity is replaced by the equal integer 5x5 box sum.
+//   k_regularize   regularizeDepthMap(removeOcclusions, validityTH), 5x5 gather.
+//   k_set_depth    Frame::setDepth (level 0; the pyramid follows).
+//   k_prop_*       propagateDepth as a deterministic scatter: project every
+//                  source, count per destination, scan, bucket, then one
+//                  thread per destination folds its sources in raster order,
+//                  which is the order the reference's sequential loop uses.
+//   k_mean_idepth  createKeyFrame's mean inverse depth (FP64, deterministic).
+//
+// Buffers are double-buffered like the reference's current/other maps: a pass
+// reads `in` and writes every pixel of `out` (copy-through for untouched
+// pixels), then the roles swap, so both buffers end up holding exactly what
+// the reference's two buffers hold.  Every per-pixel float expression keeps
+// the CPU restatement's operation order (compiled without FP contraction), so
+// masks, codes and values match it bit for bit.
+#include "lsdg_internal.h"
+
+namespace lsdg {
+
+__device__ inline float dinterp(const float* __restrict__ m, float x, float y, int w) {
+  int ix = (int)x, iy = (int)y;
+  float dx = x - (float)ix, dy = y - (float)iy;
+  float dxdy = dx * dy;
+  const float* bp = m + ix + iy * w;
+  return dxdy * bp[1 + w] + (dy - dxdy) * bp[w] + (dx - dxdy) * bp[1] + (1.0f - dx - dy + dxdy) * bp[0];
+}
+
+// getInterpolatedElement42 on (gx, gy, I, 0) gradients.
+__device__ inline float2 dinterp42(const float4* __restrict__ g, float x, float y, int w) {
+  int ix = (int)x, iy = (int)y;
+  float dx = x - (float)ix, dy = y - (float)iy;
+  float dxdy = dx * dy;
+  const float4* bp = g + ix + iy * w;
+  float4 a = bp[1 + w], b = bp[w], c = bp[1], d = bp[0];
+  float w11 = dxdy, w01 = dy - dxdy, w10 = dx - dxdy, w00 = 1.0f - dx - dy + dxdy;
+  return make_float2(w11 * a.x + w01 * b.x + w10 * c.x + w00 * d.x, w11 * a.y + w01 * b.y + w10 * c.y + w00 * d.y);
+}
+
+__device__ inline float unzero(float v) {
+  return v < 0.0f ? (v > -1e-10f ? -1e-10f : v) : (v < 1e-10f ? 1e-10f : v);
+}
+
+struct Hyp {
+  bool valid;
+  int bl;
+  float nsid;
+  int val;
+  float4 dv;
+};
+
+__device__ inline Hyp load_hyp(const HypBuf& b, int i) {
+  Hyp h;
+  h.valid = b.valid[i] != 0;
+  h.bl = b.bl[i];
+  h.nsid = b.nsid[i];
+  h.val = b.val[i];
+  h.dv = b.dv[i];
+  return h;
+}
+__device__ inline void store_hyp(const HypBuf& b, int i, const Hyp& h) {
+  b.valid[i] = h.valid ? 1 : 0;
+  b.bl[i] = h.bl;
+  b.nsid[i] = h.nsid;
+  b.val[i] = h.val;
+  b.dv[i] = h.dv;
+}
+// DepthMapPixelHypothesis(idepth, var, validity): smoothed values -1.
+__device__ inline Hyp make_hyp(float id, float var, int val) {
+  Hyp h;
+  h.valid = true;
+  h.bl = 0;
+  h.nsid = 0.0f;
+  h.val = val;
+  h.dv = make_float4(id, var, -1.0f, -1.0f);
+  return h;
+}
+
+// ------------------------------------------------------------ stereo
+struct ObsCtx {
+  DepthGeom g;
+  const float* kfImg;
+  const float4* kfGrad;
+};
+
+__device__ inline bool make_and_check_epl(const ObsCtx& o, const lsdg_params& P, int x, int y, const StereoRef& ref,
+                                          float* pepx, float* pepy) {
+  int idx = x + y * o.g.w;
+  float epx = -o.g.fx * ref.t_t2o[0] + ref.t_t2o[2] * ((float)x - o.g.cx);
+  float epy = -o.g.fy * ref.t_t2o[1] + ref.t_t2o[2] * ((float)y - o.g.cy);
+  if (isnan(epx + epy)) return false;
+  float eplLengthSquared = epx * epx + epy * epy;
+  if (eplLengthSquared < P.min_epl_length_squared) return false;
+  const float* I = o.kfImg;
+  float gx = I[idx + 1] - I[idx - 1];
+  float gy = I[idx + o.g.w] - I[idx - o.g.w];
+  float eplGradSquared = gx * epx + gy * epy;
+  eplGradSquared = eplGradSquared * eplGradSquared / eplLengthSquared;
+  if (eplGradSquared < P.min_epl_grad_squared) return false;
+  if (eplGradSquared / (gx * gx + gy * gy) < P.min_epl_angle_squared) return false;
+  float fac = P.gradient_sample_dist / sqrtf(eplLengthSquared);
+  *pepx = epx * fac;
+  *pepy = epy * fac;
+  return true;
+}
+
+// doLineStereo: the matching error (>= 0) or a code -1..-4.
+__device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, float v, float epxn, float epyn,
+                             float min_idepth, float prior_idepth, float max_idepth, const StereoRef& ref,
+                             float& r_idepth, float& r_var, float& r_epl) {
+  const int width = o.g.w, height = o.g.h;
+  const float* kfImg = o.kfImg;
+  const float* refImg = ref.img;
+  float KinvP[3] = {o.g.fxi * u + o.g.cxi, o.g.fyi * v + o.g.cyi, 1.0f};
+  float pInf[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pInf[i] = ref.KR[3 * i] * KinvP[0] + ref.KR[3 * i + 1] * KinvP[1] + ref.KR[3 * i + 2] * KinvP[2];
+  float pReal2 = pInf[2] / prior_idepth + ref.Kt[2];
+  float rescale = pReal2 * prior_idepth;
+  float firstX = u - 2.0f * epxn * rescale, firstY = v - 2.0f * epyn * rescale;
+  float lastX = u + 2.0f * epxn * rescale, lastY = v + 2.0f * epyn * rescale;
+  if (firstX <= 0.0f || firstX >= (float)(width - 2) || firstY <= 0.0f || firstY >= (float)(height - 2) ||
+      lastX <= 0.0f || lastX >= (float)(width - 2) || lastY <= 0.0f || lastY >= (float)(height - 2))
+    return -1.0f;
+  if (!(rescale > 0.7f && rescale < 1.4f)) return -1.0f;
+  float rv_p1 = dinterp(kfImg, u + epxn * rescale, v + epyn * rescale, width);
+  float rv_m1 = dinterp(kfImg, u - epxn * rescale, v - epyn * rescale, width);
+  float rv = dinterp(kfImg, u, v, width);
+  float rv_m2 = dinterp(kfImg, u - 2.0f * epxn * rescale, v - 2.0f * epyn * rescale, width);
+  float rv_p2 = dinterp(kfImg, u + 2.0f * epxn * rescale, v + 2.0f * epyn * rescale, width);
+
+  float pClose[3], pFar[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pClose[i] = pInf[i] + ref.Kt[i] * max_idepth;
+  if (pClose[2] < 0.001f) {
+    max_idepth = (0.001f - pInf[2]) / ref.Kt[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pClose[i] = pInf[i] + ref.Kt[i] * max_idepth;
+  }
+  {
+    float zc = pClose[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pClose[i] = pClose[i] / zc;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pFar[i] = pInf[i] + ref.Kt[i] * min_idepth;
+  if (pFar[2] < 0.001f || max_idepth < min_idepth) return -1.0f;
+  {
+    float zf = pFar[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pFar[i] = pFar[i] / zf;
+  }
+  if (isnan(pFar[0] + pClose[0]) || isnan(pFar[1] + pClose[1])) return -4.0f;
+
+  float incx = pClose[0] - pFar[0];
+  float incy = pClose[1] - pFar[1];
+  float eplLength = sqrtf(incx * incx + incy * incy);
+  if (!(eplLength > 0.0f) || isinf(eplLength)) return -4.0f;
+  const float crop = P.max_epl_length_crop;
+  if (eplLength > crop) {
+    pClose[0] = pFar[0] + incx * crop / eplLength;
+    pClose[1] = pFar[1] + incy * crop / eplLength;
+  }
+  incx *= P.gradient_sample_dist / eplLength;
+  incy *= P.gradient_sample_dist / eplLength;
+  pFar[0] -= incx;
+  pFar[1] -= incy;
+  pClose[0] += incx;
+  pClose[1] += incy;
+  const float mincrop = P.min_epl_length_crop;
+  if (eplLength < mincrop) {
+    float pad = (mincrop - eplLength) / 2.0f;
+    pFar[0] -= incx * pad;
+    pFar[1] -= incy * pad;
+    pClose[0] += incx * pad;
+    pClose[1] += incy * pad;
+  }
+  const float B = P.sample_point_to_border;
+  const float Wb = (float)width - B, Hb = (float)height - B;
+  if (pFar[0] <= B || pFar[0] >= Wb || pFar[1] <= B || pFar[1] >= Hb) return -1.0f;
+  if (pClose[0] <= B || pClose[0] >= Wb || pClose[1] <= B || pClose[1] >= Hb) {
+    if (pClose[0] <= B) {
+      float toAdd = (B - pClose[0]) / incx;
+      pClose[0] += toAdd * incx;
+      pClose[1] += toAdd * incy;
+    } else if (pClose[0] >= Wb) {
+      float toAdd = (Wb - pClose[0]) / incx;
+      pClose[0] += toAdd * incx;
+      pClose[1] += toAdd * incy;
+    }
+    if (pClose[1] <= B) {
+      float toAdd = (B - pClose[1]) / incy;
+      pClose[0] += toAdd * incx;
+      pClose[1] += toAdd * incy;
+    } else if (pClose[1] >= Hb) {
+      float toAdd = (Hb - pClose[1]) / incy;
+      pClose[0] += toAdd * incx;
+      pClose[1] += toAdd * incy;
+    }
+    float fincx = pClose[0] - pFar[0], fincy = pClose[1] - pFar[1];
+    float newLen = sqrtf(fincx * fincx + fincy * fincy);
+    if (pClose[0] <= B || pClose[0] >= Wb || pClose[1] <= B || pClose[1] >= Hb || newLen < 8.0f) return -1.0f;
+  }
+
+  float cpx = pFar[0], cpy = pFar[1];
+  float val_m2 = dinterp(refImg, cpx - 2.0f * incx, cpy - 2.0f * incy, width);
+  float val_m1 = dinterp(refImg, cpx - incx, cpy - incy, width);
+  float val = dinterp(refImg, cpx, cpy, width);
+  float val_p1 = dinterp(refImg, cpx + incx, cpy + incy, width);
+  float val_p2;
+  int loop = 0;
+  float best_x = -1.0f, best_y = -1.0f;
+  float best_err = INFINITY, second_err = INFINITY;
+  float best_errPre = NAN, best_errPost = NAN, best_DiffPre = NAN, best_DiffPost = NAN;
+  bool bestWasLast = false;
+  float eeLast = -1.0f;
+  float e1A = NAN, e1B = NAN, e2A = NAN, e2B = NAN, e3A = NAN, e3B = NAN, e4A = NAN, e4B = NAN, e5A = NAN, e5B = NAN;
+  int loopBest = -1, loopSecond = -1;
+  const int maxSteps = P.stereo_max_steps;
+  while ((((incx < 0.0f) == (cpx > pClose[0]) && (incy < 0.0f) == (cpy > pClose[1])) || loop == 0) &&
+         loop < maxSteps) {
+    val_p2 = dinterp(refImg, cpx + 2.0f * incx, cpy + 2.0f * incy, width);
+    float ee = 0.0f;
+    if (loop % 2 == 0) {
+      e1A = val_p2 - rv_p2; ee += e1A * e1A;
+      e2A = val_p1 - rv_p1; ee += e2A * e2A;
+      e3A = val - rv; ee += e3A * e3A;
+      e4A = val_m1 - rv_m1; ee += e4A * e4A;
+      e5A = val_m2 - rv_m2; ee += e5A * e5A;
+    } else {
+      e1B = val_p2 - rv_p2; ee += e1B * e1B;
+      e2B = val_p1 - rv_p1; ee += e2B * e2B;
+      e3B = val - rv; ee += e3B * e3B;
+      e4B = val_m1 - rv_m1; ee += e4B * e4B;
+      e5B = val_m2 - rv_m2; ee += e5B * e5B;
+    }
+    if (ee < best_err) {
+      second_err = best_err;
+      loopSecond = loopBest;
+      best_err = ee;
+      loopBest = loop;
+      best_errPre = eeLast;
+      best_DiffPre = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+      best_errPost = -1.0f;
+      best_DiffPost = -1.0f;
+      best_x = cpx;
+      best_y = cpy;
+      bestWasLast = true;
+    } else {
+      if (bestWasLast) {
+        best_errPost = ee;
+        best_DiffPost = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+        bestWasLast = false;
+      }
+      if (ee < second_err) {
+        second_err = ee;
+        loopSecond = loop;
+      }
+    }
+    eeLast = ee;
+    val_m2 = val_m1;
+    val_m1 = val;
+    val = val_p1;
+    val_p1 = val_p2;
+    cpx += incx;
+    cpy += incy;
+    loop++;
+  }
+  if (best_err > 4.0f * P.max_error_stereo) return -3.0f;
+  if ((float)abs(loopBest - loopSecond) > 1.0f && P.min_distance_error_stereo * best_err > second_err) return -2.0f;
+
+  bool didSubpixel = false;
+  if (P.use_subpixel_stereo) {
+    float gradPre_pre = -(best_errPre - best_DiffPre);
+    float gradPre_this = +(best_err - best_DiffPre);
+    float gradPost_this = -(best_err - best_DiffPost);
+    float gradPost_post = +(best_errPost - best_DiffPost);
+    bool interpPost = false, interpPre = false;
+    if (P.stereo_subpixel_oob_guard && (best_errPre < 0.0f || best_errPost < 0.0f)) {
+    } else if ((gradPost_this < 0.0f) ^ (gradPre_this < 0.0f)) {
+    } else if ((gradPre_pre < 0.0f) ^ (gradPre_this < 0.0f)) {
+      if (!((gradPost_post < 0.0f) ^ (gradPost_this < 0.0f))) interpPre = true;
+    } else if ((gradPost_post < 0.0f) ^ (gradPost_this < 0.0f)) {
+      interpPost = true;
+    }
+    if (interpPre) {
+      float d = gradPre_this / (gradPre_this - gradPre_pre);
+      best_x -= d * incx;
+      best_y -= d * incy;
+      best_err = best_err - 2.0f * d * gradPre_this - (gradPre_pre - gradPre_this) * d * d;
+      didSubpixel = true;
+    } else if (interpPost) {
+      float d = gradPost_this / (gradPost_this - gradPost_post);
+      best_x += d * incx;
+      best_y += d * incy;
+      best_err = best_err + 2.0f * d * gradPost_this + (gradPost_post - gradPost_this) * d * d;
+      didSubpixel = true;
+    }
+  }
+  float sampleDist = P.gradient_sample_dist * rescale;
+  float gradAlongLine = 0.0f, tmp;
+  tmp = rv_p2 - rv_p1; gradAlongLine += tmp * tmp;
+  tmp = rv_p1 - rv; gradAlongLine += tmp * tmp;
+  tmp = rv - rv_m1; gradAlongLine += tmp * tmp;
+  tmp = rv_m1 - rv_m2; gradAlongLine += tmp * tmp;
+  gradAlongLine /= sampleDist * sampleDist;
+  if (best_err > P.max_error_stereo + sqrtf(gradAlongLine) * 20.0f) return -3.0f;
+
+  float idnew, alpha;
+  const float* t = ref.t_o2t;
+  if (incx * incx > incy * incy) {
+    float oldX = o.g.fxi * best_x + o.g.cxi;
+    float nominator = oldX * t[2] - t[0];
+    float dot0 = KinvP[0] * ref.Rrow[0] + KinvP[1] * ref.Rrow[1] + KinvP[2] * ref.Rrow[2];
+    float dot2 = KinvP[0] * ref.Rrow[6] + KinvP[1] * ref.Rrow[7] + KinvP[2] * ref.Rrow[8];
+    idnew = (dot0 - oldX * dot2) / nominator;
+    alpha = incx * o.g.fxi * (dot0 * t[2] - dot2 * t[0]) / (nominator * nominator);
+  } else {
+    float oldY = o.g.fyi * best_y + o.g.cyi;
+    float nominator = oldY * t[2] - t[1];
+    float dot1 = KinvP[0] * ref.Rrow[3] + KinvP[1] * ref.Rrow[4] + KinvP[2] * ref.Rrow[5];
+    float dot2 = KinvP[0] * ref.Rrow[6] + KinvP[1] * ref.Rrow[7] + KinvP[2] * ref.Rrow[8];
+    idnew = (dot1 - oldY * dot2) / nominator;
+    alpha = incy * o.g.fyi * (dot1 * t[2] - dot2 * t[1]) / (nominator * nominator);
+  }
+  if (idnew < 0.0f && !P.allow_negative_idepths) return -2.0f;
+  float photoDispError = 4.0f * P.camera_pixel_noise2 / (gradAlongLine + P.division_eps);
+  float trackingErrorFac = 0.25f * (1.0f + ref.initialTrackedResidual);
+  float2 g2 = dinterp42(o.kfGrad, u, v, width);
+  float geo = (g2.x * epxn + g2.y * epyn) + P.division_eps;
+  geo = trackingErrorFac * trackingErrorFac * (g2.x * g2.x + g2.y * g2.y) / (geo * geo);
+  r_var = alpha * alpha * ((didSubpixel ? 0.05f : 0.5f) * sampleDist * sampleDist + geo + photoDispError);
+  r_idepth = idnew;
+  r_epl = eplLength;
+  return best_err;
+}
+
+struct ObsArgs {
+  DepthGeom g;
+  HypBuf cur;
+  uint8_t* obs;
+  const float* kfImg;
+  const float* kfMaxg;
+  const float4* kfGrad;
+  const StereoRef* refs;
+  const int* refById;
+  int nRefById;
+  int refByIdOffset;
+  int oldestIdx, newestIdx;
+  int kfReactivated;
+  int kfId;
+  float numTracked, numMapped;
+};
+
+__device__ inline bool was_good_gate(const StereoRef& r, int x, int y, int w) {
+  if (r.wasGood && !r.wasGood[(x >> 1) + (w >> 1) * (y >> 1)]) return false;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_observe(ObsArgs a, lsdg_params P) {
+  const int w = a.g.w, h = a.g.h;
+  int x = 3 + blockIdx.x * 16 + (threadIdx.x & 15);
+  int y = 3 + blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w - 3 || y >= h - 3) return;
+  int idx = x + y * w;
+  ObsCtx o{a.g, a.kfImg, a.kfGrad};
+  Hyp t = load_hyp(a.cur, idx);
+  float mg = a.kfMaxg[idx];
+  uint8_t code = LSDG_OBS_NONE;
+  bool has = t.valid;
+  if (has && mg < P.min_abs_grad_decrease) {
+    a.cur.valid[idx] = 0;
+    a.obs[idx] = LSDG_OBS_LOWGRAD_INVALIDATE;
+    return;
+  }
+  if (mg < P.min_abs_grad_create || t.bl < P.min_blacklist) {
+    a.obs[idx] = LSDG_OBS_NONE;
+    return;
+  }
+  if (!has) {
+    // observeDepthCreate
+    const StereoRef& ref = a.refs[a.kfReactivated ? a.newestIdx : a.oldestIdx];
+    float epx, epy, rid, rvar, repl;
+    if (!was_good_gate(ref, x, y, w)) {
+      code = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
+    } else if (!make_and_check_epl(o, P, x, y, ref, &epx, &epy)) {
+      code = LSDG_OBS_EPL_FAIL;
+    } else {
+      float err = line_stereo(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
+      if (err == -3.0f || err == -2.0f) {
+        t.bl--;
+        a.cur.bl[idx] = t.bl;
+      }
+      if (err < 0.0f) {
+        code = (uint8_t)(LSDG_OBS_STEREO_FAIL_BASE - (int)err);
+      } else if (rvar > P.max_var) {
+        code = LSDG_OBS_CREATE_VAR_TOO_BIG;
+      } else {
+        rid = unzero(rid);
+        store_hyp(a.cur, idx, make_hyp(rid, rvar, P.validity_counter_initial_observe));
+        code = LSDG_OBS_CREATED;
+      }
+    }
+    a.obs[idx] = code;
+    return;
+  }
+  // observeDepthUpdate
+  int ri;
+  if (!a.kfReactivated) {
+    int k = (int)t.nsid - a.refByIdOffset;
+    if (k >= a.nRefById) {
+      a.obs[idx] = LSDG_OBS_SKIP_ALREADY_GOOD;
+      return;
+    }
+    ri = k < 0 ? a.oldestIdx : a.refById[k];
+  } else {
+    ri = a.newestIdx;
+  }
+  const StereoRef& ref = a.refs[ri];
+  if (!was_good_gate(ref, x, y, w)) {
+    a.obs[idx] = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
+    return;
+  }
+  float epx, epy;
+  if (!make_and_check_epl(o, P, x, y, ref, &epx, &epy)) {
+    a.obs[idx] = LSDG_OBS_EPL_FAIL;
+    return;
+  }
+  float sv = sqrtf(t.dv.w);
+  float min_id = t.dv.z - sv * P.stereo_epl_var_fac;
+  float max_id = t.dv.z + sv * P.stereo_epl_var_fac;
+  if (min_id < 0.0f) min_id = 0.0f;
+  if (max_id > 1.0f / P.min_depth) max_id = 1.0f / P.min_depth;
+  float rid = 0.0f, rvar = 0.0f, repl = 0.0f;
+  float err = line_stereo(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl);
+  float diff = rid - t.dv.z;
+  if (err == -1.0f) {
+    code = LSDG_OBS_STEREO_FAIL_BASE + 1;
+  } else if (err == -2.0f) {
+    t.val -= P.validity_counter_dec;
+    if (t.val < 0) t.val = 0;
+    t.nsid = 0.0f;
+    t.dv.y *= P.fail_var_inc_fac;
+    if (t.dv.y > P.max_var) {
+      t.valid = false;
+      t.bl--;
+    }
+    store_hyp(a.cur, idx, t);
+    code = LSDG_OBS_STEREO_FAIL_BASE + 2;
+  } else if (err == -3.0f) {
+    code = LSDG_OBS_STEREO_FAIL_BASE + 3;
+  } else if (err == -4.0f) {
+    code = LSDG_OBS_STEREO_FAIL_BASE + 4;
+  } else if (P.diff_fac_observe * diff * diff > rvar + t.dv.w) {
+    t.dv.y *= P.fail_var_inc_fac;
+    if (t.dv.y > P.max_var) t.valid = false;
+    store_hyp(a.cur, idx, t);
+    code = LSDG_OBS_INCONSISTENT;
+  } else {
+    float id_var = t.dv.y * P.succ_var_inc_fac;
+    float wgt = rvar / (rvar + id_var);
+    float new_id = (1.0f - wgt) * rid + wgt * t.dv.x;
+    t.dv.x = unzero(new_id);
+    id_var = id_var * wgt;
+    if (id_var < t.dv.y) t.dv.y = id_var;
+    t.val += P.validity_counter_inc;
+    float cap = (float)P.validity_counter_max + mg * (float)P.validity_counter_max_variable / 255.0f;
+    if ((float)t.val > cap) t.val = (int)cap;
+    if (repl < P.min_epl_length_crop) {
+      float inc = a.numTracked / (a.numMapped + 5.0f);
+      if (inc < 3.0f) inc = 3.0f;
+      inc += (float)(((int)(repl * 10000.0f)) % 2);
+      if ((double)repl < 0.5 * (double)P.min_epl_length_crop) inc *= 3.0f;
+      t.nsid = (float)ref.id + inc;
+    }
+    store_hyp(a.cur, idx, t);
+    code = LSDG_OBS_UPDATED;
+  }
+  a.obs[idx] = code;
+}
+
+// ---------------------------------------------------------- fill holes
+__global__ __launch_bounds__(256) void k_fill_holes(DepthGeom g, HypBuf in, HypBuf out, const float* __restrict__ maxg,
+                                                    lsdg_params P) {
+  const int w = g.w, h = g.h;
+  int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  int idx = x + y * w;
+  Hyp d = load_hyp(in, idx);
+  if (x >= 3 && x < w - 2 && y >= 3 && y < h - 2 && !d.valid && !(maxg[idx] < P.min_abs_grad_decrease)) {
+    // 5x5 box sum of the validity of valid pixels (the reference reads it off
+    // an integral image; the integer sum is the same)
+    int val = 0;
+    for (int yy = y - 2; yy <= y + 2; ++yy)
+      for (int xx = x - 2; xx <= x + 2; ++xx) {
+        int j = xx + yy * w;
+        if (in.valid[j]) val += in.val[j];
+      }
+    if ((d.bl >= P.min_blacklist && val > P.val_sum_min_for_create) || val > P.val_sum_min_for_unblacklist) {
+      float sumId = 0.0f, sumIv = 0.0f;
+      for (int yy = y - 2; yy <= y + 2; ++yy)
+        for (int xx = x - 2; xx <= x + 2; ++xx) {
+          int j = xx + yy * w;
+          if (!in.valid[j]) continue;
+          float4 s = in.dv[j];
+          sumId += s.x / s.y;
+          sumIv += 1.0f / s.y;
+        }
+      float obs = sumId / sumIv;
+      obs = unzero(obs);
+      d = make_hyp(obs, P.var_random_init_initial, 0);
+    }
+  }
+  store_hyp(out, idx, d);
+}
+
+// ---------------------------------------------------------- regularize
+__global__ __launch_bounds__(256) void k_regularize(DepthGeom g, HypBuf in, HypBuf out, lsdg_params P, int removeOcc,
+                                                    int validityTH) {
+  const int w = g.w, h = g.h;
+  int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  int idx = x + y * w;
+  Hyp d = load_hyp(in, idx);
+  if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2 && d.valid) {
+    const float regDistVar = 0.075f * 0.075f * P.depth_smoothing_factor * P.depth_smoothing_factor;
+    float sum = 0.0f, val_sum = 0.0f, sumIvar = 0.0f;
+    int nOcc = 0, nNot = 0;
+    for (int dx = -2; dx <= 2; ++dx)
+      for (int dy = -2; dy <= 2; ++dy) {
+        int j = idx + dx + dy * w;
+        if (!in.valid[j]) continue;
+        float4 s = in.dv[j];
+        float diff = s.x - d.dv.x;
+        if (P.diff_fac_smoothing * diff * diff > s.y + d.dv.y) {
+          if (removeOcc && s.x > d.dv.x) nOcc++;
+          continue;
+        }
+        val_sum += (float)in.val[j];
+        if (removeOcc) nNot++;
+        float distFac = (float)(dx * dx + dy * dy) * regDistVar;
+        float ivar = 1.0f / (s.y + distFac);
+        sum += s.x * ivar;
+        sumIvar += ivar;
+      }
+    if (val_sum < (float)validityTH) {
+      d.valid = false;
+      d.bl--;
+    } else if (removeOcc && nOcc > nNot) {
+      d.valid = false;
+    } else {
+      sum = sum / sumIvar;
+      sum = unzero(sum);
+      d.dv.z = sum;
+      d.dv.w = 1.0f / sumIvar;
+    }
+  }
+  store_hyp(out, idx, d);
+}
+
+// -------------------------------------------------------- Frame::setDepth
+__global__ void k_set_depth(HypBuf cur, int n, float* __restrict__ idepth, float* __restrict__ var) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 dv = cur.dv[i];
+  if (cur.valid[i] && (double)dv.z >= -0.05) {
+    idepth[i] = unzero(dv.z);
+    var[i] = unzero(dv.w);
+  } else {
+    idepth[i] = -1.0f;
+    var[i] = -1.0f;
+  }
+}
+
+// ------------------------------------------------------------ propagate
+struct PropArgs {
+  DepthGeom g;
+  HypBuf src;
+  Xf x;                      // old -> new keyframe
+  const uint8_t* wasGood;    // level-1 mask of the new keyframe, or null
+  const float* kfImg;        // old keyframe image
+  const float* newMaxg;
+  const float* newImg;
+  int* dest;
+  float2* newIV;
+  int* count;
+};
+
+__global__ __launch_bounds__(256) void k_prop_project(PropArgs a, lsdg_params P) {
+  const int W = a.g.w, H = a.g.h;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  a.dest[i] = -1;
+  if (!a.src.valid[i]) return;
+  int y = i / W, x = i - y * W;
+  float4 s = a.src.dv[i];
+  float k0 = (float)x * a.g.fxi + a.g.cxi, k1 = (float)y * a.g.fyi + a.g.cyi, k2 = 1.0f;
+  float pn[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    pn[r] = (a.x.R[3 * r] * k0 + a.x.R[3 * r + 1] * k1 + a.x.R[3 * r + 2] * k2) / s.z + a.x.t[r];
+  float new_id = 1.0f / pn[2];
+  float u = pn[0] * new_id * a.g.fx + a.g.cx;
+  float v = pn[1] * new_id * a.g.fy + a.g.cy;
+  if (!(u > 2.1f && v > 2.1f && u < (float)W - 3.1f && v < (float)H - 3.1f)) return;
+  int nidx = (int)(u + 0.5f) + ((int)(v + 0.5f)) * W;
+  float destAbsGrad = a.newMaxg[nidx];
+  if (a.wasGood) {
+    if (!a.wasGood[(x >> 1) + (W >> 1) * (y >> 1)] || destAbsGrad < P.min_abs_grad_decrease) return;
+  } else {
+    float sc = a.kfImg[i];
+    float dc = dinterp(a.newImg, u, v, W);
+    float res = dc - sc;
+    if (res * res / (P.max_diff_constant + P.max_diff_grad_mult * destAbsGrad * destAbsGrad) > 1.0f ||
+        destAbsGrad < P.min_abs_grad_decrease)
+      return;
+  }
+  float r4 = new_id / s.z;
+  r4 *= r4;
+  r4 *= r4;
+  float new_var = r4 * s.y;
+  a.dest[i] = nidx;
+  a.newIV[i] = make_float2(new_id, new_var);
+  atomicAdd(&a.count[nidx], 1);
+}
+
+__global__ __launch_bounds__(256) void k_prop_bucket(const int* __restrict__ dest, const int* __restrict__ offsets,
+                                                     int* __restrict__ fill, int* __restrict__ bucket, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int d = dest[i];
+  if (d < 0) return;
+  int pos = atomicAdd(&fill[d], 1);
+  bucket[offsets[d] + pos] = i;
+}
+
+// One thread per destination: sources in ascending (raster) order, the
+// reference's occlusion / create / merge rules.  Empty destinations only get
+// the wipe (isValid = false, blacklisted = 0).
+__global__ __launch_bounds__(256) void k_prop_fold(HypBuf src, HypBuf out, const float2* __restrict__ newIV,
+                                                   const int* __restrict__ offsets, const int* __restrict__ bucket,
+                                                   int n, lsdg_params P) {
+  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  int b0 = offsets[d], b1 = offsets[d + 1];
+  if (b0 == b1) {
+    out.valid[d] = 0;
+    out.bl[d] = 0;
+    return;
+  }
+  Hyp t;
+  t.valid = false;
+  t.bl = 0;
+  t.nsid = out.nsid[d];
+  t.val = out.val[d];
+  t.dv = out.dv[d];
+  int last = -1;
+  for (int k = b0; k < b1; ++k) {
+    // next source in raster order
+    int si = 0x7fffffff;
+    for (int j = b0; j < b1; ++j) {
+      int c = bucket[j];
+      if (c > last && c < si) si = c;
+    }
+    last = si;
+    float2 iv = newIV[si];
+    float new_id = iv.x, new_var = iv.y;
+    int sval = src.val[si];
+    if (t.valid) {
+      float diff = t.dv.x - new_id;
+      if (P.diff_fac_prop_merge * diff * diff > new_var + t.dv.y) {
+        if (new_id < t.dv.x) continue;
+        t.valid = false;
+      }
+    }
+    if (!t.valid) {
+      t = make_hyp(new_id, new_var, min(P.validity_counter_max_variable, sval));
+    } else {
+      float wgt = new_var / (t.dv.y + new_var);
+      float merged = wgt * t.dv.x + (1.0f - wgt) * new_id;
+      int mv = sval + t.val;
+      int cap = P.validity_counter_max + P.validity_counter_max_variable;
+      if (mv > cap) mv = cap;
+      t = make_hyp(merged, 1.0f / (1.0f / t.dv.y + 1.0f / new_var), mv);
+    }
+  }
+  store_hyp(out, d, t);
+}
+
+// Exclusive scan of count[0..n) into offsets (in place), offsets[n] = total.
+// Three kernels: per-block sums, a one-block scan of block sums, per-block scan.
+__global__ __launch_bounds__(256) void k_scan_blocksum(const int* __restrict__ a, int n, int* __restrict__ blk) {
+  __shared__ int sh[256];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  sh[threadIdx.x] = i < n ? a[i] : 0;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int* blk, int nb) {
+  __shared__ int sh[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? blk[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) blk[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[nb] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(int* a, int n, const int* __restrict__ blk) {
+  __shared__ int sh[256];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  int v = i < n ? a[i] : 0;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) a[i] = blk[blockIdx.x] + sh[threadIdx.x] - v;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) a[n] = blk[gridDim.x];
+}
+
+// ---------------------------------------------------- mean inverse depth
+// Count and FP64 sum of idepth_smoothed over valid pixels; the last block
+// merges the per-block rows in block order and writes (count, sum).
+__global__ __launch_bounds__(256) void k_mean_idepth(HypBuf cur, int n, double* __restrict__ partials,
+                                                     unsigned* counter, double* __restrict__ out) {
+  __shared__ double shc[256], shs[256];
+  __shared__ bool amLast;
+  double c = 0.0, s = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    if (cur.valid[i]) {
+      c += 1.0;
+      s += (double)cur.dv[i].z;
+    }
+  shc[threadIdx.x] = c;
+  shs[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      shc[threadIdx.x] += shc[threadIdx.x + k];
+      shs[threadIdx.x] += shs[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = shc[0];
+    partials[2 * blockIdx.x + 1] = shs[0];
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!amLast) return;
+  __threadfence();
+  if (threadIdx.x < 2) {
+    double a = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) {
+      unsigned long long bits = __hip_atomic_load((const unsigned long long*)&partials[2 * b + threadIdx.x],
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      a += __longlong_as_double((long long)bits);
+    }
+    out[threadIdx.x] = a;
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_rescale(HypBuf cur, int n, float f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !cur.valid[i]) return;
+  float f2 = f * f;
+  float4 d = cur.dv[i];
+  d.x *= f;
+  d.z *= f;
+  d.y *= f2;
+  d.w *= f2;
+  cur.dv[i] = d;
+}
+
+// Hypotheses from (idepth, var) arrays: valid where var > 0 and idepth is
+// finite (gt != 0: also idepth > 0, as initializeFromGTDepth).
+__global__ void k_hyp_prior(HypBuf cur, int n, const float* __restrict__ id, const float* __restrict__ var,
+                            int validity, int gt, float gtVar) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v = id[i], s = gt ? gtVar : var[i];
+  bool ok = gt ? (!isnan(v) && v > 0.0f) : (isfinite(v) && s > 0.0f);
+  Hyp h;
+  if (ok) {
+    h = make_hyp(v, s, validity);
+    h.dv.z = v;
+    h.dv.w = s;
+  } else {
+    h.valid = false;
+    h.bl = 0;
+    h.nsid = gt ? cur.nsid[i] : 0.0f;
+    h.val = gt ? cur.val[i] : 0;
+    h.dv = gt ? cur.dv[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  store_hyp(cur, i, h);
+}
+
+// ------------------------------------------------------------ launchers
+static inline dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }
+
+hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* d_refs, const int* d_refById,
+                          int nRefById, int refByIdOffset, int oldestIdx, int newestIdx, float numTracked,
+                          float numMapped, hipStream_t s) {
+  ObsArgs a;
+  a.g = dm->g;
+  a.cur = dm->buf[dm->cur];
+  a.obs = dm->d_obs;
+  lsdg_frame kf = dm->kf;
+  a.kfImg = kf->d_image[0];
+  a.kfMaxg = kf->d_maxg[0];
+  a.kfGrad = kf->d_grad[0];
+  a.refs = d_refs;
+  a.refById = d_refById;
+  a.nRefById = nRefById;
+  a.refByIdOffset = refByIdOffset;
+  a.oldestIdx = oldestIdx;
+  a.newestIdx = newestIdx;
+  a.kfReactivated = dm->kfReactivated ? 1 : 0;
+  a.kfId = kf->id;
+  a.numTracked = numTracked;
+  a.numMapped = numMapped;
+  (void)hipMemsetAsync(dm->d_obs, 0, (size_t)dm->g.w * dm->g.h, s);
+  int iw = dm->g.w - 6, ih = dm->g.h - 6;
+  if (iw <= 0 || ih <= 0) return hipSuccess;
+  k_observe<<<grid16(iw, ih), dim3(256), 0, s>>>(a, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s) {
+  int in = dm->cur, out = 1 - dm->cur;
+  k_fill_holes<<<grid16(dm->g.w, dm->g.h), dim3(256), 0, s>>>(dm->g, dm->buf[in], dm->buf[out], dm->kf->d_maxg[0], P);
+  dm->cur = out;
+  return hipGetLastError();
+}
+
+hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th, hipStream_t s) {
+  int in = dm->cur, out = 1 - dm->cur;
+  k_regularize<<<grid16(dm->g.w, dm->g.h), dim3(256), 0, s>>>(dm->g, dm->buf[in], dm->buf[out], P,
+                                                               removeOcclusions ? 1 : 0, th);
+  dm->cur = out;
+  return hipGetLastError();
+}
+
+hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s) {
+  int n = dm->g.w * dm->g.h;
+  k_set_depth<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(dm->buf[dm->cur], n, f->d_idepth[0], f->d_ivar[0]);
+  return hipGetLastError();
+}
+
+hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame nkf, const Xf& oldToNew,
+                            const uint8_t* d_wasGood, hipStream_t s) {
+  const int n = dm->g.w * dm->g.h;
+  const int nb = (n + 255) / 256;
+  hipError_t e;
+  if ((e = hipMemsetAsync(dm->d_count, 0, sizeof(int) * (n + 1), s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(dm->d_fill, 0, sizeof(int) * n, s)) != hipSuccess) return e;
+  PropArgs a;
+  a.g = dm->g;
+  a.src = dm->buf[dm->cur];
+  a.x = oldToNew;
+  a.wasGood = d_wasGood;
+  a.kfImg = dm->kf->d_image[0];
+  a.newMaxg = nkf->d_maxg[0];
+  a.newImg = nkf->d_image[0];
+  a.dest = dm->d_dest;
+  a.newIV = dm->d_newIV;
+  a.count = dm->d_count;
+  k_prop_project<<<dim3(nb), dim3(256), 0, s>>>(a, P);
+  k_scan_blocksum<<<dim3(nb), dim3(256), 0, s>>>(dm->d_count, n, dm->d_scanblk);
+  k_scan_top<<<dim3(1), dim3(1024), 0, s>>>(dm->d_scanblk, nb);
+  k_scan_apply<<<dim3(nb), dim3(256), 0, s>>>(dm->d_count, n, dm->d_scanblk);
+  k_prop_bucket<<<dim3(nb), dim3(256), 0, s>>>(dm->d_dest, dm->d_count, dm->d_fill, dm->d_bucket, n);
+  int in = dm->cur, out = 1 - dm->cur;
+  k_prop_fold<<<dim3(nb), dim3(256), 0, s>>>(dm->buf[in], dm->buf[out], dm->d_newIV, dm->d_count, dm->d_bucket, n, P);
+  dm->cur = out;
+  return hipGetLastError();
+}
+
+hipError_t launch_mean_idepth(lsdg_depthmap dm, hipStream_t s) {
+  int n = dm->g.w * dm->g.h;
+  int nb = (n + 255 * 8) / (256 * 8);
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  k_mean_idepth<<<dim3(nb), dim3(256), 0, s>>>(dm->buf[dm->cur], n, dm->d_red, dm->d_red_counter, dm->d_red_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rescale(lsdg_depthmap dm, float f, hipStream_t s) {
+  int n = dm->g.w * dm->g.h;
+  k_rescale<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(dm->buf[dm->cur], n, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_hyp_prior(lsdg_depthmap dm, const float* d_id, const float* d_var, int validity, int gt,
+                            float gtVar, hipStream_t s) {
+  int n = dm->g.w * dm->g.h;
+  k_hyp_prior<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(dm->buf[dm->cur], n, d_id, d_var, validity, gt, gtVar);
+  return hipGetLastError();
+}
+
+}  // namespace lsdg

@@ -54,6 +54,34 @@ struct TrackKParams {
   float roll[4];      // Sim3 ESM roll (xRoll0, xRoll1, yRoll0, yRoll1)
 };
 
+// Depth hypotheses (DepthMapPixelHypothesis) as structure of arrays:
+// dv = (idepth, idepth_var, idepth_smoothed, idepth_var_smoothed).
+struct HypBuf {
+  uint8_t* valid;
+  int* bl;       // blacklisted
+  float* nsid;   // nextStereoFrameMinID
+  int* val;      // validity_counter
+  float4* dv;
+};
+
+// Frame::prepareForStereoWith products of one reference frame, FP32.
+struct StereoRef {
+  const float* img;
+  const uint8_t* wasGood;  // refPixelWasGood gate, null when not tracked on the keyframe
+  float KR[9];             // K_otherToThis_R
+  float Kt[3];             // K_otherToThis_t
+  float t_o2t[3];          // otherToThis_t
+  float t_t2o[3];          // thisToOther_t
+  float Rrow[9];           // otherToThis_R_row0..2
+  float initialTrackedResidual;
+  int id;
+};
+
+struct DepthGeom {
+  int w, h;
+  float fx, fy, cx, cy, fxi, fyi, cxi, cyi;
+};
+
 struct PointsLevel {
   const float4* posI;  // x, y, z, I
   const float* var;
@@ -61,6 +89,21 @@ struct PointsLevel {
   const int* idx;      // pixel index in the keyframe level
   int n;
 };
+
+// depthmap.hip
+hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* d_refs, const int* d_refById,
+                          int nRefById, int refByIdOffset, int oldestIdx, int newestIdx, float numTracked,
+                          float numMapped, hipStream_t s);
+hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s);  // swaps buffers
+hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th,
+                             hipStream_t s);  // swaps buffers
+hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s);
+hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame nkf, const Xf& oldToNew,
+                            const uint8_t* d_wasGood, hipStream_t s);  // swaps buffers
+hipError_t launch_mean_idepth(lsdg_depthmap dm, hipStream_t s);  // h_red[0] = count, h_red[1] = sum
+hipError_t launch_rescale(lsdg_depthmap dm, float f, hipStream_t s);
+hipError_t launch_hyp_prior(lsdg_depthmap dm, const float* d_id, const float* d_var, int validity, int gt,
+                            float gtVar, hipStream_t s);
 
 }  // namespace lsdg
 
@@ -133,6 +176,37 @@ struct lsdg_ref_s {
   int blockcounts_cap = 0;
 };
 
+struct lsdg_depthmap_s {
+  lsdg_ctx ctx = nullptr;
+  lsdg::DepthGeom g;
+  void* d_pool = nullptr;
+  lsdg::HypBuf buf[2];
+  int cur = 0;
+  uint8_t* d_obs = nullptr;
+  lsdg_frame kf = nullptr;
+  bool kfReactivated = false;
+  bool valid = false;
+  // observe inputs
+  lsdg::StereoRef* d_refs = nullptr;
+  int refs_cap = 0;
+  int* d_refById = nullptr;
+  int refById_cap = 0;
+  lsdg::StereoRef* h_refs = nullptr;  // pinned staging
+  int* h_refById = nullptr;
+  // propagation scratch
+  int* d_dest = nullptr;
+  float2* d_newIV = nullptr;
+  int* d_count = nullptr;   // P + 1
+  int* d_fill = nullptr;    // P
+  int* d_bucket = nullptr;  // P
+  int* d_scanblk = nullptr;
+  // rescale reduction
+  double* d_red = nullptr;  // partial rows
+  unsigned* d_red_counter = nullptr;
+  double* h_red = nullptr;  // pinned, host-mapped (count, sum)
+  double* d_red_out = nullptr;
+};
+
 // ---------------------------------------------------------- launchers
 namespace lsdg {
 
@@ -150,5 +224,20 @@ hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, in
                             const TrackKParams& kp, double* d_out);
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                               const TrackKParams& kp, float* d_grad_d, double* d_out);
+
+// depthmap.hip
+hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* d_refs, const int* d_refById,
+                          int nRefById, int refByIdOffset, int oldestIdx, int newestIdx, float numTracked,
+                          float numMapped, hipStream_t s);
+hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s);  // swaps buffers
+hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th,
+                             hipStream_t s);  // swaps buffers
+hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s);
+hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame nkf, const Xf& oldToNew,
+                            const uint8_t* d_wasGood, hipStream_t s);  // swaps buffers
+hipError_t launch_mean_idepth(lsdg_depthmap dm, hipStream_t s);  // h_red[0] = count, h_red[1] = sum
+hipError_t launch_rescale(lsdg_depthmap dm, float f, hipStream_t s);
+hipError_t launch_hyp_prior(lsdg_depthmap dm, const float* d_id, const float* d_var, int validity, int gt,
+                            float gtVar, hipStream_t s);
 
 }  // namespace lsdg

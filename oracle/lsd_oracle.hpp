@@ -944,6 +944,7 @@ struct DepthMap {
   std::vector<Frame<S>*> refById;
   int refByIdOffset = 0;
   int threads = 1;
+  bool accDouble = false;  // FP64 accumulation of the rescale mean
 
   void init(int w, int h, float fx0, float fy0, float cx0, float cy0) {
     width = w; height = h;
@@ -1039,7 +1040,9 @@ struct DepthMap {
       S zf = pFar[2];
       for (int i = 0; i < 3; ++i) pFar[i] = pFar[i] / zf;
     }
-    if (std::isnan(pFar[0] + pClose[0])) return -4;
+    // the reference tests only the x components; the y components are tested
+    // too so that no NaN coordinate reaches the bilinear sampler
+    if (std::isnan(pFar[0] + pClose[0]) || std::isnan(pFar[1] + pClose[1])) return -4;
 
     S incx = pClose[0] - pFar[0];
     S incy = pClose[1] - pFar[1];
@@ -1550,18 +1553,22 @@ struct DepthMap {
     regularize(true, P.val_sum_min_for_keep);
     fillHoles();
     regularize(false, P.val_sum_min_for_keep);
-    float sumIdepth = 0, numIdepth = 0;
-    for (auto& h : cur) {
-      if (!h.isValid) continue;
-      sumIdepth += (float)h.idepth_smoothed;
-      numIdepth++;
-    }
-    S rescale = (S)(numIdepth / sumIdepth);
-    if (sizeof(S) == 8) {  // FP64 oracle: exact mean
+    // mean-idepth rescale: a sequential FP32 sum as the reference, or an FP64
+    // sum (FP64 oracle, and the FP32 restatement with FP64 accumulation)
+    S rescale;
+    if (sizeof(S) == 8 || accDouble) {
       double s = 0, n = 0;
       for (auto& h : cur)
         if (h.isValid) { s += (double)h.idepth_smoothed; n += 1; }
       rescale = (S)(n / s);
+    } else {
+      float sumIdepth = 0, numIdepth = 0;
+      for (auto& h : cur) {
+        if (!h.isValid) continue;
+        sumIdepth += (float)h.idepth_smoothed;
+        numIdepth++;
+      }
+      rescale = (S)(numIdepth / sumIdepth);
     }
     S rescale2 = rescale * rescale;
     for (auto& h : cur) {

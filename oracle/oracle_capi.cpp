@@ -483,7 +483,7 @@ extern "C" void* orc_depth_new(int prec, int w, int h, float fx, float fy, float
   if (d->dbl) {
     d->d64.P = *P; d->d64.threads = threads; d->d64.init(w, h, fx, fy, cx, cy);
   } else {
-    d->d32.P = *P; d->d32.threads = threads; d->d32.init(w, h, fx, fy, cx, cy);
+    d->d32.P = *P; d->d32.threads = threads; d->d32.accDouble = (prec == 2); d->d32.init(w, h, fx, fy, cx, cy);
   }
   return d;
 }
