@@ -1,0 +1,426 @@
+// Batched tracking: many frame-vs-keyframe alignments of uneven size advance
+// one LM evaluation per launch of k_batch_step, and a hipGraph replays K such
+// launches with no host round trip.
+//
+// Work decomposition.  Each job's points at its current level are cut into
+// tiles of TILE = 256 * ppt points.  A persistent grid walks the concatenated
+// tile list of all active jobs (the per-job tile counts are double-buffered
+// per step so every block sees the same list).  Each tile's FP64 partial sums
+// go to a fixed slot; the block that finishes a job's last tile merges that
+// job's slots in slot order (deterministic), then runs the job's LM step in
+// one lane: the reference's accept / reject / lambda / convergence logic
+// (SE3Tracker::trackFrame, Sim3Tracker::trackFrameSim3), an FP64 LDL^T solve
+// and exp(d) * T, and writes the pose the job's next evaluation uses.
+#include "batch_internal.h"
+#include "track_device.h"
+
+namespace lsdg {
+
+constexpr int BATCH_BS = 256;
+constexpr int BATCH_WAVES = BATCH_BS / 64;
+
+// f^n by repeated multiplication (exact for the default factor 2).
+__host__ __device__ inline float pow_int(float f, int n) {
+  float r = 1.0f;
+  for (int i = 0; i < n; ++i) r *= f;
+  return r;
+}
+
+// ------------------------------------------------------------ LM steps
+__device__ inline void set_eval_pose(JobState& s, const Pose& T, bool sim3) {
+  pose_rt_f(T, sim3, s.xf.R, s.xf.t);
+  if (sim3) pose_roll_f(T, s.roll);
+}
+
+__device__ inline void finish_job(JobState& s, int status, int* ntNext, int* remaining) {
+  s.status = status;
+  *ntNext = 0;
+  atomicSub(remaining, 1);
+}
+
+// Solve (A diag*(1+lambda)) inc = b from the stored LGS and propose Tn = exp(inc) T.
+// Returns false when a Sim3 step diverges (|inc|^2 not in [0, 1)).
+template <int KIND>
+__device__ inline bool solve_and_propose(JobState& s) {
+  constexpr int N = KIND == LSDG_JOB_SIM3 ? 7 : 6;
+  double A[N * N], b[N], inc[N];
+  double n = s.lgsN;
+  int k = 0;
+  for (int a = 0; a < N; ++a)
+    for (int c = a; c < N; ++c) {
+      double v = s.lgsA[k++] / n;
+      A[N * a + c] = v;
+      A[N * c + a] = v;
+    }
+  for (int a = 0; a < N; ++a) b[a] = -s.lgsB[a] / n;
+  for (int a = 0; a < N; ++a) A[(N + 1) * a] *= 1.0 + (double)s.lambda;
+  ldlt_solve<N>(A, b, inc);
+  s.incTry++;
+  double dd = 0;
+  for (int a = 0; a < N; ++a) {
+    s.inc[a] = inc[a];
+    dd += inc[a] * inc[a];
+  }
+  s.incSq = dd;
+  if (KIND == LSDG_JOB_SIM3) {
+    if (!(dd >= 0 && dd < 1)) return false;
+    s.Tn = pose_mul(sim3_exp(inc), s.T);
+  } else {
+    s.Tn = pose_mul(se3_exp(inc), s.T);
+  }
+  s.phase = 1;
+  set_eval_pose(s, s.Tn, KIND == LSDG_JOB_SIM3);
+  return true;
+}
+
+// Keeps the evaluation's unnormalised LGS (SE3: 6x6 upper triangle from the
+// 7-stride sums; Sim3: 7x7) as the accepted pose's normal equations.
+template <int KIND>
+__device__ inline void store_lgs(JobState& s, const double* S) {
+  if (KIND == LSDG_JOB_SIM3) {
+    for (int k = 0; k < 28; ++k) s.lgsA[k] = S[SM_A + k];
+    for (int a = 0; a < 7; ++a) s.lgsB[a] = S[SM_B + a];
+    s.lgsN = 2.0 * S[SM_WARPED];
+  } else {
+    for (int k = 0; k < 21; ++k) s.lgsA[k] = S[S3_A + k];
+    for (int a = 0; a < 6; ++a) s.lgsB[a] = S[S3_B + a];
+    s.lgsN = S[S3_WARPED];
+  }
+}
+
+template <int KIND>
+__device__ inline void begin_level(JobState& s, const JobDesc& jd, const lsdg_params& P, int lvl, int* ntNext,
+                                   int* remaining) {
+  if (KIND == LSDG_JOB_SIM3) {
+    while (lvl >= jd.finalLevel && P.max_its_per_lvl[lvl] == 0) lvl--;
+  }
+  if (lvl < jd.finalLevel) {
+    if (KIND == LSDG_JOB_SIM3 && !s.upToDate) {
+      // a final evaluation at the accepted pose for the information matrix
+      s.level = jd.finalLevel;
+      s.phase = 2;
+      set_eval_pose(s, s.T, true);
+      *ntNext = jd.lv[s.level].ntiles;
+      return;
+    }
+    if (KIND == LSDG_JOB_SIM3 && s.T.s <= 0) {
+      finish_job(s, 2, ntNext, remaining);
+      return;
+    }
+    finish_job(s, 1, ntNext, remaining);
+    return;
+  }
+  s.level = lvl;
+  s.phase = 0;
+  set_eval_pose(s, s.T, KIND == LSDG_JOB_SIM3);
+  *ntNext = jd.lv[lvl].ntiles;
+}
+
+// One LM step of an SE3 job, after the evaluation whose sums are S.
+__device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, const lsdg_params& P, int* ntNext,
+                            int* remaining) {
+  const int lvl = s.level;
+  const JobLevel& L = jd.lv[lvl];
+  s.evals++;
+  s.good = S[S3_GOOD];
+  s.bad = S[S3_BAD];
+  s.meanRes = S[S3_SIGNED] / S[S3_GOOD];
+  s.usage = S[S3_USAGE] / (double)L.pts.n;
+  {
+    double sxx = S[S3_SXX], syy = S[S3_SYY], sx = S[S3_SX], sy = S[S3_SY], sw = S[S3_SW];
+    s.affAIt = (float)sqrt((syy - sy * sy / sw) / (sxx - sx * sx / sw));
+    s.affBIt = (float)((sy - (double)s.affAIt * sx) / sw);
+  }
+  const float lim = P.min_goodperall_pixel_absmin * (float)(L.g.w * L.g.h);
+  if ((float)S[S3_WARPED] < lim) {
+    finish_job(s, 2, ntNext, remaining);
+    return;
+  }
+  const double err = S[S3_ERR] / S[S3_WARPED];
+  s.numRes[lvl]++;
+  if (s.phase == 0) {
+    if (P.use_affine_lightning_estimation) { s.affA = s.affAIt; s.affB = s.affBIt; }
+    s.lastErr = err;
+    s.lambda = P.lambda_initial[lvl];
+    s.iteration = 0;
+    if (P.max_its_per_lvl[lvl] <= 0) {
+      begin_level<LSDG_JOB_SE3>(s, jd, P, lvl - 1, ntNext, remaining);
+      return;
+    }
+    store_lgs<LSDG_JOB_SE3>(s, S);
+    s.numWarp[lvl]++;
+    s.incTry = 0;
+    solve_and_propose<LSDG_JOB_SE3>(s);
+    *ntNext = L.ntiles;
+    return;
+  }
+  if (err < s.lastErr) {
+    s.T = s.Tn;
+    if (P.use_affine_lightning_estimation) { s.affA = s.affAIt; s.affB = s.affBIt; }
+    bool conv = err / s.lastErr > (double)P.convergence_eps[lvl];
+    s.lastResidual = s.lastErr = err;
+    s.lambda = s.lambda <= 0.2f ? 0.0f : s.lambda * P.lambda_success_fac[lvl];
+    s.iteration++;
+    if (conv || s.iteration >= P.max_its_per_lvl[lvl]) {
+      begin_level<LSDG_JOB_SE3>(s, jd, P, lvl - 1, ntNext, remaining);
+      return;
+    }
+    store_lgs<LSDG_JOB_SE3>(s, S);
+    s.numWarp[lvl]++;
+    s.incTry = 0;
+    solve_and_propose<LSDG_JOB_SE3>(s);
+    *ntNext = L.ntiles;
+    return;
+  }
+  if (!(s.incSq > (double)P.step_size_min[lvl])) {
+    begin_level<LSDG_JOB_SE3>(s, jd, P, lvl - 1, ntNext, remaining);
+    return;
+  }
+  s.lambda = s.lambda == 0.0f ? 0.2f : s.lambda * pow_int(P.lambda_fail_fac[lvl], s.incTry);
+  solve_and_propose<LSDG_JOB_SE3>(s);
+  *ntNext = L.ntiles;
+}
+
+__device__ inline void sim3_res(const double* S, double* r) {
+  // r = {sumResD, sumResP, numTermsD, numTermsP, meanD, meanP, mean}
+  r[0] = S[SM_RESD];
+  r[1] = S[SM_RESP];
+  r[2] = S[SM_NTD];
+  r[3] = S[SM_WARPED];
+  r[6] = (S[SM_RESD] + S[SM_RESP]) / (S[SM_NTD] + S[SM_WARPED]);
+  r[4] = S[SM_RESD] / S[SM_NTD];
+  r[5] = S[SM_RESP] / S[SM_WARPED];
+}
+
+// One LM step of a Sim3 job.
+__device__ void lm_step_sim3(JobState& s, const JobDesc& jd, const double* S, const lsdg_params& P, int* ntNext,
+                             int* remaining) {
+  const int lvl = s.level;
+  const JobLevel& L = jd.lv[lvl];
+  s.evals++;
+  s.usage = S[SM_USAGE] / (double)L.pts.n;
+  if (s.phase == 2) {
+    // final evaluation for lastSim3Hessian
+    sim3_res(S, s.finalRes);
+    store_lgs<LSDG_JOB_SIM3>(s, S);
+    s.upToDate = 1;
+    finish_job(s, s.T.s <= 0 ? 2 : 1, ntNext, remaining);
+    return;
+  }
+  const float lim = 0.5f * P.min_goodperall_pixel_absmin * (float)(L.g.w * L.g.h);
+  if ((float)S[SM_WARPED] < lim || S[SM_WARPED] < 10) {
+    finish_job(s, 2, ntNext, remaining);
+    return;
+  }
+  double res[7];
+  sim3_res(S, res);
+  s.numRes[lvl]++;
+  if (s.phase == 0) {
+    for (int k = 0; k < 7; ++k) s.lastRes[k] = res[k];
+    s.lambda = P.lambda_initial[lvl];
+    s.upToDate = 0;
+    s.iteration = 0;
+    store_lgs<LSDG_JOB_SIM3>(s, S);
+    s.upToDate = 1;
+    s.numWarp[lvl]++;
+    s.incTry = 0;
+    if (!solve_and_propose<LSDG_JOB_SIM3>(s)) {
+      for (int k = 0; k < 28; ++k) s.lgsA[k] = 0.0;
+      finish_job(s, 2, ntNext, remaining);
+      return;
+    }
+    *ntNext = L.ntiles;
+    return;
+  }
+  if (res[6] < s.lastRes[6]) {
+    s.T = s.Tn;
+    s.upToDate = 0;
+    bool conv = res[6] / s.lastRes[6] > (double)P.convergence_eps[lvl];
+    for (int k = 0; k < 7; ++k) s.finalRes[k] = s.lastRes[k] = res[k];
+    s.lambda = s.lambda <= 0.2f ? 0.0f : s.lambda * P.lambda_success_fac[lvl];
+    s.iteration++;
+    if (conv || s.iteration >= P.max_its_per_lvl[lvl]) {
+      begin_level<LSDG_JOB_SIM3>(s, jd, P, lvl - 1, ntNext, remaining);
+      return;
+    }
+    store_lgs<LSDG_JOB_SIM3>(s, S);
+    s.upToDate = 1;
+    s.numWarp[lvl]++;
+    s.incTry = 0;
+  } else {
+    if (!(s.incSq > (double)P.step_size_min[lvl])) {
+      begin_level<LSDG_JOB_SIM3>(s, jd, P, lvl - 1, ntNext, remaining);
+      return;
+    }
+    s.lambda = s.lambda == 0.0f ? 0.2f : s.lambda * pow_int(P.lambda_fail_fac[lvl], s.incTry);
+  }
+  if (!solve_and_propose<LSDG_JOB_SIM3>(s)) {
+    for (int k = 0; k < 28; ++k) s.lgsA[k] = 0.0;
+    finish_job(s, 2, ntNext, remaining);
+    return;
+  }
+  *ntNext = L.ntiles;
+}
+
+// ------------------------------------------------------------- kernels
+// Start state of every job: T = (frameToRef_init)^-1, first level, counters
+// zero.  One block, so the remaining-jobs counter can be reset first.
+__global__ __launch_bounds__(256) void k_batch_init(BatchArgs a) {
+  if (threadIdx.x == 0) *a.remaining = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < a.njobs; j += blockDim.x) {
+    const JobDesc& jd = a.jobs[j];
+    JobState s = {};
+    s.T = pose_inv(jd.init);
+    if (jd.kind == LSDG_JOB_SE3) s.T.s = 1.0;
+    s.affA = 1.0f;
+    s.affB = 0.0f;
+    int nt = 0, dummy = 0;
+    if (jd.kind == LSDG_JOB_SIM3) begin_level<LSDG_JOB_SIM3>(s, jd, a.P, jd.startLevel, &nt, &dummy);
+    else begin_level<LSDG_JOB_SE3>(s, jd, a.P, jd.startLevel, &nt, &dummy);
+    a.st[j] = s;
+    a.ntiles[j] = nt;  // parity 0
+    a.ntiles[a.njobs + j] = 0;
+    a.counters[j] = 0u;
+    if (nt > 0) atomicAdd(a.remaining, 1);
+  }
+}
+
+template <int KIND, int PREC, int ACC>
+__global__ __launch_bounds__(BATCH_BS) void k_batch_step(BatchArgs a, int parity) {
+  constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
+  __shared__ int pref[MAX_BATCH_JOBS + 1];
+  __shared__ int scan_tmp[BATCH_BS];
+  __shared__ double sh[BATCH_WAVES][NS];
+  __shared__ double Ssum[NS];
+  __shared__ int amLast;
+  const int tid = threadIdx.x;
+  const int nj = a.njobs;
+  const int* nt = a.ntiles + parity * nj;
+  int* ntNext = a.ntiles + (1 - parity) * nj;
+  // exclusive prefix of this step's tile counts (chunked block scan)
+  {
+    const int per = (nj + BATCH_BS - 1) / BATCH_BS;
+    int base = tid * per, local = 0;
+    for (int k = 0; k < per; ++k)
+      if (base + k < nj) local += nt[base + k];
+    scan_tmp[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < BATCH_BS; off <<= 1) {
+      int v = tid >= off ? scan_tmp[tid - off] : 0;
+      __syncthreads();
+      scan_tmp[tid] += v;
+      __syncthreads();
+    }
+    int run = scan_tmp[tid] - local;
+    for (int k = 0; k < per; ++k)
+      if (base + k < nj) {
+        pref[base + k] = run;
+        run += nt[base + k];
+      }
+    if (tid == BATCH_BS - 1) pref[nj] = scan_tmp[BATCH_BS - 1];
+    __syncthreads();
+  }
+  if (blockIdx.x == 0)
+    for (int j = tid; j < nj; j += BATCH_BS)
+      if (nt[j] == 0) ntNext[j] = 0;
+  const int total = pref[nj];
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (pref[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const int j = lo;
+    const int tile = t - pref[j];
+    const JobDesc& jd = a.jobs[j];
+    JobState* st = a.st + j;
+    const int lvl = st->level;
+    const JobLevel& L = jd.lv[lvl];
+    const Xf x = st->xf;
+    TrackKParams kp = a.kp;
+    kp.aff_a = st->affA;
+    kp.aff_b = st->affB;
+    Accum<ACC, NS> acc;
+    acc.zero();
+    const int tileSize = BATCH_BS * a.ppt;
+    const int i0 = tile * tileSize;
+    const int i1 = min(i0 + tileSize, L.pts.n);
+    if (KIND == LSDG_JOB_SIM3) {
+      kp.roll[0] = st->roll[0]; kp.roll[1] = st->roll[1]; kp.roll[2] = st->roll[2]; kp.roll[3] = st->roll[3];
+      for (int i = i0 + tid; i < i1; i += BATCH_BS) sim3_point<PREC>(L.pts, i, L.img, L.fid, L.fvar, L.g, x, kp, acc);
+    } else {
+      uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
+      for (int i = i0 + tid; i < i1; i += BATCH_BS) se3_point<PREC>(L.pts, i, L.img, L.g, x, kp, mask, acc);
+    }
+    acc.wave_reduce();
+    if (lane == 0)
+      for (int k = 0; k < NS; ++k) sh[wave][k] = acc.value(k);
+    __syncthreads();
+    double* row = a.partials + ((size_t)jd.slotBase + tile) * NS;
+    if (tid < NS) {
+      double v = 0.0;
+      for (int w = 0; w < BATCH_WAVES; ++w) v += sh[w][tid];
+      row[tid] = v;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) amLast = atomicAdd(&a.counters[j], 1u) == (unsigned)(nt[j] - 1);
+    __syncthreads();
+    if (amLast) {
+      __threadfence();
+      if (tid < NS) {
+        double v = 0.0;
+        const double* rows = a.partials + (size_t)jd.slotBase * NS;
+        for (int k = 0; k < nt[j]; ++k) {
+          unsigned long long bits = __hip_atomic_load((const unsigned long long*)&rows[(size_t)k * NS + tid],
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          v += __longlong_as_double((long long)bits);
+        }
+        Ssum[tid] = v;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        JobState s = *st;
+        int next = 0;
+        if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, Ssum, a.P, &next, a.remaining);
+        else lm_step_se3(s, jd, Ssum, a.P, &next, a.remaining);
+        *st = s;
+        ntNext[j] = next;
+        __hip_atomic_store(&a.counters[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
+  k_batch_init<<<dim3(1), dim3(256), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int KIND, int PREC>
+static void step_prec(const BatchArgs& a, int parity, int grid, int acc, hipStream_t s) {
+  if (acc == BATCH_ACC_KAHAN)
+    k_batch_step<KIND, PREC, ACC_KAHAN><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
+  else
+    k_batch_step<KIND, PREC, ACC_F64><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
+}
+
+hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s) {
+  if (kind == LSDG_JOB_SIM3) {
+    if (prec == LSDG_FP16) step_prec<LSDG_JOB_SIM3, LSDG_FP16>(a, parity, grid, acc, s);
+    else if (prec == LSDG_BF16) step_prec<LSDG_JOB_SIM3, LSDG_BF16>(a, parity, grid, acc, s);
+    else step_prec<LSDG_JOB_SIM3, LSDG_FP32>(a, parity, grid, acc, s);
+  } else {
+    if (prec == LSDG_FP16) step_prec<LSDG_JOB_SE3, LSDG_FP16>(a, parity, grid, acc, s);
+    else if (prec == LSDG_BF16) step_prec<LSDG_JOB_SE3, LSDG_BF16>(a, parity, grid, acc, s);
+    else step_prec<LSDG_JOB_SE3, LSDG_FP32>(a, parity, grid, acc, s);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace lsdg

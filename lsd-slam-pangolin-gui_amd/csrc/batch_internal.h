@@ -1,0 +1,66 @@
+// Device-side layout of a batched tracking plan (jobs, per-job LM control
+// blocks) shared by batch.hip and capi_batch.cpp.
+#pragma once
+#include "lsdg_internal.h"
+
+namespace lsdg {
+
+constexpr int MAX_BATCH_JOBS = 2048;
+constexpr int BATCH_ACC_F64 = 0;
+constexpr int BATCH_ACC_KAHAN = 1;
+
+// One pyramid level of one job: the reference's points, the frame's
+// (I, gx, gy) image (FP32 float4 or packed FP16/BF16) and, for Sim3, the
+// frame's inverse depth and variance.
+struct JobLevel {
+  PointsLevel pts;
+  const void* img;
+  const float* fid;
+  const float* fvar;
+  LevelGeom g;
+  int ntiles;  // ceil(n / tile), at least 1
+};
+
+struct JobDesc {
+  JobLevel lv[PYR];
+  uint8_t* isGood;  // the frame's level-1 refPixelWasGood (SE3), written at level-1 evaluations
+  Pose init;        // frameToRef_init
+  int startLevel, finalLevel;
+  int slotBase;     // first partial-sum row of this job
+  int kind;
+};
+
+// LM control block of one job (SE3Tracker / Sim3Tracker locals, on the device).
+struct JobState {
+  Pose T, Tn;        // accepted referenceToFrame, trial
+  Xf xf;             // pose of the next evaluation, FP32
+  float roll[4];     // its ESM roll (Sim3)
+  double inc[7];
+  double incSq;
+  double lgsA[28], lgsB[7], lgsN;  // unnormalised LGS of the accepted pose
+  double lastErr, lastResidual;    // SE3
+  double lastRes[7], finalRes[7];  // Sim3 (sumResD, sumResP, numTermsD, numTermsP, meanD, meanP, mean)
+  double good, bad, meanRes, usage;
+  float lambda, affA, affB, affAIt, affBIt;
+  int status;        // 0 running, 1 done, 2 diverged
+  int level, phase, iteration, incTry, upToDate, evals;
+  int numRes[PYR], numWarp[PYR];
+};
+
+struct BatchArgs {
+  const JobDesc* jobs;
+  JobState* st;
+  int njobs;
+  int* ntiles;        // [2][njobs] tile counts of this / next step
+  unsigned* counters; // [njobs] finished tiles of this step
+  double* partials;   // rows of NS doubles
+  int* remaining;     // jobs still running
+  int ppt;            // points per thread per tile
+  lsdg_params P;
+  TrackKParams kp;
+};
+
+hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s);
+hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
+
+}  // namespace lsdg

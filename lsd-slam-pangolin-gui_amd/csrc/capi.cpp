@@ -494,10 +494,7 @@ extern "C" lsdg_status lsdg_ref_read_points(lsdg_ref r, int32_t level, float* ou
 // ---------------------------------------------------------------- tracking
 Xf lsdg::pose_xf(const Pose& T, bool scaled) {
   Xf x;
-  double M[9];
-  quat_R(T.q, M);
-  for (int i = 0; i < 9; ++i) x.R[i] = (float)(scaled ? M[i] * T.s : M[i]);
-  for (int i = 0; i < 3; ++i) x.t[i] = (float)T.t[i];
+  pose_rt_f(T, scaled, x.R, x.t);
   return x;
 }
 
@@ -515,30 +512,7 @@ TrackKParams lsdg::track_kparams(const lsdg_params& P, bool sim3) {
   return k;
 }
 
-// In-plane roll of R: rollMat = Q^T R with Q the shortest rotation taking the
-// optical axis (0,0,-1) to R (0,0,-1) (Sim3Tracker's ESM gradient rotation).
-void lsdg::esm_roll(const Pose& T, float* r4) {
-  double R[9];
-  quat_R(T.q, R);
-  double f[3] = {0, 0, -1};
-  double g[3] = {-R[2], -R[5], -R[8]};
-  double cth = f[0] * g[0] + f[1] * g[1] + f[2] * g[2];
-  double ax[3] = {f[1] * g[2] - f[2] * g[1], f[2] * g[0] - f[0] * g[2], f[0] * g[1] - f[1] * g[0]};
-  double q[4];
-  if (cth < -1.0 + 1e-12) {
-    q[0] = 1; q[1] = 0; q[2] = 0; q[3] = 0;
-  } else {
-    double s = std::sqrt((1.0 + cth) * 2.0);
-    q[0] = ax[0] / s; q[1] = ax[1] / s; q[2] = ax[2] / s; q[3] = 0.5 * s;
-  }
-  quat_normalize(q);
-  double Q[9];
-  quat_R(q, Q);
-  double M[9];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) M[3 * i + j] = Q[i] * R[j] + Q[3 + i] * R[3 + j] + Q[6 + i] * R[6 + j];
-  r4[0] = (float)M[0]; r4[1] = (float)M[1]; r4[2] = (float)M[3]; r4[3] = (float)M[4];
-}
+void lsdg::esm_roll(const Pose& T, float* r4) { pose_roll_f(T, r4); }
 
 // Algorithmic bytes of one evaluation at level l: per point xyz + I_ref + var
 // (20 B) and the isGood byte, plus the level's (I, gx, gy) image; Sim3 adds

@@ -1,0 +1,359 @@
+// C-ABI of batched tracking: lsdg_batch_create / run / destroy / track.
+//
+// A plan holds the device job table, one LM control block per job, the
+// per-tile partial-sum rows and a hipGraph of
+//     [batched pyramid rebuild] -> init -> K x k_batch_step -> copy of the
+//     remaining-jobs counter to host memory.
+// lsdg_batch_run replays it (plus K-step continuation graphs while jobs are
+// still running) and copies the control blocks back into lsdg_job_result.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "batch_internal.h"
+#include "capi_common.h"
+
+using namespace lsdg;
+
+struct lsdg_batch_s {
+  lsdg_ctx ctx = nullptr;
+  int n = 0;
+  int kind = LSDG_JOB_SE3;
+  int prec = LSDG_FP32;
+  int acc = BATCH_ACC_F64;
+  int ppt = 4;
+  int grid = 1024;
+  int steps = 16;  // LM steps per graph launch (even)
+  std::vector<lsdg_job> jobs;
+  std::vector<int> depthVersion;
+  std::vector<JobDesc> h_jobs;
+  JobDesc* d_jobs = nullptr;
+  JobState* d_st = nullptr;
+  int* d_ntiles = nullptr;
+  unsigned* d_counters = nullptr;
+  int* d_remaining = nullptr;
+  int* h_remaining = nullptr;  // pinned
+  double* d_partials = nullptr;
+  size_t rows = 0;
+  // pyramid rebuild
+  std::vector<lsdg_frame> pyrFrames;
+  void* d_pyr = nullptr;
+  void* h_pyr = nullptr;
+  PyrBatch pyr{};
+  hipGraphExec_t first = nullptr, cont = nullptr;
+  hipGraph_t g_first = nullptr, g_cont = nullptr;
+  std::vector<JobState> h_st;
+};
+
+static void batch_free(lsdg_batch b) {
+  if (b->first) (void)hipGraphExecDestroy(b->first);
+  if (b->cont) (void)hipGraphExecDestroy(b->cont);
+  if (b->g_first) (void)hipGraphDestroy(b->g_first);
+  if (b->g_cont) (void)hipGraphDestroy(b->g_cont);
+  if (b->d_jobs) (void)hipFree(b->d_jobs);
+  if (b->d_st) (void)hipFree(b->d_st);
+  if (b->d_ntiles) (void)hipFree(b->d_ntiles);
+  if (b->d_counters) (void)hipFree(b->d_counters);
+  if (b->d_remaining) (void)hipFree(b->d_remaining);
+  if (b->h_remaining) (void)hipHostFree(b->h_remaining);
+  if (b->d_partials) (void)hipFree(b->d_partials);
+  if (b->d_pyr) (void)hipFree(b->d_pyr);
+  if (b->h_pyr) (void)hipHostFree(b->h_pyr);
+  delete b;
+}
+
+// Per-job device description: point clouds of every tracked level (built if
+// needed), frame images, geometry, tile counts and partial-row slots.
+static lsdg_status build_job_table(lsdg_batch b) {
+  lsdg_ctx c = b->ctx;
+  const int tile = 256 * b->ppt;
+  b->h_jobs.assign(b->n, JobDesc{});
+  size_t slot = 0;
+  for (int j = 0; j < b->n; ++j) {
+    const lsdg_job& J = b->jobs[j];
+    lsdg_ref r = J.ref;
+    lsdg_frame f = J.frame;
+    JobDesc& d = b->h_jobs[j];
+    d.kind = J.kind;
+    d.startLevel = J.start_level;
+    d.finalLevel = J.final_level;
+    d.init = J.kind == LSDG_JOB_SIM3 ? pose_from8(J.frame_to_ref_init) : pose_from7(J.frame_to_ref_init);
+    d.isGood = J.kind == LSDG_JOB_SE3 ? f->d_wasGood : nullptr;
+    int maxTiles = 1;
+    for (int l = 0; l < PYR; ++l) {
+      JobLevel& L = d.lv[l];
+      L.g = LevelGeom{f->w[l], f->h[l], f->fx[l], f->fy[l], f->cx[l], f->cy[l]};
+      L.img = f->precision == LSDG_FP32 ? (const void*)f->d_grad[l] : (const void*)f->d_gradh[l];
+      L.fid = f->d_idepth[l];
+      L.fvar = f->d_ivar[l];
+      if (l < d.finalLevel || l > d.startLevel) {
+        L.pts = PointsLevel{nullptr, nullptr, nullptr, nullptr, 0};
+        L.ntiles = 1;
+        continue;
+      }
+      lsdg_status st = ref_points(r, l);
+      if (st != LSDG_OK) return st;
+      L.pts = r->lv[l];
+      L.ntiles = std::max(1, (L.pts.n + tile - 1) / tile);
+      maxTiles = std::max(maxTiles, L.ntiles);
+    }
+    d.slotBase = (int)slot;
+    slot += (size_t)maxTiles;
+    b->depthVersion[j] = r->kf->depthVersion;
+  }
+  const int NS = b->kind == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
+  if (slot > b->rows) {
+    if (b->d_partials) (void)hipFree(b->d_partials);
+    b->d_partials = nullptr;
+    CK(c, hipMalloc(&b->d_partials, sizeof(double) * NS * slot));
+    b->rows = slot;
+  }
+  CK(c, hipMemcpyAsync(b->d_jobs, b->h_jobs.data(), sizeof(JobDesc) * b->n, hipMemcpyHostToDevice, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return LSDG_OK;
+}
+
+static BatchArgs batch_args(lsdg_batch b) {
+  BatchArgs a;
+  a.jobs = b->d_jobs;
+  a.st = b->d_st;
+  a.njobs = b->n;
+  a.ntiles = b->d_ntiles;
+  a.counters = b->d_counters;
+  a.partials = b->d_partials;
+  a.remaining = b->d_remaining;
+  a.ppt = b->ppt;
+  a.P = b->ctx->P;
+  a.kp = track_kparams(b->ctx->P, b->kind == LSDG_JOB_SIM3);
+  return a;
+}
+
+// Captures one graph: optional pyramid rebuild and init, then `steps` LM
+// steps starting at parity 0, then the remaining-jobs counter to host memory.
+static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec_t* exec) {
+  lsdg_ctx c = b->ctx;
+  BatchArgs a = batch_args(b);
+  CK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  if (first) {
+    if (b->pyr.nf > 0) e = launch_pyr_batch(b->pyr, c->stream);
+    if (e == hipSuccess) e = launch_batch_init(a, c->stream);
+  }
+  for (int s = 0; s < b->steps && e == hipSuccess; ++s)
+    e = launch_batch_step(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->h_remaining, b->d_remaining, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(c, LSDG_EDEVICE, "batch graph capture", e != hipSuccess ? e : e2);
+  }
+  hipError_t e3 = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  if (e3 != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(c, LSDG_EDEVICE, "hipGraphInstantiate", e3);
+  }
+  *g = graph;
+  return LSDG_OK;
+}
+
+static lsdg_status batch_graphs(lsdg_batch b) {
+  if (b->first) (void)hipGraphExecDestroy(b->first);
+  if (b->cont) (void)hipGraphExecDestroy(b->cont);
+  if (b->g_first) (void)hipGraphDestroy(b->g_first);
+  if (b->g_cont) (void)hipGraphDestroy(b->g_cont);
+  b->first = b->cont = nullptr;
+  b->g_first = b->g_cont = nullptr;
+  lsdg_status st = capture(b, true, &b->g_first, &b->first);
+  if (st != LSDG_OK) return st;
+  return capture(b, false, &b->g_cont, &b->cont);
+}
+
+extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32_t n, lsdg_batch* out) {
+  if (!c || !jobs || !out || n < 1) return LSDG_EINVAL;
+  if (n > MAX_BATCH_JOBS) return fail(c, LSDG_EINVAL, "too many jobs in one batch");
+  const lsdg_params& P = c->P;
+  int kind = jobs[0].kind;
+  int prec = jobs[0].frame ? jobs[0].frame->precision : LSDG_FP32;
+  for (int j = 0; j < n; ++j) {
+    const lsdg_job& J = jobs[j];
+    if (!J.ref || !J.frame) return fail(c, LSDG_EINVAL, "job without reference or frame");
+    if (J.ref->ctx != c || J.frame->ctx != c) return fail(c, LSDG_EINVAL, "job objects belong to another context");
+    if (J.kind != kind) return fail(c, LSDG_EINVAL, "a batch holds one kind of job (SE3 or Sim3)");
+    if (J.frame->precision != prec) return fail(c, LSDG_EINVAL, "a batch holds frames of one precision");
+    if (J.frame->w[0] != J.ref->kf->w[0] || J.frame->h[0] != J.ref->kf->h[0])
+      return fail(c, LSDG_EINVAL, "frame / keyframe size mismatch");
+    if (!J.ref->kf->hasDepth) return fail(c, LSDG_ESTATE, "keyframe has no depth");
+    if (kind == LSDG_JOB_SIM3) {
+      if (!J.frame->hasDepth) return fail(c, LSDG_ESTATE, "Sim3 job needs a frame with depth");
+      if (J.start_level < 0 || J.start_level >= PYR || J.final_level < 0 || J.final_level > J.start_level)
+        return fail(c, LSDG_EINVAL, "level range");
+    } else if (kind != LSDG_JOB_SE3) {
+      return fail(c, LSDG_EINVAL, "unknown job kind");
+    }
+  }
+  lsdg_batch b = new (std::nothrow) lsdg_batch_s();
+  if (!b) return LSDG_ENOMEM;
+  b->ctx = c;
+  b->n = n;
+  b->kind = kind;
+  b->prec = prec;
+  // FP16 / BF16 pyramids run with compensated FP32 block sums
+  b->acc = prec == LSDG_FP32 ? BATCH_ACC_F64 : BATCH_ACC_KAHAN;
+  if (const char* e = std::getenv("LSDG_BATCH_ACC")) b->acc = std::atoi(e) ? BATCH_ACC_KAHAN : BATCH_ACC_F64;
+  if (const char* e = std::getenv("LSDG_BATCH_PPT")) b->ppt = std::max(1, std::min(64, std::atoi(e)));
+  if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));
+  if (const char* e = std::getenv("LSDG_BATCH_STEPS")) b->steps = std::max(2, std::min(512, std::atoi(e) & ~1));
+  b->jobs.assign(jobs, jobs + n);
+  for (auto& J : b->jobs) {
+    if (kind == LSDG_JOB_SE3) {
+      J.start_level = P.se3_max_level - 1;
+      J.final_level = P.se3_min_level;
+    }
+  }
+  b->depthVersion.assign(n, -1);
+  lsdg_status st = LSDG_OK;
+  hipError_t e;
+  if ((e = hipMalloc(&b->d_jobs, sizeof(JobDesc) * n)) != hipSuccess ||
+      (e = hipMalloc(&b->d_st, sizeof(JobState) * n)) != hipSuccess ||
+      (e = hipMalloc(&b->d_ntiles, sizeof(int) * 2 * n)) != hipSuccess ||
+      (e = hipMalloc(&b->d_counters, sizeof(unsigned) * n)) != hipSuccess ||
+      (e = hipMalloc(&b->d_remaining, 64)) != hipSuccess ||
+      (e = hipHostMalloc(&b->h_remaining, 64, hipHostMallocDefault)) != hipSuccess) {
+    batch_free(b);
+    return fail(c, LSDG_ENOMEM, "batch allocation", e);
+  }
+  // frames whose pyramid the batch rebuilds
+  for (auto& J : b->jobs)
+    if (J.rebuild_pyramid && std::find(b->pyrFrames.begin(), b->pyrFrames.end(), J.frame) == b->pyrFrames.end())
+      b->pyrFrames.push_back(J.frame);
+  if (!b->pyrFrames.empty()) {
+    size_t bytes = pyr_batch_bytes((int)b->pyrFrames.size());
+    if ((e = hipMalloc(&b->d_pyr, bytes)) != hipSuccess || (e = hipHostMalloc(&b->h_pyr, bytes)) != hipSuccess) {
+      batch_free(b);
+      return fail(c, LSDG_ENOMEM, "batch pyramid tables", e);
+    }
+    pyr_batch_tables(b->pyrFrames.data(), (int)b->pyrFrames.size(), b->h_pyr, b->d_pyr, &b->pyr);
+    if ((e = hipMemcpy(b->d_pyr, b->h_pyr, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+      batch_free(b);
+      return fail(c, LSDG_EDEVICE, "batch pyramid tables", e);
+    }
+  }
+  for (auto& J : b->jobs)
+    if (kind == LSDG_JOB_SE3 && (st = frame_ensure_was_good(J.frame)) != LSDG_OK) {
+      batch_free(b);
+      return st;
+    }
+  if ((st = build_job_table(b)) != LSDG_OK || (st = batch_graphs(b)) != LSDG_OK) {
+    batch_free(b);
+    return st;
+  }
+  b->h_st.resize(n);
+  *out = b;
+  return LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_batch_destroy(lsdg_batch b) {
+  if (!b) return LSDG_EINVAL;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_free(b);
+  return LSDG_OK;
+}
+
+static void fill_result(lsdg_batch b, int j, const JobState& s, lsdg_job_result* r) {
+  const lsdg_job& J = b->jobs[j];
+  std::memset(r, 0, sizeof(*r));
+  lsdg_track_stats& st = r->stats;
+  for (int l = 0; l < PYR; ++l) {
+    st.num_residual_calls[l] = s.numRes[l];
+    st.num_warp_update_calls[l] = s.numWarp[l];
+  }
+  st.point_usage = (float)s.usage;
+  st.affine_a = s.affA;
+  st.affine_b = s.affB;
+  Pose I = pose_identity();
+  if (s.status == 2) {
+    st.diverged = 1;
+    st.tracking_was_good = 0;
+    st.last_good_count = (int)s.good;
+    st.last_bad_count = (int)s.bad;
+    pose_to8(I, r->frame_to_ref);
+    r->status = LSDG_DIVERGED;
+    if (b->kind == LSDG_JOB_SIM3) std::memset(st.hessian, 0, sizeof st.hessian);
+    return;
+  }
+  Pose res = pose_inv(s.T);
+  if (b->kind == LSDG_JOB_SE3) {
+    res.s = 1.0;
+    st.last_good_count = (int)s.good;
+    st.last_bad_count = (int)s.bad;
+    st.last_mean_res = (float)s.meanRes;
+    st.last_residual = (float)s.lastResidual;
+    lsdg_frame f = J.frame;
+    const int ml = b->ctx->P.se3_min_level;
+    bool good = s.good / (double)(f->w[ml] * f->h[ml]) > (double)b->ctx->P.min_goodperall_pixel &&
+                s.good / (s.good + s.bad) > (double)b->ctx->P.min_goodpergoodbad_pixel;
+    st.tracking_was_good = good ? 1 : 0;
+    // the frame fields SE3Tracker::trackFrame writes
+    if (good) J.ref->kf->numFramesTrackedOnThis++;
+    f->initialTrackedResidual = (float)(s.lastResidual / s.usage);
+    f->thisToParent = res;
+    f->parent = J.ref->kf;
+    f->parentId = J.ref->kf->id;
+  } else {
+    st.last_residual = (float)s.finalRes[6];
+    st.last_depth_residual = (float)s.finalRes[4];
+    st.last_photometric_residual = (float)s.finalRes[5];
+    st.tracking_was_good = 1;
+    int k = 0;
+    for (int a = 0; a < 7; ++a)
+      for (int cc = a; cc < 7; ++cc) {
+        st.hessian[7 * a + cc] = s.lgsA[k];
+        st.hessian[7 * cc + a] = s.lgsA[k];
+        ++k;
+      }
+  }
+  pose_to8(res, r->frame_to_ref);
+  r->status = LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
+  if (!b) return LSDG_EINVAL;
+  lsdg_ctx c = b->ctx;
+  // keyframes whose depth changed since the plan was built need new point clouds
+  bool stale = false;
+  for (int j = 0; j < b->n; ++j) stale |= b->jobs[j].ref->kf->depthVersion != b->depthVersion[j];
+  if (stale) {
+    lsdg_status st = build_job_table(b);
+    if (st == LSDG_OK) st = batch_graphs(b);
+    if (st != LSDG_OK) return st;
+  }
+  ProfMark pm = prof_begin(c, LSDG_KERNEL_BATCH_EVAL);
+  CK(c, hipGraphLaunch(b->first, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  int launches = 0;
+  while (*b->h_remaining > 0) {
+    if (++launches > 64) return fail(c, LSDG_ESTATE, "batched LM did not finish within 64 continuation graphs");
+    CK(c, hipGraphLaunch(b->cont, c->stream));
+    CK(c, hipStreamSynchronize(c->stream));
+  }
+  prof_end(c, pm, 0.0);
+  if (results) {
+    CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
+    for (int j = 0; j < b->n; ++j) fill_result(b, j, b->h_st[j], &results[j]);
+  }
+  return LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_batch_track(lsdg_ctx c, const lsdg_job* jobs, int32_t n, lsdg_job_result* results) {
+  lsdg_batch b = nullptr;
+  lsdg_status st = lsdg_batch_create(c, jobs, n, &b);
+  if (st != LSDG_OK) return st;
+  st = lsdg_batch_run(b, results);
+  lsdg_batch_destroy(b);
+  return st;
+}

@@ -82,6 +82,16 @@ struct DepthGeom {
   float fx, fy, cx, cy, fxi, fyi, cxi, cyi;
 };
 
+// Device tables of a batched pyramid rebuild.
+struct PyrBatch {
+  const void* img_args;
+  const void* grad_args;
+  const int* img_off;
+  const int* img_tx;
+  const int* grad_off;
+  int img_blocks, grad_blocks, nf;
+};
+
 struct PointsLevel {
   const float4* posI;  // x, y, z, I
   const float* var;
@@ -216,6 +226,9 @@ hipError_t launch_pyr_grads(lsdg_frame f, hipStream_t s);
 hipError_t launch_pyr_depth(lsdg_frame f, hipStream_t s);
 hipError_t launch_depth_normalize(float* idepth, float* ivar, int n, hipStream_t s);
 hipError_t launch_point_cloud(lsdg_ref r, int level, hipStream_t s);  // fills r->lv[level], r->num[level]
+size_t pyr_batch_bytes(int nf);
+void pyr_batch_tables(lsdg_frame const* fs, int nf, void* h, void* d, PyrBatch* out);
+hipError_t launch_pyr_batch(const PyrBatch& b, hipStream_t s);
 
 // track.hip
 hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,

@@ -178,6 +178,41 @@ LSDG_HD Pose sim3_exp(const double* xi) {
   return r;
 }
 
+// FP32 rotation (scaled by s for Sim3) and translation of a pose, as the
+// per-point kernels consume them.
+LSDG_HD void pose_rt_f(const Pose& T, bool scaled, float* R, float* t) {
+  double M[9];
+  quat_R(T.q, M);
+  for (int i = 0; i < 9; ++i) R[i] = (float)(scaled ? M[i] * T.s : M[i]);
+  for (int i = 0; i < 3; ++i) t[i] = (float)T.t[i];
+}
+
+// In-plane roll of R for Sim3Tracker's ESM gradient rotation: rollMat = Q^T R,
+// Q the shortest rotation taking the optical axis f = (0,0,-1) to R f;
+// r4 = (roll00, roll01, roll10, roll11).
+LSDG_HD void pose_roll_f(const Pose& T, float* r4) {
+  double R[9];
+  quat_R(T.q, R);
+  const double f[3] = {0.0, 0.0, -1.0};
+  const double g[3] = {-R[2], -R[5], -R[8]};
+  double cth = f[0] * g[0] + f[1] * g[1] + f[2] * g[2];
+  double ax[3] = {f[1] * g[2] - f[2] * g[1], f[2] * g[0] - f[0] * g[2], f[0] * g[1] - f[1] * g[0]};
+  double q[4];
+  if (cth < -1.0 + 1e-12) {
+    q[0] = 1.0; q[1] = 0.0; q[2] = 0.0; q[3] = 0.0;
+  } else {
+    double s = sqrt((1.0 + cth) * 2.0);
+    q[0] = ax[0] / s; q[1] = ax[1] / s; q[2] = ax[2] / s; q[3] = 0.5 * s;
+  }
+  quat_normalize(q);
+  double Q[9];
+  quat_R(q, Q);
+  double M[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) M[3 * i + j] = Q[i] * R[j] + Q[3 + i] * R[3 + j] + Q[6 + i] * R[6 + j];
+  r4[0] = (float)M[0]; r4[1] = (float)M[1]; r4[2] = (float)M[3]; r4[3] = (float)M[4];
+}
+
 // LDL^T solve of a symmetric N x N system (row-major); zero pivots give zero
 // solution components.
 template <int N>

@@ -20,12 +20,11 @@ struct PyrImgArgs {
   int w[PYR], h[PYR];
 };
 
-__global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, const uint8_t* __restrict__ u8) {
+__device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __restrict__ u8, int bx, int by) {
   __shared__ float L1[16][17];
   __shared__ float L2[8][9];
   __shared__ float L3[4][5];
   const int t = threadIdx.x;
-  const int bx = blockIdx.x, by = blockIdx.y;
   const int w0 = p.w[0], h0 = p.h[0];
   if (u8) {
     for (int k = t; k < 1024; k += 256) {
@@ -85,6 +84,10 @@ __global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, const uint8_t*
   }
 }
 
+__global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, const uint8_t* __restrict__ u8) {
+  pyr_images_tile(p, u8, blockIdx.x, blockIdx.y);
+}
+
 struct PyrGradArgs {
   const float* img[PYR];
   float4* grad[PYR];
@@ -131,8 +134,7 @@ __device__ inline uint2 pack_grad(float4 g, int prec) {
   return make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c);
 }
 
-__global__ __launch_bounds__(256) void k_pyr_grads(PyrGradArgs p) {
-  int b = blockIdx.x;
+__device__ inline void pyr_grads_block(const PyrGradArgs& p, int b) {
   int l = 0;
   while (l < PYR - 1 && b >= p.blk0[l + 1]) l++;
   int idx = (b - p.blk0[l]) * 256 + threadIdx.x;
@@ -157,6 +159,34 @@ __global__ __launch_bounds__(256) void k_pyr_grads(PyrGradArgs p) {
     m = absgrad_at(I, idx, w, h);
   }
   p.maxg[l][idx] = m;
+}
+
+__global__ __launch_bounds__(256) void k_pyr_grads(PyrGradArgs p) { pyr_grads_block(p, blockIdx.x); }
+
+// Batched pyramids: every frame of a batch in one launch per stage.  Block b
+// belongs to the frame f with off[f] <= b < off[f + 1].
+__device__ inline int frame_of_block(const int* __restrict__ off, int nf, int b) {
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_pyr_images_batch(const PyrImgArgs* __restrict__ args,
+                                                          const int* __restrict__ off, const int* __restrict__ tx,
+                                                          int nf) {
+  int f = frame_of_block(off, nf, blockIdx.x);
+  int local = blockIdx.x - off[f];
+  pyr_images_tile(args[f], nullptr, local % tx[f], local / tx[f]);
+}
+
+__global__ __launch_bounds__(256) void k_pyr_grads_batch(const PyrGradArgs* __restrict__ args,
+                                                         const int* __restrict__ off, int nf) {
+  int f = frame_of_block(off, nf, blockIdx.x);
+  pyr_grads_block(args[f], blockIdx.x - off[f]);
 }
 
 struct PyrDepthArgs {
@@ -368,6 +398,71 @@ hipError_t launch_pyr_grads(lsdg_frame f, hipStream_t s) {
   p.blk0[PYR] = blk;
   p.prec = f->precision;
   hipLaunchKernelGGL(k_pyr_grads, dim3(blk), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+// Host tables for the batched launches (filled once per batch plan).
+size_t pyr_batch_bytes(int nf) {
+  return (sizeof(PyrImgArgs) + sizeof(PyrGradArgs)) * (size_t)nf + sizeof(int) * (size_t)(3 * nf + 2) + 64;
+}
+
+// Writes the argument tables for frames fs[0..nf) into host memory `h` (laid
+// out as the device copy at `d`), returns the block totals of both stages.
+void pyr_batch_tables(lsdg_frame const* fs, int nf, void* h, void* d, PyrBatch* out) {
+  char* hb = (char*)h;
+  PyrImgArgs* ia = (PyrImgArgs*)hb;
+  PyrGradArgs* ga = (PyrGradArgs*)(hb + sizeof(PyrImgArgs) * nf);
+  int* ioff = (int*)(hb + (sizeof(PyrImgArgs) + sizeof(PyrGradArgs)) * nf);
+  int* itx = ioff + nf + 1;
+  int* goff = itx + nf;
+  int ib = 0, gb = 0;
+  for (int k = 0; k < nf; ++k) {
+    lsdg_frame f = fs[k];
+    PyrImgArgs p;
+    PyrGradArgs q;
+    int blk = 0;
+    for (int l = 0; l < PYR; ++l) {
+      p.img[l] = f->d_image[l];
+      p.w[l] = f->w[l];
+      p.h[l] = f->h[l];
+      q.img[l] = f->d_image[l];
+      q.grad[l] = f->d_grad[l];
+      q.gradh[l] = f->d_gradh[l];
+      q.maxg[l] = f->d_maxg[l];
+      q.w[l] = f->w[l];
+      q.h[l] = f->h[l];
+      q.blk0[l] = blk;
+      blk += (f->w[l] * f->h[l] + 255) / 256;
+    }
+    q.blk0[PYR] = blk;
+    q.prec = f->precision;
+    ia[k] = p;
+    ga[k] = q;
+    int tx = (f->w[0] + 31) / 32, ty = (f->h[0] + 31) / 32;
+    ioff[k] = ib;
+    itx[k] = tx;
+    ib += tx * ty;
+    goff[k] = gb;
+    gb += blk;
+  }
+  ioff[nf] = ib;
+  goff[nf] = gb;
+  char* db = (char*)d;
+  out->img_args = db;
+  out->grad_args = db + sizeof(PyrImgArgs) * nf;
+  out->img_off = (const int*)(db + (sizeof(PyrImgArgs) + sizeof(PyrGradArgs)) * nf);
+  out->img_tx = out->img_off + nf + 1;
+  out->grad_off = out->img_tx + nf;
+  out->img_blocks = ib;
+  out->grad_blocks = gb;
+  out->nf = nf;
+}
+
+hipError_t launch_pyr_batch(const PyrBatch& b, hipStream_t s) {
+  if (b.nf == 0) return hipSuccess;
+  k_pyr_images_batch<<<dim3(b.img_blocks), dim3(256), 0, s>>>((const PyrImgArgs*)b.img_args, b.img_off, b.img_tx,
+                                                               b.nf);
+  k_pyr_grads_batch<<<dim3(b.grad_blocks), dim3(256), 0, s>>>((const PyrGradArgs*)b.grad_args, b.grad_off, b.nf);
   return hipGetLastError();
 }
 

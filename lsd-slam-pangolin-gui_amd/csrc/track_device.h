@@ -1,0 +1,340 @@
+// Per-point tracker math shared by the plain (one alignment per launch) and
+// batched (many alignments per launch) evaluation kernels: bilinear taps of the
+// (I, gx, gy) pyramid in FP32 / FP16 / BF16, the Jacobian row, the per-thread
+// accumulators, and one function per point for SE3, Sim3 and the adjoint.
+// The float expressions keep the reference's operation order (SE3Tracker::
+// calcResidualAndBuffers / calcWeightsAndResidual / calculateWarpUpdate and the
+// Sim3Tracker counterparts) so masks and counts match the CPU restatement.
+#pragma once
+#include "lsdg_internal.h"
+
+namespace lsdg {
+
+constexpr int EVAL_BS = 256;
+constexpr int EVAL_WAVES = EVAL_BS / 64;
+
+struct Taps {
+  float gx, gy, I;
+};
+
+__device__ inline float bf16_to_f32(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+template <int PREC>
+__device__ inline Taps load_tap(const void* img, int i) {
+  if constexpr (PREC == LSDG_FP32) {
+    float4 g = ((const float4*)img)[i];
+    return {g.x, g.y, g.z};
+  } else {
+    uint2 g = ((const uint2*)img)[i];
+    if constexpr (PREC == LSDG_FP16) {
+      return {__half2float(__ushort_as_half((unsigned short)(g.x & 0xFFFFu))),
+              __half2float(__ushort_as_half((unsigned short)(g.x >> 16))),
+              __half2float(__ushort_as_half((unsigned short)(g.y & 0xFFFFu)))};
+    } else {
+      return {bf16_to_f32((unsigned short)(g.x & 0xFFFFu)), bf16_to_f32((unsigned short)(g.x >> 16)),
+              bf16_to_f32((unsigned short)(g.y & 0xFFFFu))};
+    }
+  }
+}
+
+// getInterpolatedElement43: weights dxdy, dy-dxdy, dx-dxdy, 1-dx-dy+dxdy.
+template <int PREC>
+__device__ inline Taps interp3(const void* img, float x, float y, int w) {
+  int ix = (int)x, iy = (int)y;
+  float dx = x - (float)ix, dy = y - (float)iy;
+  float dxdy = dx * dy;
+  int b = ix + iy * w;
+  Taps a11 = load_tap<PREC>(img, b + 1 + w), a01 = load_tap<PREC>(img, b + w);
+  Taps a10 = load_tap<PREC>(img, b + 1), a00 = load_tap<PREC>(img, b);
+  float w11 = dxdy, w01 = dy - dxdy, w10 = dx - dxdy, w00 = 1.0f - dx - dy + dxdy;
+  Taps o;
+  o.gx = w11 * a11.gx + w01 * a01.gx + w10 * a10.gx + w00 * a00.gx;
+  o.gy = w11 * a11.gy + w01 * a01.gy + w10 * a10.gy + w00 * a00.gy;
+  o.I = w11 * a11.I + w01 * a01.I + w10 * a10.I + w00 * a00.I;
+  return o;
+}
+
+// -v = dr/dxi of the photometric residual (SE3Tracker::calculateWarpUpdate).
+__device__ inline void jac6(float px, float py, float pz, float gx, float gy, float* v) {
+  float z = 1.0f / pz;
+  float z_sqr = 1.0f / (pz * pz);
+  v[0] = z * gx;
+  v[1] = z * gy;
+  v[2] = (-px * z_sqr) * gx + (-py * z_sqr) * gy;
+  v[3] = (-px * py * z_sqr) * gx + (-(1.0f + py * py * z_sqr)) * gy;
+  v[4] = (1.0f + px * px * z_sqr) * gx + (px * py * z_sqr) * gy;
+  v[5] = (-py * z) * gx + (px * z) * gy;
+}
+
+// Per-thread accumulators of the per-point FP32 terms.
+//  ACC_F64:   each FP32 term is widened and summed in FP64 (the parity mode:
+//             the same terms as the CPU restatement, summed to ~1e-16).
+//  ACC_KAHAN: FP32 sums with an error-free TwoSum compensation term; the wave
+//             and block trees combine (hi, lo) pairs in FP32, and FP64 is used
+//             only when the block's pair is written out for the cross-block merge.
+constexpr int ACC_F64 = 0;
+constexpr int ACC_KAHAN = 1;
+
+template <int MODE, int NS>
+struct Accum;
+
+template <int NS>
+struct Accum<ACC_F64, NS> {
+  double v[NS];
+  __device__ inline void zero() {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) v[k] = 0.0;
+  }
+  __device__ inline void add(int k, float x) { v[k] += (double)x; }
+  __device__ inline void wave_reduce() {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      double a = v[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+      v[k] = a;
+    }
+  }
+  __device__ inline double value(int k) const { return v[k]; }
+};
+
+// Error-free sum of two floats: hi + lo == a + b exactly (round-to-nearest,
+// no contraction).
+__device__ inline void two_sum(float a, float b, float& hi, float& lo) {
+  hi = a + b;
+  float bp = hi - a;
+  lo = (a - (hi - bp)) + (b - bp);
+}
+
+template <int NS>
+struct Accum<ACC_KAHAN, NS> {
+  float s[NS], c[NS];
+  __device__ inline void zero() {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = c[k] = 0.f;
+  }
+  __device__ inline void add(int k, float x) {
+    float hi, lo;
+    two_sum(s[k], x, hi, lo);
+    s[k] = hi;
+    c[k] += lo;
+  }
+  __device__ inline void wave_reduce() {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      float a = s[k], b = c[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        float a2 = __shfl_xor(a, off), b2 = __shfl_xor(b, off);
+        float hi, lo;
+        two_sum(a, a2, hi, lo);
+        a = hi;
+        b = b + b2 + lo;
+      }
+      s[k] = a;
+      c[k] = b;
+    }
+  }
+  __device__ inline double value(int k) const { return (double)s[k] + (double)c[k]; }
+};
+
+// Index of (i, j), i <= j, in the row-major upper triangle of a 7x7 matrix.
+__host__ __device__ constexpr int tri7(int i, int j) { return i * 7 - i * (i - 1) / 2 + (j - i); }
+
+// One reference point of an SE3 evaluation: warp, bounds, taps, residual,
+// isGood, affine statistics, weights, Jacobian and normal-equation terms.
+template <int PREC, class ACC>
+__device__ inline void se3_point(const PointsLevel& pts, int i, const void* __restrict__ img, const LevelGeom& g,
+                                 const Xf& x, const TrackKParams& kp, uint8_t* __restrict__ isGood, ACC& acc) {
+  const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
+  float4 p = pts.posI[i];
+  float Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
+  float Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
+  float Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
+  float u = (Wx / Wz) * g.fx + g.cx;
+  float v = (Wy / Wz) * g.fy + g.cy;
+  if (!(u > 1.0f && v > 1.0f && u < wlim && v < hlim)) {
+    if (isGood) isGood[pts.idx[i]] = 0;
+    return;
+  }
+  Taps ri = interp3<PREC>(img, u, v, g.w);
+  float c1 = kp.aff_a * p.w + kp.aff_b;
+  float c2 = ri.I;
+  float r = c1 - c2;
+  float wgt = fabsf(r) < 5.0f ? 1.0f : 5.0f / fabsf(r);
+  acc.add(S3_SXX, c1 * c1 * wgt);
+  acc.add(S3_SYY, c2 * c2 * wgt);
+  acc.add(S3_SX, c1 * wgt);
+  acc.add(S3_SY, c2 * wgt);
+  acc.add(S3_SW, wgt);
+  bool good = r * r / (kp.max_diff_constant + kp.max_diff_grad_mult * (ri.gx * ri.gx + ri.gy * ri.gy)) < 1.0f;
+  if (isGood) isGood[pts.idx[i]] = good ? 1 : 0;
+  float gx = g.fx * ri.gx, gy = g.fy * ri.gy;
+  float d = 1.0f / p.z;
+  if (good) {
+    acc.add(S3_RESUNW, r * r);
+    acc.add(S3_SIGNED, r);
+    acc.add(S3_GOOD, 1.0f);
+  } else {
+    acc.add(S3_BAD, 1.0f);
+  }
+  float dc = p.z / Wz;
+  acc.add(S3_USAGE, dc < 1.0f ? dc : 1.0f);
+  acc.add(S3_WARPED, 1.0f);
+  // calcWeightsAndResidual
+  float s = kp.var_weight * pts.var[i];
+  float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
+  float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
+  float drpdd = gx * g0 + gy * g1;
+  float w_p = 1.0f / (kp.noise2 + s * drpdd * drpdd);
+  float wr = fabsf(r * sqrtf(w_p));
+  float wh = fabsf(wr < kp.huber ? 1.0f : kp.huber / wr);
+  float W = wh * w_p;
+  // calculateWarpUpdate
+  float jv[6];
+  jac6(Wx, Wy, Wz, gx, gy, jv);
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int c = a; c < 6; ++c) acc.add(S3_A + (k++), jv[a] * jv[c] * W);
+  float rW = r * W;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) acc.add(S3_B + a, -(jv[a] * rW));
+  acc.add(S3_ERR, r * r * W);
+}
+
+// One reference point of a Sim3 evaluation (photometric + depth residual).
+template <int PREC, class ACC>
+__device__ inline void sim3_point(const PointsLevel& pts, int i, const void* __restrict__ img,
+                                  const float* __restrict__ fid, const float* __restrict__ fvar, const LevelGeom& g,
+                                  const Xf& x, const TrackKParams& kp, ACC& acc) {
+  const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
+  float4 p = pts.posI[i];
+  float Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
+  float Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
+  float Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
+  float u = (Wx / Wz) * g.fx + g.cx;
+  float v = (Wy / Wz) * g.fy + g.cy;
+  if (!(u > 1.0f && v > 1.0f && u < wlim && v < hlim)) return;
+  Taps ri = interp3<PREC>(img, u, v, g.w);
+  float gx, gy;
+  if (kp.use_esm) {
+    float2 rg = pts.grad[i];
+    float rgx = kp.roll[0] * rg.x + kp.roll[1] * rg.y;
+    float rgy = kp.roll[2] * rg.x + kp.roll[3] * rg.y;
+    gx = g.fx * 0.5f * (ri.gx + rgx);
+    gy = g.fy * 0.5f * (ri.gy + rgy);
+  } else {
+    gx = g.fx * ri.gx;
+    gy = g.fy * ri.gy;
+  }
+  float c1 = kp.aff_a * p.w + kp.aff_b;
+  float c2 = ri.I;
+  float rp = c1 - c2;
+  float wgt = fabsf(rp) < 2.0f ? 1.0f : 2.0f / fabsf(rp);
+  acc.add(SM_SXX, c1 * c1 * wgt);
+  acc.add(SM_SYY, c2 * c2 * wgt);
+  acc.add(SM_SX, c1 * wgt);
+  acc.add(SM_SY, c2 * wgt);
+  acc.add(SM_SW, wgt);
+  int ir = (int)(u + 0.5f) + g.w * (int)(v + 0.5f);
+  float varF = fvar[ir];
+  float refId = 1.0f / Wz;
+  float d = 1.0f / p.z;
+  float rd, vard;
+  if (varF > 0.0f) {
+    rd = refId - fid[ir];
+    vard = varF;
+  } else {
+    rd = -1.0f;
+    vard = -1.0f;
+  }
+  float dc = p.z / Wz;
+  acc.add(SM_USAGE, dc < 1.0f ? dc : 1.0f);
+  acc.add(SM_WARPED, 1.0f);
+  // calcSim3WeightsAndResidual
+  float s = kp.var_weight * pts.var[i];
+  float sv = kp.var_weight * vard;
+  float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
+  float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
+  float g2 = (Wz - x.t[2]) / (Wz * Wz * d);
+  float drpdd = gx * g0 + gy * g1;
+  float w_p = 1.0f / (kp.noise2 + s * drpdd * drpdd);
+  float w_d = 1.0f / (sv + g2 * g2 * s);
+  float wrd = fabsf(rd * sqrtf(w_d));
+  float wrp = fabsf(rp * sqrtf(w_p));
+  float wabs = sv > 0.0f ? wrd + wrp : wrp;
+  float wh = fabsf(wabs < kp.huber ? 1.0f : kp.huber / wabs);
+  if (sv > 0.0f) {
+    acc.add(SM_RESD, wh * w_d * rd * rd);
+    acc.add(SM_NTD, 1.0f);
+  }
+  acc.add(SM_RESP, wh * w_p * rp * rp);
+  float wp = wh * w_p;
+  float wd = sv > 0.0f ? wh * w_d : 0.0f;
+  // calcSim3LGS: LGS6 over (v, w) and LGS4 over (v3, w1, w2, sigma), merged
+  // into the 7x7 upper triangle at rows / columns {2, 3, 4, 6}.
+  float jv[6];
+  jac6(Wx, Wy, Wz, gx, gy, jv);
+  float z = 1.0f / Wz;
+  float z_sqr = 1.0f / (Wz * Wz);
+  float v4[4] = {z_sqr, z_sqr * Wy, -z_sqr * Wx, z};
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int c = a; c < 6; ++c) acc.add(SM_A + tri7(a, c), jv[a] * jv[c] * wp);
+  float rpW = rp * wp;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) acc.add(SM_B + a, -(jv[a] * rpW));
+  if (wd != 0.0f) {
+    const int remap[4] = {2, 3, 4, 6};
+    float rdW = rd * wd;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+      for (int c = a; c < 4; ++c) acc.add(SM_A + tri7(remap[a], remap[c]), v4[a] * v4[c] * wd);
+      acc.add(SM_B + remap[a], -(v4[a] * rdW));
+    }
+  }
+}
+
+// One reference point of the adjoint: dE/dxi into acc[0..6), E into acc[6],
+// count into acc[7], dE/d(idepth) into grad_d[i].
+template <int PREC, class ACC>
+__device__ inline void adjoint_point(const PointsLevel& pts, int i, const void* __restrict__ img, const LevelGeom& g,
+                                     const Xf& x, const TrackKParams& kp, float* __restrict__ grad_d, ACC& acc) {
+  const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
+  float4 p = pts.posI[i];
+  float Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
+  float Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
+  float Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
+  float u = (Wx / Wz) * g.fx + g.cx;
+  float v = (Wy / Wz) * g.fy + g.cy;
+  if (!(u > 1.0f && v > 1.0f && u < wlim && v < hlim)) {
+    grad_d[i] = 0.0f;
+    return;
+  }
+  Taps ri = interp3<PREC>(img, u, v, g.w);
+  float r = p.w - ri.I;
+  float gx = g.fx * ri.gx, gy = g.fy * ri.gy;
+  float d = 1.0f / p.z;
+  float s = kp.var_weight * pts.var[i];
+  float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
+  float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
+  float drpdd = gx * g0 + gy * g1;
+  float w_p = 1.0f / (kp.noise2 + s * drpdd * drpdd);
+  float e = fabsf(r * sqrtf(w_p));
+  float k = kp.huber;
+  float wh = e < k ? 1.0f : k / e;
+  acc.add(6, e < k ? 0.5f * e * e : k * e - 0.5f * k * k);
+  acc.add(7, 1.0f);
+  float Wr = wh * w_p * r;
+  float jv[6];
+  jac6(Wx, Wy, Wz, gx, gy, jv);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) acc.add(a, -(Wr * jv[a]));
+  grad_d[i] = -Wr * drpdd;
+}
+
+}  // namespace lsdg

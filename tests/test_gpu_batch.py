@@ -1,0 +1,114 @@
+"""Batched tracking (one persistent launch per LM step, hipGraph-replayed)
+against the plain path, per instance.
+
+Bars: FP32 pyramids with FP64 sums take the plain path's LM path (poses within
+1e-9 twist, identical call counts); FP16 pyramids with compensated FP32 block
+sums stay within 1e-5 of the plain FP16 path; replays are bit-identical."""
+import numpy as np
+import pytest
+
+import lsdg
+import posemath
+import synth
+from helpers import tracking_params, twist
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(n, seed, resolutions):
+    return synth.batch_instances(n, seed=seed, resolutions=resolutions)
+
+
+def _setup(ctx, insts, precision):
+    items = []
+    for i, inst in enumerate(insts):
+        kf = ctx.frame(2 * i, inst["cam"], inst["kf_img"], precision)
+        kf.set_depth(inst["idepth"], inst["var"])
+        fr = ctx.frame(2 * i + 1, inst["cam"], inst["fr_img"], precision)
+        items.append((kf, fr, ctx.ref(kf)))
+    return items
+
+
+def _jobs(items, kind=lsdg.JOB_SE3, init=None, rebuild=1, levels=(4, 1)):
+    jobs = []
+    for i, (kf, fr, ref) in enumerate(items):
+        p = init[i] if init is not None else list(synth.IDENTITY7) + [1.0]
+        jobs.append(lsdg.Job(ref.h, fr.h, lsdg.pose8(p), kind, levels[0], levels[1], rebuild))
+    return jobs
+
+
+@pytest.mark.parametrize("precision", [lsdg.FP32, lsdg.FP16])
+def test_batch_se3_matches_plain(ctx, precision):
+    insts = _instances(12, 2031, [(320, 240), (640, 480)])
+    ctx.set_params(tracking_params(20))
+    try:
+        items = _setup(ctx, insts, precision)
+        plain = [ctx.se3_track(ref, fr, synth.IDENTITY7) for kf, fr, ref in items]
+        b = ctx.batch(_jobs(items))
+        r1 = b.run()
+        r2 = b.run()
+        b.destroy()
+    finally:
+        ctx.set_params(lsdg.default_params())
+    for i, (pose, st, status) in enumerate(plain):
+        res = r1[i]
+        assert res.status == status, i
+        e = twist(res.frame_to_ref[:7], pose)
+        if precision == lsdg.FP32:
+            assert e <= 1e-9, (i, e)
+            assert list(res.stats.num_residual_calls) == list(st.num_residual_calls), i
+            assert res.stats.last_good_count == st.last_good_count
+        else:
+            assert e <= 1e-5, (i, e)
+        assert twist(res.frame_to_ref[:7], insts[i]["gt"]) < 2e-2
+        # replay of the same graph is bit-identical
+        assert list(r2[i].frame_to_ref) == list(res.frame_to_ref)
+
+
+def test_batch_divergent_job(ctx):
+    insts = _instances(3, 2077, [(320, 240)])
+    items = _setup(ctx, insts, lsdg.FP32)
+    init = [list(synth.IDENTITY7) + [1.0]] * 3
+    init[1] = [0, 0, 0, 1, 50.0, 0, 0, 1.0]
+    b = ctx.batch(_jobs(items, init=init))
+    r = b.run()
+    b.destroy()
+    assert r[1].status == lsdg.DIVERGED and r[1].stats.diverged == 1
+    assert np.allclose(r[1].frame_to_ref[:7], synth.IDENTITY7)
+    assert r[0].status == lsdg.OK and r[2].status == lsdg.OK
+
+
+def test_batch_sim3_matches_plain(ctx):
+    cam = synth.camera(320, 240)
+    items, inits = [], []
+    params = tracking_params(100)
+    for l in range(5):
+        params.convergence_eps[l] = 0.9999
+    ctx.set_params(params)
+    try:
+        for i in range(6):
+            sc = synth.Scene(3100 + i)
+            kf_img, kf_dep = sc.render(synth.IDENTITY7, cam)
+            pose = synth.sample_pose(3200 + i, 0.02, 1.0)
+            fr_img, fr_dep = sc.render(pose, cam, 2.0, 3300 + i)
+            s = 0.95 + 0.02 * i
+            idk, vk, _, _ = synth.semi_dense_depth(kf_img, kf_dep, 3400 + i, fraction=0.3)
+            idf, vf, _, _ = synth.semi_dense_depth(fr_img, fr_dep * s, 3500 + i, fraction=0.3)
+            kf = ctx.frame(2 * i, cam, kf_img)
+            kf.set_depth(idk, vk)
+            fr = ctx.frame(2 * i + 1, cam, fr_img)
+            fr.set_depth(idf, vf)
+            items.append((kf, fr, ctx.ref(kf)))
+            inits.append(list(synth.IDENTITY7) + [1.0])
+        plain = [ctx.sim3_track(ref, fr, synth.IDENTITY7, 4, 1) for kf, fr, ref in items]
+        b = ctx.batch(_jobs(items, kind=lsdg.JOB_SIM3, init=inits, rebuild=0))
+        r = b.run()
+        b.destroy()
+    finally:
+        ctx.set_params(lsdg.default_params())
+    for i, (pose, st, status) in enumerate(plain):
+        assert r[i].status == status
+        rig, sc = posemath.sim3_error(list(r[i].frame_to_ref), pose)
+        assert rig <= 1e-9 and sc <= 1e-9, (i, rig, sc)
+        H, Hp = np.array(r[i].stats.hessian[:]), np.array(st.hessian[:])
+        assert np.abs(H - Hp).max() <= 1e-9 * np.abs(Hp).max()
