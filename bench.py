@@ -1,13 +1,15 @@
 #!/usr/bin/env python3
 """Throughput of the LSD-SLAM front end on one MI355X.
 
-Metric: SE3 tracked frames/s (and depth-map updates/s) at 640x480, batched vs
-plain.  A step is one pass of the hot path over one batch of synthetic
-640x480 alignments (keyframe with ~25% semi-dense points, new frame at a
-2 cm / 1 deg perturbation with sigma-2 noise, 5-level pyramid, levels 4..1,
-up to 20 LM iterations per level): every frame's pyramid is rebuilt from its
-resident level-0 image and the frame is tracked.  Inputs are resident in HBM
-when the timed region starts.
+Metric: SE3 tracked frames/s (plus depth-map updates/s) at 640x480, batched vs
+plain.  A step is one pass of the hot path over one batch of 64 synthetic
+640x480 alignments (keyframe with ~25% semi-dense points, new frame at a 2 cm /
+1 deg perturbation with sigma-2 noise, 5-level pyramid, levels 4..1, up to 20
+LM iterations per level): every frame's pyramid is rebuilt from its resident
+level-0 image and every frame is tracked.  `value` is the batched path with
+FP16 pyramids (one persistent launch per LM step, hipGraph-replayed); the plain
+path (one alignment per call) and FP32 pyramids are reported beside it.
+Inputs are resident in HBM when the timed region starts.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -31,21 +33,19 @@ import synth  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def tracking_params(its=20):
+def tracking_params(its=20, crop=None):
     p = lsdg.default_params()
     for l in range(5):
         p.max_its_per_lvl[l] = its
+    if crop is not None:
+        p.max_epl_length_crop = float(crop)
     return p
 
 
-def make_instances(n, seed0=1001):
-    return [synth.tracking_instance(seed0 + 17 * i) for i in range(n)]
+class Tracking:
+    """64 alignments on the device: plain calls or one batch plan."""
 
-
-class PlainPath:
-    """One alignment per call: lsdg_frame_build_pyramid + lsdg_se3_track."""
-
-    def __init__(self, ctx, insts, precision=lsdg.FP32):
+    def __init__(self, ctx, insts, precision):
         self.ctx = ctx
         self.items = []
         for i, inst in enumerate(insts):
@@ -56,13 +56,23 @@ class PlainPath:
             for l in range(1, 5):
                 ref.num_points(l)  # point clouds are per keyframe, built outside the timed region
             self.items.append((kf, fr, ref))
+        self.batch = None
         self.poses = [None] * len(self.items)
 
-    def step(self):
+    def plain_step(self):
         for i, (kf, fr, ref) in enumerate(self.items):
             fr.build_pyramid()
-            pose, st, status = self.ctx.se3_track(ref, fr, synth.IDENTITY7)
-            self.poses[i] = pose
+            self.poses[i] = self.ctx.se3_track(ref, fr, synth.IDENTITY7)[0]
+
+    def make_batch(self):
+        jobs = [lsdg.Job(ref.h, fr.h, lsdg.pose8(synth.IDENTITY7), lsdg.JOB_SE3, 4, 1, 1)
+                for kf, fr, ref in self.items]
+        self.batch = self.ctx.batch(jobs)
+
+    def batch_step(self):
+        res = self.batch.run()
+        self.poses = [np.array(r.frame_to_ref[:7]) for r in res]
+        return res
 
 
 def timed(ctx, fn, steps, warmup):
@@ -76,10 +86,48 @@ def timed(ctx, fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(insts, params, seconds, gpu_poses):
-    """The oracle's FP32 restatement of the reference CPU path (timing build),
-    one thread, over a bounded sample of the same alignments; plus the FP64
-    oracle's poses for the pose-error report."""
+def kernel_roofline(ctx, fn, kind):
+    ctx.profile(True)
+    fn()
+    ks = ctx.kernel_stats(kind)
+    ctx.profile(False)
+    achieved = ks["bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
+    return achieved, ks
+
+
+def depth_leg(ctx, steps, warmup):
+    """One DepthMap::updateKeyframe with one reference frame at a 5 cm
+    baseline, 640x480, epipolar search up to 100 px; the prior state is
+    restored (untimed) before every update."""
+    d = synth.depth_instance()
+    cam = d["cam"]
+    ctx.set_params(tracking_params(20, crop=100))
+    kf = ctx.frame(0, cam, d["kf_img"])
+    ref = ctx.frame(1, cam, d["ref_img"])
+    ref.set_tracking(kf, list(d["ref_pose"]) + [1.0], 0.4)
+    dm = ctx.depthmap(cam)
+    dm.set_prior(kf, d["prior_idepth"], d["prior_var"], 20)
+    state = dm.read()
+    times = []
+    for k in range(warmup + steps):
+        dm.write(state)
+        ctx.sync()
+        t0 = time.perf_counter()
+        dm.update([ref])
+        ctx.sync()
+        if k >= warmup:
+            times.append(time.perf_counter() - t0)
+    dm.write(state)
+    achieved, ks = kernel_roofline(ctx, lambda: dm.update([ref]), lsdg.KERNEL_DEPTH)
+    out = dict(updates_per_s=len(times) / sum(times), ms_per_update=1e3 * float(np.median(times)),
+               roofline_gbs=round(achieved, 2), device_ms_per_update=ks["ms"] / max(ks["launches"], 1))
+    return out, d
+
+
+def cpu_tracking_baseline(insts, params, seconds):
+    """The oracle's FP32 restatement of the reference CPU path (timing build,
+    upstream FP32 sums), one thread, on a bounded sample of the same
+    alignments; the per-frame pyramid build is included as on the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     o = pyoracle.Oracle(0, params=params, fast=True)
@@ -90,27 +138,54 @@ def cpu_baseline(insts, params, seconds, gpu_poses):
         ref = o.ref(kf)
         for l in range(1, 5):
             ref.num_points(l)
-        pairs.append((kf, ref, inst))
+        pairs.append((ref, inst["cam"], inst["fr_img"].astype(np.float32)))
     n = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        kf, ref, inst = pairs[n % len(pairs)]
-        fr = o.frame(10000 + n, inst["cam"], inst["fr_img"].astype(np.float32))  # pyramid build is per frame
+        ref, cam, img = pairs[n % len(pairs)]
+        fr = o.frame(10000 + n, cam, img)
         o.se3_track(ref, fr, synth.IDENTITY7)
         n += 1
     dt = time.perf_counter() - t0
-    o64 = pyoracle.Oracle(1, params=params)
+    return dict(value=round(n / dt, 2), unit="frames/s", cores=1, kind="port",
+                sample=f"{n} plain SE3 alignments of the 64 640x480 instances (oracle FP32 restatement of the "
+                       f"reference CPU path, -O3 x86-64-v3, 1 thread, {dt:.1f} s)")
+
+
+def cpu_depth_baseline(d, params, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle(0, params=params, fast=True)
+    cam = d["cam"]
+    kf = o.frame(0, cam, d["kf_img"].astype(np.float32))
+    ref = o.frame(1, cam, d["ref_img"].astype(np.float32))
+    ref.set_tracking(kf, list(d["ref_pose"]) + [1.0], 0.4)
+    out = {}
+    for threads in (1, os.cpu_count() or 1):
+        dm = o.depthmap(cam, threads=threads)
+        n, t = 0, 0.0
+        while t < seconds / 2:
+            dm.set_prior(kf, d["prior_idepth"], d["prior_var"], 20)
+            t0 = time.perf_counter()
+            dm.update([ref])
+            t += time.perf_counter() - t0
+            n += 1
+        out[threads] = n / t
+    return out
+
+
+def fp64_pose_errors(insts, params, poses, k=4):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle(1, params=params)
     errs = []
-    for i in range(min(4, len(insts))):
+    for i in range(min(k, len(insts))):
         inst = insts[i]
-        kf = o64.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
+        kf = o.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
         kf.set_depth(inst["idepth"], inst["var"])
-        fr = o64.frame(1, inst["cam"], inst["fr_img"].astype(np.float32))
-        pose64 = o64.se3_track(o64.ref(kf), fr, synth.IDENTITY7)[0]
-        errs.append(posemath.twist_error(gpu_poses[i], pose64))
-    return dict(value=n / dt, unit="frames/s", cores=1, kind="port",
-                sample=f"{n} plain SE3 alignments at 640x480 (FP32 restatement, -O3 x86-64-v3, 1 thread, "
-                       f"{dt:.1f} s)"), max(errs)
+        fr = o.frame(1, inst["cam"], inst["fr_img"].astype(np.float32))
+        errs.append(posemath.twist_error(poses[i], o.se3_track(o.ref(kf), fr, synth.IDENTITY7)[0]))
+    return errs
 
 
 def main():
@@ -119,8 +194,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -129,23 +205,55 @@ def main():
 
     params = tracking_params(20)
     ctx = lsdg.Context(0, params)
-    insts = make_instances(args.batch)
+    insts = [synth.tracking_instance(1001 + 17 * i) for i in range(args.batch)]
+    B = args.batch
 
-    plain = PlainPath(ctx, insts)
-    t = timed(ctx, plain.step, args.steps, args.warmup)
-    plain_fps = args.batch * args.steps / t
+    # headline: batched, FP16 pyramids
+    t16 = Tracking(ctx, insts, lsdg.FP16)
+    t16.make_batch()
+    t = timed(ctx, t16.batch_step, args.steps, args.warmup)
+    batched16_fps = B * args.steps / t
+    ms_per_step = 1e3 * t / args.steps
+    res16 = t16.batch_step()
+    achieved, ks = kernel_roofline(ctx, t16.batch_step, lsdg.KERNEL_BATCH_EVAL)
+    solve = ctx.kernel_stats(lsdg.KERNEL_BATCH_SOLVE)
+    poses16 = list(t16.poses)
 
-    # roofline of the dominant kernel (fused evaluation), one profiled step
-    ctx.profile(True)
-    plain.step()
-    ks = ctx.kernel_stats(lsdg.KERNEL_EVAL)
-    ctx.profile(False)
-    achieved = ks["bytes"] / (ks["ms"] * 1e-3) / 1e9
+    # batched, FP32 pyramids (FP64 sums: the plain path's LM path)
+    t32 = Tracking(ctx, insts, lsdg.FP32)
+    t32.make_batch()
+    t = timed(ctx, t32.batch_step, args.steps, args.warmup)
+    batched32_fps = B * args.steps / t
+    t32.batch_step()
+    poses32 = list(t32.poses)
 
-    value, ms_per_step = plain_fps, 1e3 * t / args.steps
+    out_plain = {}
+    if not args.no_plain:
+        t = timed(ctx, t32.plain_step, max(1, args.steps // 2), 1)
+        plain_fps = B * max(1, args.steps // 2) / t
+        plain_ach, plain_ks = kernel_roofline(ctx, t32.plain_step, lsdg.KERNEL_EVAL)
+        diff32 = max(posemath.twist_error(a, b) for a, b in zip(poses32, t32.poses))
+        diff16 = max(posemath.twist_error(a, b) for a, b in zip(poses16, t32.poses))
+        out_plain = {"plain_frames_per_s": round(plain_fps, 2),
+                     "batched_vs_plain_speedup": round(batched16_fps / plain_fps, 1),
+                     "plain_eval_kernel_gbs": round(plain_ach, 2),
+                     "plain_eval_avg_launch_us": round(1e3 * plain_ks["ms"] / max(plain_ks["launches"], 1), 3),
+                     "max_twist_batched_fp32_vs_plain": diff32,
+                     "max_twist_batched_fp16_vs_plain": diff16}
+
+    # mixed-resolution batch (320x240 / 640x480 / 1280x960, up to 5 cm / 3 deg)
+    mixed = synth.batch_instances(64, seed=1003)
+    tm = Tracking(ctx, mixed, lsdg.FP16)
+    tm.make_batch()
+    t = timed(ctx, tm.batch_step, args.steps, args.warmup)
+    mixed_aps = 64 * args.steps / t
+
+    depth, dinst = depth_leg(ctx, args.steps, args.warmup)
+    ctx.set_params(params)
+
     out = {
         "metric": "SE3 track frames/s + depth updates/s @640x480, batched vs plain, 1 GPU",
-        "value": round(value, 2),
+        "value": round(batched16_fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -156,20 +264,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain SE3 alignment, {args.batch} x 640x480 frames per step, ~25% semi-dense "
-                               "points, 5-level pyramid, levels 4..1, <=20 LM iterations/level",
-                   "batch": args.batch, "resolution": "640x480", "precision": "fp32 pyramids"},
-        "plain_frames_per_s": round(plain_fps, 2),
+        "config": {"workload": f"batched SE3 alignment: {B} x 640x480 frames per step (pyramid rebuild + LM "
+                               "levels 4..1, <=20 iterations/level), ~25% semi-dense points, 2 cm / 1 deg, "
+                               "FP16 (I, gx, gy) pyramids, FP32 residuals, compensated FP32 block sums, FP64 "
+                               "merge and LDLT", "batch": B, "resolution": "640x480", "storage": "fp16"},
+        "batched_fp32_frames_per_s": round(batched32_fps, 2),
+        **out_plain,
+        "mixed_resolution_batch_alignments_per_s": round(mixed_aps, 2),
+        "depth_updates_per_s": round(depth["updates_per_s"], 2),
+        "depth_update_ms": round(depth["ms_per_update"], 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_se3_eval (fused residual + weights + LGS6)",
+                     "kernel": "k_batch_eval (fused residual + weights + LGS over all jobs' tiles, FP16 taps)",
                      "avg_launch_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 3),
-                     "launches": ks["launches"]},
+                     "launches": ks["launches"],
+                     "bytes_per_launch": round(ks["bytes"] / max(ks["launches"], 1), 1),
+                     "solve_kernel_avg_us": round(1e3 * solve["ms"] / max(solve["launches"], 1), 3)},
     }
     if not args.no_cpu:
-        cb, err = cpu_baseline(insts, params, args.cpu_seconds, plain.poses)
+        cb = cpu_tracking_baseline(insts, params, args.cpu_seconds * 0.6)
         out["cpu_baseline"] = cb
-        out["pose_error_vs_fp64_oracle"] = err
+        cd = cpu_depth_baseline(dinst, tracking_params(20, crop=100), args.cpu_seconds * 0.4)
+        out["cpu_depth_updates_per_s"] = {str(k): round(v, 2) for k, v in cd.items()}
+        errs16 = fp64_pose_errors(insts, params, poses16)
+        out["pose_error_vs_fp64_oracle"] = {"batched_fp16_max": max(errs16), "batched_fp16_median":
+                                            float(np.median(errs16)), "instances": len(errs16)}
     print(json.dumps(out))
 
 

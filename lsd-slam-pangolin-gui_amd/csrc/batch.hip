@@ -1,16 +1,19 @@
 // Batched tracking: many frame-vs-keyframe alignments of uneven size advance
-// one LM evaluation per launch of k_batch_step, and a hipGraph replays K such
-// launches with no host round trip.
+// one LM evaluation per step, a step being two launches (k_batch_eval over
+// all jobs' points, k_batch_solve per job), and a hipGraph replays K steps
+// with no host round trip.
 //
 // Work decomposition.  Each job's points at its current level are cut into
-// tiles of TILE = 256 * ppt points.  A persistent grid walks the concatenated
-// tile list of all active jobs (the per-job tile counts are double-buffered
-// per step so every block sees the same list).  Each tile's FP64 partial sums
-// go to a fixed slot; the block that finishes a job's last tile merges that
-// job's slots in slot order (deterministic), then runs the job's LM step in
-// one lane: the reference's accept / reject / lambda / convergence logic
+// tiles of TILE = 256 * ppt points.  k_batch_eval's persistent grid walks the
+// concatenated tile list of all active jobs (the per-job tile counts are
+// double-buffered per step so every block sees the same list) and writes each
+// tile's FP64 partial sums to a fixed slot.  k_batch_solve then merges each
+// job's slots in slot order (deterministic) and runs the job's LM step in one
+// lane: the reference's accept / reject / lambda / convergence logic
 // (SE3Tracker::trackFrame, Sim3Tracker::trackFrameSim3), an FP64 LDL^T solve
-// and exp(d) * T, and writes the pose the job's next evaluation uses.
+// and exp(d) * T, and writes the pose the job's next evaluation uses.  The
+// kernel boundary between the two orders the partial sums, so no block waits
+// on or fences for another.
 #include "batch_internal.h"
 #include "track_device.h"
 
@@ -122,6 +125,7 @@ __device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, con
   const int lvl = s.level;
   const JobLevel& L = jd.lv[lvl];
   s.evals++;
+  s.evalsAt[lvl]++;
   s.good = S[S3_GOOD];
   s.bad = S[S3_BAD];
   s.meanRes = S[S3_SIGNED] / S[S3_GOOD];
@@ -198,6 +202,7 @@ __device__ void lm_step_sim3(JobState& s, const JobDesc& jd, const double* S, co
   const int lvl = s.level;
   const JobLevel& L = jd.lv[lvl];
   s.evals++;
+  s.evalsAt[lvl]++;
   s.usage = S[SM_USAGE] / (double)L.pts.n;
   if (s.phase == 2) {
     // final evaluation for lastSim3Hessian
@@ -286,18 +291,18 @@ __global__ __launch_bounds__(256) void k_batch_init(BatchArgs a) {
   }
 }
 
+// Tile evaluation of one LM step: a persistent grid walks the concatenated
+// tile list of this step's jobs; each tile's FP64 partial row goes to a fixed
+// slot.  No cross-block communication: the merge runs in k_batch_solve.
 template <int KIND, int PREC, int ACC>
-__global__ __launch_bounds__(BATCH_BS) void k_batch_step(BatchArgs a, int parity) {
+__global__ __launch_bounds__(BATCH_BS) void k_batch_eval(BatchArgs a, int parity) {
   constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
   __shared__ int pref[MAX_BATCH_JOBS + 1];
   __shared__ int scan_tmp[BATCH_BS];
   __shared__ double sh[BATCH_WAVES][NS];
-  __shared__ double Ssum[NS];
-  __shared__ int amLast;
   const int tid = threadIdx.x;
   const int nj = a.njobs;
   const int* nt = a.ntiles + parity * nj;
-  int* ntNext = a.ntiles + (1 - parity) * nj;
   // exclusive prefix of this step's tile counts (chunked block scan)
   {
     const int per = (nj + BATCH_BS - 1) / BATCH_BS;
@@ -321,9 +326,6 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_step(BatchArgs a, int parity
     if (tid == BATCH_BS - 1) pref[nj] = scan_tmp[BATCH_BS - 1];
     __syncthreads();
   }
-  if (blockIdx.x == 0)
-    for (int j = tid; j < nj; j += BATCH_BS)
-      if (nt[j] == 0) ntNext[j] = 0;
   const int total = pref[nj];
   const int lane = tid & 63, wave = tid >> 6;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -336,7 +338,7 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_step(BatchArgs a, int parity
     const int j = lo;
     const int tile = t - pref[j];
     const JobDesc& jd = a.jobs[j];
-    JobState* st = a.st + j;
+    const JobState* st = a.st + j;
     const int lvl = st->level;
     const JobLevel& L = jd.lv[lvl];
     const Xf x = st->xf;
@@ -359,40 +361,45 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_step(BatchArgs a, int parity
     if (lane == 0)
       for (int k = 0; k < NS; ++k) sh[wave][k] = acc.value(k);
     __syncthreads();
-    double* row = a.partials + ((size_t)jd.slotBase + tile) * NS;
     if (tid < NS) {
       double v = 0.0;
       for (int w = 0; w < BATCH_WAVES; ++w) v += sh[w][tid];
-      row[tid] = v;
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) amLast = atomicAdd(&a.counters[j], 1u) == (unsigned)(nt[j] - 1);
-    __syncthreads();
-    if (amLast) {
-      __threadfence();
-      if (tid < NS) {
-        double v = 0.0;
-        const double* rows = a.partials + (size_t)jd.slotBase * NS;
-        for (int k = 0; k < nt[j]; ++k) {
-          unsigned long long bits = __hip_atomic_load((const unsigned long long*)&rows[(size_t)k * NS + tid],
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          v += __longlong_as_double((long long)bits);
-        }
-        Ssum[tid] = v;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        JobState s = *st;
-        int next = 0;
-        if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, Ssum, a.P, &next, a.remaining);
-        else lm_step_se3(s, jd, Ssum, a.P, &next, a.remaining);
-        *st = s;
-        ntNext[j] = next;
-        __hip_atomic_store(&a.counters[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      a.partials[((size_t)jd.slotBase + tile) * NS + tid] = v;
     }
     __syncthreads();
+  }
+}
+
+// Merge and LM step of one LM iteration: one 64-thread block per job merges
+// the job's tile rows in slot order and lane 0 runs the LM step.  Also
+// zeroes the next step's tile count of jobs that are already finished.
+template <int KIND>
+__global__ __launch_bounds__(64) void k_batch_solve(BatchArgs a, int parity) {
+  constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
+  __shared__ double S[NS];
+  const int j = blockIdx.x;
+  const int nj = a.njobs;
+  const int n = a.ntiles[parity * nj + j];
+  int* ntNext = a.ntiles + (1 - parity) * nj + j;
+  if (n == 0) {
+    if (threadIdx.x == 0) *ntNext = 0;
+    return;
+  }
+  const JobDesc& jd = a.jobs[j];
+  if (threadIdx.x < NS) {
+    const double* rows = a.partials + (size_t)jd.slotBase * NS + threadIdx.x;
+    double v = 0.0;
+    for (int k = 0; k < n; ++k) v += rows[(size_t)k * NS];
+    S[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    JobState s = a.st[j];
+    int next = 0;
+    if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, S, a.P, &next, a.remaining);
+    else lm_step_se3(s, jd, S, a.P, &next, a.remaining);
+    a.st[j] = s;
+    *ntNext = next;
   }
 }
 
@@ -403,24 +410,36 @@ hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
 }
 
 template <int KIND, int PREC>
-static void step_prec(const BatchArgs& a, int parity, int grid, int acc, hipStream_t s) {
+static void eval_prec(const BatchArgs& a, int parity, int grid, int acc, hipStream_t s) {
   if (acc == BATCH_ACC_KAHAN)
-    k_batch_step<KIND, PREC, ACC_KAHAN><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
+    k_batch_eval<KIND, PREC, ACC_KAHAN><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
   else
-    k_batch_step<KIND, PREC, ACC_F64><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
+    k_batch_eval<KIND, PREC, ACC_F64><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
+}
+
+hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s) {
+  if (kind == LSDG_JOB_SIM3) {
+    if (prec == LSDG_FP16) eval_prec<LSDG_JOB_SIM3, LSDG_FP16>(a, parity, grid, acc, s);
+    else if (prec == LSDG_BF16) eval_prec<LSDG_JOB_SIM3, LSDG_BF16>(a, parity, grid, acc, s);
+    else eval_prec<LSDG_JOB_SIM3, LSDG_FP32>(a, parity, grid, acc, s);
+  } else {
+    if (prec == LSDG_FP16) eval_prec<LSDG_JOB_SE3, LSDG_FP16>(a, parity, grid, acc, s);
+    else if (prec == LSDG_BF16) eval_prec<LSDG_JOB_SE3, LSDG_BF16>(a, parity, grid, acc, s);
+    else eval_prec<LSDG_JOB_SE3, LSDG_FP32>(a, parity, grid, acc, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s) {
+  if (kind == LSDG_JOB_SIM3) k_batch_solve<LSDG_JOB_SIM3><<<dim3(a.njobs), dim3(64), 0, s>>>(a, parity);
+  else k_batch_solve<LSDG_JOB_SE3><<<dim3(a.njobs), dim3(64), 0, s>>>(a, parity);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s) {
-  if (kind == LSDG_JOB_SIM3) {
-    if (prec == LSDG_FP16) step_prec<LSDG_JOB_SIM3, LSDG_FP16>(a, parity, grid, acc, s);
-    else if (prec == LSDG_BF16) step_prec<LSDG_JOB_SIM3, LSDG_BF16>(a, parity, grid, acc, s);
-    else step_prec<LSDG_JOB_SIM3, LSDG_FP32>(a, parity, grid, acc, s);
-  } else {
-    if (prec == LSDG_FP16) step_prec<LSDG_JOB_SE3, LSDG_FP16>(a, parity, grid, acc, s);
-    else if (prec == LSDG_BF16) step_prec<LSDG_JOB_SE3, LSDG_BF16>(a, parity, grid, acc, s);
-    else step_prec<LSDG_JOB_SE3, LSDG_FP32>(a, parity, grid, acc, s);
-  }
-  return hipGetLastError();
+  hipError_t e = launch_batch_eval(a, kind, prec, acc, parity, grid, s);
+  if (e != hipSuccess) return e;
+  return launch_batch_solve(a, kind, parity, s);
 }
 
 }  // namespace lsdg

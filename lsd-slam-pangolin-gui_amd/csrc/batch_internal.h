@@ -45,6 +45,7 @@ struct JobState {
   int status;        // 0 running, 1 done, 2 diverged
   int level, phase, iteration, incTry, upToDate, evals;
   int numRes[PYR], numWarp[PYR];
+  int evalsAt[PYR];  // evaluations run per level (every step the job was active)
 };
 
 struct BatchArgs {
@@ -62,5 +63,7 @@ struct BatchArgs {
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s);
 hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
+hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
+hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s);
 
 }  // namespace lsdg

@@ -321,6 +321,39 @@ static void fill_result(lsdg_batch b, int j, const JobState& s, lsdg_job_result*
   r->status = LSDG_OK;
 }
 
+// The same work as the graphs, launched in stream order with an event pair
+// around every LM-step kernel (and the pyramid rebuild) for lsdg_ctx_profile.
+static lsdg_status batch_run_profiled(lsdg_batch b) {
+  lsdg_ctx c = b->ctx;
+  BatchArgs a = batch_args(b);
+  if (b->pyr.nf > 0) {
+    double pb = 0.0;
+    for (lsdg_frame f : b->pyrFrames) {
+      double bpx = f->precision == LSDG_FP32 ? 12.0 : 6.0;
+      pb += (double)f->w[0] * f->h[0] * 4.0;
+      for (int l = 0; l < PYR; ++l) pb += (double)f->w[l] * f->h[l] * (4.0 + bpx + 4.0);
+    }
+    ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);
+    CK(c, launch_pyr_batch(b->pyr, c->stream));
+    prof_end(c, pm, pb);
+  }
+  CK(c, launch_batch_init(a, c->stream));
+  for (int round = 0; round <= 64; ++round) {
+    for (int s = 0; s < b->steps; ++s) {
+      ProfMark pm = prof_begin(c, LSDG_KERNEL_BATCH_EVAL);
+      CK(c, launch_batch_eval(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream));
+      prof_end(c, pm, 0.0);
+      ProfMark ps = prof_begin(c, LSDG_KERNEL_BATCH_SOLVE);
+      CK(c, launch_batch_solve(a, b->kind, s & 1, c->stream));
+      prof_end(c, ps, 0.0);
+    }
+    CK(c, hipMemcpyAsync(b->h_remaining, b->d_remaining, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CK(c, hipStreamSynchronize(c->stream));
+    if (*b->h_remaining <= 0) return prof_resolve(c);
+  }
+  return fail(c, LSDG_ESTATE, "batched LM did not finish");
+}
+
 extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
   if (!b) return LSDG_EINVAL;
   lsdg_ctx c = b->ctx;
@@ -332,19 +365,45 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
     if (st == LSDG_OK) st = batch_graphs(b);
     if (st != LSDG_OK) return st;
   }
-  ProfMark pm = prof_begin(c, LSDG_KERNEL_BATCH_EVAL);
-  CK(c, hipGraphLaunch(b->first, c->stream));
-  CK(c, hipStreamSynchronize(c->stream));
-  int launches = 0;
-  while (*b->h_remaining > 0) {
-    if (++launches > 64) return fail(c, LSDG_ESTATE, "batched LM did not finish within 64 continuation graphs");
-    CK(c, hipGraphLaunch(b->cont, c->stream));
+  if (c->prof_on) {
+    lsdg_status st = batch_run_profiled(b);
+    if (st != LSDG_OK) return st;
+  } else {
+    CK(c, hipGraphLaunch(b->first, c->stream));
     CK(c, hipStreamSynchronize(c->stream));
+    int launches = 0;
+    while (*b->h_remaining > 0) {
+      if (++launches > 64) return fail(c, LSDG_ESTATE, "batched LM did not finish within 64 continuation graphs");
+      CK(c, hipGraphLaunch(b->cont, c->stream));
+      CK(c, hipStreamSynchronize(c->stream));
+    }
+    if (launches > 0 && b->steps < 128) {
+      // the graph was too short for this workload: size the next run's graph
+      // to the steps this one needed (LM steps = evaluations of the longest job)
+      CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
+      int need = 0;
+      for (int j = 0; j < b->n; ++j) need = std::max(need, b->h_st[j].evals);
+      int steps = std::min(128, ((need + 2) + 1) & ~1);
+      if (steps > b->steps) {
+        b->steps = steps;
+        lsdg_status st = batch_graphs(b);
+        if (st != LSDG_OK) return st;
+      }
+    }
   }
-  prof_end(c, pm, 0.0);
-  if (results) {
+  if (results || c->prof_on) {
     CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
-    for (int j = 0; j < b->n; ++j) fill_result(b, j, b->h_st[j], &results[j]);
+    if (results)
+      for (int j = 0; j < b->n; ++j) fill_result(b, j, b->h_st[j], &results[j]);
+    if (c->prof_on) {
+      // algorithmic bytes of every evaluation the steps ran
+      double bytes = 0.0;
+      for (int j = 0; j < b->n; ++j)
+        for (int l = 0; l < PYR; ++l)
+          bytes += b->h_st[j].evalsAt[l] *
+                   eval_bytes(b->h_jobs[j].lv[l].pts.n, b->jobs[j].frame, l, b->kind == LSDG_JOB_SIM3);
+      c->prof[LSDG_KERNEL_BATCH_EVAL].bytes += bytes;
+    }
   }
   return LSDG_OK;
 }

@@ -96,6 +96,41 @@ struct Accum<ACC_F64, NS> {
     }
   }
   __device__ inline double value(int k) const { return v[k]; }
+  // Reduce-scatter over the wave (NS <= 64): afterwards lane l holds the wave
+  // total of value l (0 for l >= NS).  Halving butterfly: at offset o each
+  // lane keeps one half of its values and sends the other half to lane ^ o,
+  // 63 shuffles in all instead of 6 * NS.
+  __device__ inline double wave_reduce_scatter() const {
+    static_assert(NS <= 64, "reduce-scatter holds one value per lane");
+    const int lane = threadIdx.x & 63;
+    double a[32];
+    {
+      const bool hi = (lane & 32) != 0;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        if (i < NS) {
+          double lo = v[i];
+          double up = (i + 32 < NS) ? v[i + 32] : 0.0;
+          double send = hi ? lo : up;
+          double keep = hi ? up : lo;
+          a[i] = keep + __shfl_xor(send, 32);
+        } else {
+          a[i] = 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) {
+      const bool hi = (lane & h) != 0;
+#pragma unroll
+      for (int i = 0; i < h; ++i) {
+        double send = hi ? a[i] : a[i + h];
+        double keep = hi ? a[i + h] : a[i];
+        a[i] = keep + __shfl_xor(send, h);
+      }
+    }
+    return a[0];
+  }
 };
 
 // Error-free sum of two floats: hi + lo == a + b exactly (round-to-nearest,
@@ -136,6 +171,47 @@ struct Accum<ACC_KAHAN, NS> {
     }
   }
   __device__ inline double value(int k) const { return (double)s[k] + (double)c[k]; }
+  // Reduce-scatter of the (hi, lo) pairs over the wave; lane l returns value l.
+  __device__ inline double wave_reduce_scatter() const {
+    static_assert(NS <= 64, "reduce-scatter holds one value per lane");
+    const int lane = threadIdx.x & 63;
+    float as[32], ac[32];
+    {
+      const bool hi = (lane & 32) != 0;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        if (i < NS) {
+          float los = s[i], loc = c[i];
+          float ups = (i + 32 < NS) ? s[i + 32] : 0.f, upc = (i + 32 < NS) ? c[i + 32] : 0.f;
+          float ss = hi ? los : ups, sc = hi ? loc : upc;
+          float ks = hi ? ups : los, kc = hi ? upc : loc;
+          float rs = __shfl_xor(ss, 32), rc = __shfl_xor(sc, 32);
+          float h2, l2;
+          two_sum(ks, rs, h2, l2);
+          as[i] = h2;
+          ac[i] = kc + rc + l2;
+        } else {
+          as[i] = 0.f;
+          ac[i] = 0.f;
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) {
+      const bool hi = (lane & h) != 0;
+#pragma unroll
+      for (int i = 0; i < h; ++i) {
+        float ss = hi ? as[i] : as[i + h], sc = hi ? ac[i] : ac[i + h];
+        float ks = hi ? as[i + h] : as[i], kc = hi ? ac[i + h] : ac[i];
+        float rs = __shfl_xor(ss, h), rc = __shfl_xor(sc, h);
+        float h2, l2;
+        two_sum(ks, rs, h2, l2);
+        as[i] = h2;
+        ac[i] = kc + rc + l2;
+      }
+    }
+    return (double)as[0] + (double)ac[0];
+  }
 };
 
 // Index of (i, j), i <= j, in the row-major upper triangle of a 7x7 matrix.
