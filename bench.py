@@ -61,7 +61,7 @@ class Tracking:
 
     def plain_step(self):
         for i, (kf, fr, ref) in enumerate(self.items):
-            fr.build_pyramid()
+            fr.invalidate()  # the tracker rebuilds the levels it reads, as the batch does
             self.poses[i] = self.ctx.se3_track(ref, fr, synth.IDENTITY7)[0]
 
     def make_batch(self):
@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="run only the batched FP16 leg (for kernel profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -217,7 +219,16 @@ def main():
     res16 = t16.batch_step()
     achieved, ks = kernel_roofline(ctx, t16.batch_step, lsdg.KERNEL_BATCH_EVAL)
     solve = ctx.kernel_stats(lsdg.KERNEL_BATCH_SOLVE)
+    jobk = ctx.kernel_stats(lsdg.KERNEL_BATCH_JOB)
     poses16 = list(t16.poses)
+    if args.headline_only:
+        print(json.dumps({"value": round(batched16_fps, 2), "unit": "frames/s", "ms_per_step": round(ms_per_step, 3),
+                          "eval_avg_launch_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 3),
+                          "eval_launches": ks["launches"], "eval_gbs": round(achieved, 2),
+                          "solve_avg_us": round(1e3 * solve["ms"] / max(solve["launches"], 1), 3),
+                          "job_kernel_us": round(1e3 * jobk["ms"] / max(jobk["launches"], 1), 3),
+                          "job_kernel_gbs": round(jobk["bytes"] / (jobk["ms"] * 1e-3) / 1e9, 2) if jobk["ms"] else 0}))
+        return
 
     # batched, FP32 pyramids (FP64 sums: the plain path's LM path)
     t32 = Tracking(ctx, insts, lsdg.FP32)

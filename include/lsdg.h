@@ -215,9 +215,12 @@ const char* lsdg_ctx_last_error(lsdg_ctx ctx);
 /* ------------------------------------------------------------------ frames */
 
 /* Frame(id, width, height, K, timestamp, image): uploads the level-0 image
- * (float, w*h) from host memory and builds image, gradient and
- * max-gradient pyramids on the device (Frame::buildImage / buildGradients /
- * buildMaxGradients).  The wrapper's input is an undistorted CV_8UC1 image
+ * (float, w*h) from host memory.  Image, gradient and max-gradient levels
+ * (Frame::buildImage / buildGradients / buildMaxGradients) are built on the
+ * device the first time a call reads them, per level and kind, as
+ * Frame::require does: a tracker builds the gradients of the levels it
+ * tracks, a depth map the keyframe's level-0 gradients and max-gradients.
+ * The wrapper's input is an undistorted CV_8UC1 image
  * (lib/App/InputThread.cpp:59-65); lsdg_frame_create_u8 takes that form. */
 lsdg_status lsdg_frame_create(lsdg_ctx ctx, int32_t id, const lsdg_camera* cam,
                               const float* image, int32_t precision, lsdg_frame* out);
@@ -225,9 +228,11 @@ lsdg_status lsdg_frame_create_u8(lsdg_ctx ctx, int32_t id, const lsdg_camera* ca
                                  const uint8_t* image, int32_t stride, int32_t precision,
                                  lsdg_frame* out);
 lsdg_status lsdg_frame_destroy(lsdg_frame f);
-/* Rebuilds every pyramid level from the resident level-0 image (the per-frame
- * work a new frame costs the tracker). */
+/* Rebuilds every pyramid level of every kind from the resident level-0 image. */
 lsdg_status lsdg_frame_build_pyramid(lsdg_frame f);
+/* Drops every pyramid level above the level-0 image; the next call that reads
+ * one rebuilds it (what a new frame costs the tracker). */
+lsdg_status lsdg_frame_invalidate(lsdg_frame f);
 /* Frame::setDepth from plain arrays (idepth <= 0 or var <= 0: no depth), then
  * the inverse-variance depth pyramid (Frame::buildIDepthAndIDepthVar). */
 lsdg_status lsdg_frame_set_depth(lsdg_frame f, const float* idepth, const float* idepth_var);
@@ -336,6 +341,7 @@ lsdg_status lsdg_depth_active_keyframe(lsdg_depthmap dm, int32_t* id);
 #define LSDG_KERNEL_DEPTH 2       /* depth-map kernels of one update */
 #define LSDG_KERNEL_BATCH_EVAL 3  /* batched evaluation sweep */
 #define LSDG_KERNEL_BATCH_SOLVE 4 /* batched LM step (merge, solve, accept) */
+#define LSDG_KERNEL_BATCH_JOB 5   /* batched whole-job LM, one block per job */
 #define LSDG_KERNEL_KINDS 8
 
 typedef struct lsdg_kernel_stats {

@@ -1,19 +1,23 @@
-// Batched tracking: many frame-vs-keyframe alignments of uneven size advance
-// one LM evaluation per step, a step being two launches (k_batch_eval over
-// all jobs' points, k_batch_solve per job), and a hipGraph replays K steps
-// with no host round trip.
+// Batched tracking: many frame-vs-keyframe alignments of uneven size, with no
+// host round trip between LM iterations.  Two schedules share the LM code:
 //
-// Work decomposition.  Each job's points at its current level are cut into
-// tiles of TILE = 256 * ppt points.  k_batch_eval's persistent grid walks the
+// Whole-job kernel (k_batch_job).  While a job's level is small, one block
+// runs the job's LM loop by itself: evaluation over the level's points,
+// deterministic block reduction, LM step in lane 0, repeat.  No launch and no
+// inter-block communication per iteration.
+//
+// Multi-block steps (k_batch_eval + k_batch_solve).  For larger levels each
+// step is two launches.  Each job's points are cut into tiles of
+// TILE = 256 * ppt points; k_batch_eval's persistent grid walks the
 // concatenated tile list of all active jobs (the per-job tile counts are
 // double-buffered per step so every block sees the same list) and writes each
 // tile's FP64 partial sums to a fixed slot.  k_batch_solve then merges each
-// job's slots in slot order (deterministic) and runs the job's LM step in one
-// lane: the reference's accept / reject / lambda / convergence logic
+// job's slots in a fixed order (deterministic) and runs the job's LM step in
+// one lane: the reference's accept / reject / lambda / convergence logic
 // (SE3Tracker::trackFrame, Sim3Tracker::trackFrameSim3), an FP64 LDL^T solve
 // and exp(d) * T, and writes the pose the job's next evaluation uses.  The
 // kernel boundary between the two orders the partial sums, so no block waits
-// on or fences for another.
+// on or fences for another.  A hipGraph replays K such steps.
 #include "batch_internal.h"
 #include "track_device.h"
 
@@ -21,6 +25,11 @@ namespace lsdg {
 
 constexpr int BATCH_BS = 256;
 constexpr int BATCH_WAVES = BATCH_BS / 64;
+#ifndef LSDG_BATCH_U
+#define LSDG_BATCH_U 2
+#endif
+constexpr int BATCH_U = LSDG_BATCH_U;  // points per thread with loads in flight together
+constexpr int BATCH_SOLVE_BS = 256;
 
 // f^n by repeated multiplication (exact for the default factor 2).
 __host__ __device__ inline float pow_int(float f, int n) {
@@ -352,14 +361,13 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_eval(BatchArgs a, int parity
     const int i1 = min(i0 + tileSize, L.pts.n);
     if (KIND == LSDG_JOB_SIM3) {
       kp.roll[0] = st->roll[0]; kp.roll[1] = st->roll[1]; kp.roll[2] = st->roll[2]; kp.roll[3] = st->roll[3];
-      for (int i = i0 + tid; i < i1; i += BATCH_BS) sim3_point<PREC>(L.pts, i, L.img, L.fid, L.fvar, L.g, x, kp, acc);
+      sim3_points<PREC, BATCH_U>(L.pts, i0 + tid, BATCH_BS, i1, L.img, L.fid, L.fvar, L.g, x, kp, acc);
     } else {
       uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
-      for (int i = i0 + tid; i < i1; i += BATCH_BS) se3_point<PREC>(L.pts, i, L.img, L.g, x, kp, mask, acc);
+      se3_points<PREC, BATCH_U>(L.pts, i0 + tid, BATCH_BS, i1, L.img, L.g, x, kp, mask, acc);
     }
-    acc.wave_reduce();
-    if (lane == 0)
-      for (int k = 0; k < NS; ++k) sh[wave][k] = acc.value(k);
+    const double wsum = acc.wave_reduce_scatter();
+    if (lane < NS) sh[wave][lane] = wsum;
     __syncthreads();
     if (tid < NS) {
       double v = 0.0;
@@ -370,40 +378,162 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_eval(BatchArgs a, int parity
   }
 }
 
-// Merge and LM step of one LM iteration: one 64-thread block per job merges
-// the job's tile rows in slot order and lane 0 runs the LM step.  Also
-// zeroes the next step's tile count of jobs that are already finished.
+// Copies a trivially copyable object between global memory and LDS with the
+// whole block (32-bit words).
+template <class T>
+__device__ inline void block_copy(T* dst, const T* src, int nthreads) {
+  static_assert(sizeof(T) % 4 == 0, "copied as 32-bit words");
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  for (int w = threadIdx.x; w < (int)(sizeof(T) / 4); w += nthreads) d[w] = s[w];
+}
+
+// Merge and LM step of one LM iteration: one block per job.  Wave w sums the
+// job's tile rows w, w + 4, ... (row order within a wave), the four wave sums
+// are added in wave order, and lane 0 runs the LM step on the job's
+// descriptor and control block staged in LDS (their loads are issued
+// alongside the row loads).  Also zeroes the next step's tile count of jobs
+// that are already finished.
 template <int KIND>
-__global__ __launch_bounds__(64) void k_batch_solve(BatchArgs a, int parity) {
+__global__ __launch_bounds__(BATCH_SOLVE_BS) void k_batch_solve(BatchArgs a, int parity) {
   constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
+  constexpr int WAVES = BATCH_SOLVE_BS / 64;
+  __shared__ double part[WAVES][NS];
   __shared__ double S[NS];
-  const int j = blockIdx.x;
+  __shared__ JobState sst;
+  __shared__ JobDesc sjd;
+  __shared__ int next;
+  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nj = a.njobs;
   const int n = a.ntiles[parity * nj + j];
   int* ntNext = a.ntiles + (1 - parity) * nj + j;
   if (n == 0) {
-    if (threadIdx.x == 0) *ntNext = 0;
+    if (tid == 0) *ntNext = 0;
     return;
   }
-  const JobDesc& jd = a.jobs[j];
-  if (threadIdx.x < NS) {
-    const double* rows = a.partials + (size_t)jd.slotBase * NS + threadIdx.x;
+  const int slotBase = a.jobs[j].slotBase;
+  block_copy(&sst, a.st + j, BATCH_SOLVE_BS);
+  block_copy(&sjd, a.jobs + j, BATCH_SOLVE_BS);
+  if (lane < NS) {
+    const double* rows = a.partials + (size_t)slotBase * NS + lane;
     double v = 0.0;
-    for (int k = 0; k < n; ++k) v += rows[(size_t)k * NS];
-    S[threadIdx.x] = v;
+    int k = wave;
+    for (; k + 7 * WAVES < n; k += 8 * WAVES) {
+      double r[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) r[u] = rows[(size_t)(k + u * WAVES) * NS];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v += r[u];
+    }
+    for (; k < n; k += WAVES) v += rows[(size_t)k * NS];
+    part[wave][lane] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    JobState s = a.st[j];
-    int next = 0;
-    if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, S, a.P, &next, a.remaining);
-    else lm_step_se3(s, jd, S, a.P, &next, a.remaining);
-    a.st[j] = s;
-    *ntNext = next;
+  if (tid < NS) {
+    double v = part[0][tid];
+    for (int w = 1; w < WAVES; ++w) v += part[w][tid];
+    S[tid] = v;
   }
+  __syncthreads();
+  if (tid == 0) {
+    int nx = 0;
+    if (KIND == LSDG_JOB_SIM3) lm_step_sim3(sst, sjd, S, a.P, &nx, a.remaining);
+    else lm_step_se3(sst, sjd, S, a.P, &nx, a.remaining);
+    next = nx;
+  }
+  __syncthreads();
+  block_copy(a.st + j, &sst, BATCH_SOLVE_BS);
+  if (tid == 0) *ntNext = next;
+}
+
+
+// Whole LM of one job in one block, for as long as the job's current level
+// has at most a.jobMaxPoints points: the level's evaluation over the block,
+// the block's deterministic reduction (wave reduce-scatter, then waves in
+// order) and the LM step in lane 0, repeated with no launch and no
+// inter-block communication in between.  The job's descriptor and control
+// block live in LDS meanwhile.  On exit the job is finished or stands at a
+// larger level, whose tile count goes to the parity-0 slot for the
+// multi-block steps.
+template <int KIND, int PREC, int ACC>
+__global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
+  constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
+  constexpr int WAVES = BATCH_JOB_BS / 64;
+  __shared__ JobState s;
+  __shared__ JobDesc jd;
+  __shared__ double sh[WAVES][NS];
+  __shared__ double S[NS];
+  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  block_copy(&s, a.st + j, BATCH_JOB_BS);
+  block_copy(&jd, a.jobs + j, BATCH_JOB_BS);
+  __syncthreads();
+  for (int it = 0; it < BATCH_JOB_MAX_EVALS; ++it) {
+    // s and jd are read after a barrier: the exit test is uniform over the block
+    if (s.status != 0) break;
+    const int lvl = s.level;
+    const JobLevel& L = jd.lv[lvl];
+    if (L.pts.n > a.jobMaxPoints) break;
+    const Xf x = s.xf;
+    TrackKParams kp = a.kp;
+    kp.aff_a = s.affA;
+    kp.aff_b = s.affB;
+    Accum<ACC, NS> acc;
+    acc.zero();
+    const int n = L.pts.n;
+    if (KIND == LSDG_JOB_SIM3) {
+      kp.roll[0] = s.roll[0]; kp.roll[1] = s.roll[1]; kp.roll[2] = s.roll[2]; kp.roll[3] = s.roll[3];
+      sim3_points<PREC, BATCH_U>(L.pts, tid, BATCH_JOB_BS, n, L.img, L.fid, L.fvar, L.g, x, kp, acc);
+    } else {
+      uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
+      se3_points<PREC, BATCH_U>(L.pts, tid, BATCH_JOB_BS, n, L.img, L.g, x, kp, mask, acc);
+    }
+    const double wsum = acc.wave_reduce_scatter();
+    if (lane < NS) sh[wave][lane] = wsum;
+    __syncthreads();
+    if (tid < NS) {
+      double v = 0.0;
+      for (int w = 0; w < WAVES; ++w) v += sh[w][tid];
+      S[tid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int next = 0;
+      if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, S, a.P, &next, a.remaining);
+      else lm_step_se3(s, jd, S, a.P, &next, a.remaining);
+    }
+    __syncthreads();
+  }
+  block_copy(a.st + j, &s, BATCH_JOB_BS);
+  if (tid == 0) a.ntiles[j] = s.status == 0 ? jd.lv[s.level].ntiles : 0;
 }
 
 // ------------------------------------------------------------ launchers
+hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s) {
+  const dim3 grid(a.njobs), block(BATCH_JOB_BS);
+  if (kind == LSDG_JOB_SIM3) {
+    if (acc == BATCH_ACC_KAHAN) {
+      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
+      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
+      else k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_KAHAN><<<grid, block, 0, s>>>(a);
+    } else {
+      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_F64><<<grid, block, 0, s>>>(a);
+      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_F64><<<grid, block, 0, s>>>(a);
+      else k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_F64><<<grid, block, 0, s>>>(a);
+    }
+  } else {
+    if (acc == BATCH_ACC_KAHAN) {
+      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
+      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
+      else k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_KAHAN><<<grid, block, 0, s>>>(a);
+    } else {
+      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_F64><<<grid, block, 0, s>>>(a);
+      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_F64><<<grid, block, 0, s>>>(a);
+      else k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_F64><<<grid, block, 0, s>>>(a);
+    }
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
   k_batch_init<<<dim3(1), dim3(256), 0, s>>>(a);
   return hipGetLastError();
@@ -431,8 +561,8 @@ hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, in
 }
 
 hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s) {
-  if (kind == LSDG_JOB_SIM3) k_batch_solve<LSDG_JOB_SIM3><<<dim3(a.njobs), dim3(64), 0, s>>>(a, parity);
-  else k_batch_solve<LSDG_JOB_SE3><<<dim3(a.njobs), dim3(64), 0, s>>>(a, parity);
+  if (kind == LSDG_JOB_SIM3) k_batch_solve<LSDG_JOB_SIM3><<<dim3(a.njobs), dim3(BATCH_SOLVE_BS), 0, s>>>(a, parity);
+  else k_batch_solve<LSDG_JOB_SE3><<<dim3(a.njobs), dim3(BATCH_SOLVE_BS), 0, s>>>(a, parity);
   return hipGetLastError();
 }
 

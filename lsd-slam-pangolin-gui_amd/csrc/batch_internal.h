@@ -8,6 +8,11 @@ namespace lsdg {
 constexpr int MAX_BATCH_JOBS = 2048;
 constexpr int BATCH_ACC_F64 = 0;
 constexpr int BATCH_ACC_KAHAN = 1;
+// Whole-job kernel: block size and the bound on the evaluations one job may
+// run inside it (the LM loop ends long before; the bound makes every block's
+// exit unconditional).
+constexpr int BATCH_JOB_BS = 512;
+constexpr int BATCH_JOB_MAX_EVALS = 1 << 16;
 
 // One pyramid level of one job: the reference's points, the frame's
 // (I, gx, gy) image (FP32 float4 or packed FP16/BF16) and, for Sim3, the
@@ -57,6 +62,7 @@ struct BatchArgs {
   double* partials;   // rows of NS doubles
   int* remaining;     // jobs still running
   int ppt;            // points per thread per tile
+  int jobMaxPoints;   // levels with at most this many points run in k_batch_job
   lsdg_params P;
   TrackKParams kp;
 };
@@ -65,5 +71,6 @@ hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s);
 hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
 hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
 hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s);
+hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s);
 
 }  // namespace lsdg

@@ -239,22 +239,54 @@ extern "C" lsdg_status lsdg_frame_destroy(lsdg_frame f) {
   return LSDG_OK;
 }
 
-// Algorithmic bytes of one pyramid build: level-0 input read, then per level
-// image, gradients (b_px) and max-gradient written.
-static double pyramid_bytes(lsdg_frame f) {
-  double b = (double)f->w[0] * f->h[0] * 4.0;
-  double bpx = f->precision == LSDG_FP32 ? 12.0 : 6.0;
-  for (int l = 0; l < PYR; ++l) b += (double)f->w[l] * f->h[l] * (4.0 + bpx + 4.0);
+// Algorithmic bytes of a pyramid build: the image stage reads level 0 and
+// writes levels 1-4; the gradient stage reads each built level's image and
+// writes what `grads` names (16 B float4, 8 B packed, 4 B max-gradient).
+double lsdg::pyramid_bytes(lsdg_frame f, bool images, unsigned grads) {
+  double b = 0.0;
+  if (images) {
+    b += (double)f->w[0] * f->h[0] * 4.0;
+    for (int l = 1; l < PYR; ++l) b += (double)f->w[l] * f->h[l] * 4.0;
+  }
+  for (int l = 0; l < PYR; ++l) {
+    double n = (double)f->w[l] * f->h[l], w = 0.0;
+    if (grads & fd_bit(FD_GRAD, l)) w += 16.0;
+    if (grads & fd_bit(FD_GRADH, l)) w += 8.0;
+    if (grads & fd_bit(FD_MAXG, l)) w += 4.0;
+    if (w > 0.0) b += n * (4.0 + w);
+  }
   return b;
 }
 
-static lsdg_status frame_pyramid(lsdg_frame f, const uint8_t* d_u8) {
+unsigned lsdg::fd_track(lsdg_frame f, int lo, int hi) {
+  return fd_levels(f->precision == LSDG_FP32 ? FD_GRAD : FD_GRADH, lo, hi);
+}
+
+lsdg_status lsdg::frame_require(lsdg_frame f, unsigned need) {
+  if (!f->d_gradh[0]) need &= ~fd_levels(FD_GRADH, 0, PYR - 1);
+  unsigned miss = need & ~f->valid;
+  if (!miss) return LSDG_OK;
   lsdg_ctx c = f->ctx;
+  const unsigned images = fd_levels(FD_IMAGE, 1, PYR - 1);
+  const unsigned derived = fd_levels(FD_GRAD, 0, PYR - 1) | fd_levels(FD_GRADH, 0, PYR - 1) |
+                           fd_levels(FD_MAXG, 0, PYR - 1);
+  const unsigned level0 = fd_bit(FD_IMAGE, 0) | fd_bit(FD_GRAD, 0) | fd_bit(FD_GRADH, 0) | fd_bit(FD_MAXG, 0);
+  // data above level 0 reads the image pyramid, which one launch builds whole
+  bool buildImages = (f->valid & images) != images && (miss & ~level0) != 0;
+  unsigned grads = miss & derived;
   ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);
-  CK(c, launch_pyr_images(f, d_u8, c->stream));
-  CK(c, launch_pyr_grads(f, c->stream));
-  prof_end(c, pm, pyramid_bytes(f));
+  if (buildImages) CK(c, launch_pyr_images(f, nullptr, c->stream));
+  if (grads) CK(c, launch_pyr_grads(f, grads, c->stream));
+  prof_end(c, pm, pyramid_bytes(f, buildImages, grads));
+  f->valid |= (buildImages ? images : 0u) | grads;
   return LSDG_OK;
+}
+
+// Every pyramid level of every kind the frame can hold.
+static unsigned fd_all(lsdg_frame f) {
+  unsigned m = fd_levels(FD_IMAGE, 0, PYR - 1) | fd_levels(FD_GRAD, 0, PYR - 1) | fd_levels(FD_MAXG, 0, PYR - 1);
+  if (f->d_gradh[0]) m |= fd_levels(FD_GRADH, 0, PYR - 1);
+  return m;
 }
 
 extern "C" lsdg_status lsdg_frame_create(lsdg_ctx c, int32_t id, const lsdg_camera* cam, const float* image,
@@ -265,7 +297,7 @@ extern "C" lsdg_status lsdg_frame_create(lsdg_ctx c, int32_t id, const lsdg_came
   if (st != LSDG_OK) return st;
   size_t n = (size_t)cam->w * cam->h;
   hipError_t e = hipMemcpyAsync(f->d_image[0], image, n * 4, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) st = frame_pyramid(f, nullptr);
+  if (e == hipSuccess) f->valid = fd_bit(FD_IMAGE, 0);  // the rest is built on first use
   else st = fail(c, LSDG_EDEVICE, "upload image", e);
   if (st == LSDG_OK) {
     e = hipStreamSynchronize(c->stream);
@@ -287,8 +319,16 @@ extern "C" lsdg_status lsdg_frame_create_u8(lsdg_ctx c, int32_t id, const lsdg_c
   if (st != LSDG_OK) return st;
   hipError_t e = hipMemcpy2DAsync(f->d_input_u8, cam->w, image, stride, cam->w, cam->h, hipMemcpyHostToDevice,
                                   c->stream);
-  if (e == hipSuccess) st = frame_pyramid(f, f->d_input_u8);
-  else st = fail(c, LSDG_EDEVICE, "upload image", e);
+  if (e == hipSuccess) {
+    // the image stage converts level 0 and builds levels 1-4; the rest is built on first use
+    ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);
+    e = launch_pyr_images(f, f->d_input_u8, c->stream);
+    prof_end(c, pm, pyramid_bytes(f, true, 0));
+    if (e == hipSuccess) f->valid = fd_levels(FD_IMAGE, 0, PYR - 1);
+    else st = fail(c, LSDG_EDEVICE, "frame pyramid", e);
+  } else {
+    st = fail(c, LSDG_EDEVICE, "upload image", e);
+  }
   if (st == LSDG_OK) {
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) st = fail(c, LSDG_EDEVICE, "frame pyramid", e);
@@ -303,9 +343,16 @@ extern "C" lsdg_status lsdg_frame_create_u8(lsdg_ctx c, int32_t id, const lsdg_c
 
 extern "C" lsdg_status lsdg_frame_build_pyramid(lsdg_frame f) {
   if (!f) return LSDG_EINVAL;
-  lsdg_status st = frame_pyramid(f, nullptr);
+  f->valid &= fd_bit(FD_IMAGE, 0);
+  lsdg_status st = frame_require(f, fd_all(f));
   if (st != LSDG_OK) return st;
   CK(f->ctx, hipStreamSynchronize(f->ctx->stream));
+  return LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_frame_invalidate(lsdg_frame f) {
+  if (!f) return LSDG_EINVAL;
+  f->valid &= fd_bit(FD_IMAGE, 0);
   return LSDG_OK;
 }
 
@@ -343,10 +390,11 @@ extern "C" lsdg_status lsdg_frame_read(lsdg_frame f, int32_t what, int32_t level
   size_t n = (size_t)f->w[level] * f->h[level];
   const void* src = nullptr;
   size_t need = n * 4;
+  unsigned req = 0;
   switch (what) {
-    case LSDG_WHAT_IMAGE: src = f->d_image[level]; break;
-    case LSDG_WHAT_GRADIENTS: src = f->d_grad[level]; need = n * 16; break;
-    case LSDG_WHAT_MAXGRAD: src = f->d_maxg[level]; break;
+    case LSDG_WHAT_IMAGE: src = f->d_image[level]; req = fd_bit(FD_IMAGE, level); break;
+    case LSDG_WHAT_GRADIENTS: src = f->d_grad[level]; need = n * 16; req = fd_bit(FD_GRAD, level); break;
+    case LSDG_WHAT_MAXGRAD: src = f->d_maxg[level]; req = fd_bit(FD_MAXG, level); break;
     case LSDG_WHAT_IDEPTH:
     case LSDG_WHAT_IDEPTH_VAR:
       if (!f->hasDepth) return fail(c, LSDG_ESTATE, "frame has no depth");
@@ -364,6 +412,8 @@ extern "C" lsdg_status lsdg_frame_read(lsdg_frame f, int32_t what, int32_t level
     default: return fail(c, LSDG_EINVAL, "unknown 'what'");
   }
   if (bytes < need) return fail(c, LSDG_EINVAL, "output buffer too small");
+  lsdg_status st = frame_require(f, req);
+  if (st != LSDG_OK) return st;
   CK(c, hipMemcpyAsync(out, src, need, hipMemcpyDeviceToHost, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   return LSDG_OK;
@@ -444,6 +494,8 @@ lsdg_status lsdg::ref_points(lsdg_ref r, int level) {
     r->depthVersion = r->kf->depthVersion;
   }
   if (r->num[level] >= 0) return LSDG_OK;
+  lsdg_status st = frame_require(r->kf, fd_bit(FD_IMAGE, level) | fd_bit(FD_GRAD, level));
+  if (st != LSDG_OK) return st;
   CK(c, launch_point_cloud(r, level, c->stream));
   return LSDG_OK;
 }
@@ -535,6 +587,8 @@ static lsdg_status se3_eval_sync(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int lvl, 
     std::memset(S, 0, sizeof(double) * NS_SE3);
     return LSDG_OK;
   }
+  st = frame_require(f, fd_track(f, lvl, lvl));
+  if (st != LSDG_OK) return st;
   TrackKParams kp = track_kparams(c->P, false);
   kp.aff_a = aa;
   kp.aff_b = ab;
@@ -745,6 +799,8 @@ static lsdg_status sim3_eval_sync(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int lvl,
     std::memset(S, 0, sizeof(double) * NS_SIM3);
     return LSDG_OK;
   }
+  st = frame_require(f, fd_track(f, lvl, lvl));
+  if (st != LSDG_OK) return st;
   TrackKParams kp = track_kparams(c->P, true);
   kp.aff_a = aa;
   kp.aff_b = ab;
@@ -929,6 +985,8 @@ extern "C" lsdg_status lsdg_energy_grad(lsdg_ctx c, lsdg_ref r, lsdg_frame f, in
   if (st != LSDG_OK) return st;
   const PointsLevel& pl = r->lv[level];
   if (grad_idepth && max_points < pl.n) return fail(c, LSDG_EINVAL, "grad_idepth buffer too small");
+  st = frame_require(f, fd_track(f, level, level));
+  if (st != LSDG_OK) return st;
   float* d_g = nullptr;
   CK(c, hipMalloc(&d_g, sizeof(float) * (pl.n > 0 ? pl.n : 1)));
   TrackKParams kp = track_kparams(c->P, false);

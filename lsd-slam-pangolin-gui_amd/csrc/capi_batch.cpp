@@ -2,10 +2,13 @@
 //
 // A plan holds the device job table, one LM control block per job, the
 // per-tile partial-sum rows and a hipGraph of
-//     [batched pyramid rebuild] -> init -> K x k_batch_step -> copy of the
-//     remaining-jobs counter to host memory.
+//     [batched pyramid rebuild] -> init -> k_batch_job (levels with at most
+//     jobMaxPoints points, one block per job) -> K x (k_batch_eval,
+//     k_batch_solve) for larger levels -> copy of the remaining-jobs counter
+//     to host memory.
 // lsdg_batch_run replays it (plus K-step continuation graphs while jobs are
-// still running) and copies the control blocks back into lsdg_job_result.
+// still running), sizes K to what the workload needs, and copies the control
+// blocks back into lsdg_job_result.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -26,7 +29,11 @@ struct lsdg_batch_s {
   int acc = BATCH_ACC_F64;
   int ppt = 4;
   int grid = 1024;
-  int steps = 16;  // LM steps per graph launch (even)
+  int steps = 16;  // multi-block LM steps in the first graph (even, may be 0)
+  int contSteps = 16;  // LM steps per continuation graph (even, > 0)
+  // Levels of up to this many points run in k_batch_job (12 points per
+  // thread of its 512-thread block); larger ones in the multi-block steps.
+  int jobMaxPoints = 12 * BATCH_JOB_BS;
   std::vector<lsdg_job> jobs;
   std::vector<int> depthVersion;
   std::vector<JobDesc> h_jobs;
@@ -40,6 +47,7 @@ struct lsdg_batch_s {
   size_t rows = 0;
   // pyramid rebuild
   std::vector<lsdg_frame> pyrFrames;
+  std::vector<unsigned> pyrNeed;  // data the rebuild produces per frame (fd_bit flags)
   void* d_pyr = nullptr;
   void* h_pyr = nullptr;
   PyrBatch pyr{};
@@ -126,6 +134,7 @@ static BatchArgs batch_args(lsdg_batch b) {
   a.partials = b->d_partials;
   a.remaining = b->d_remaining;
   a.ppt = b->ppt;
+  a.jobMaxPoints = b->jobMaxPoints;
   a.P = b->ctx->P;
   a.kp = track_kparams(b->ctx->P, b->kind == LSDG_JOB_SIM3);
   return a;
@@ -141,8 +150,10 @@ static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec
   if (first) {
     if (b->pyr.nf > 0) e = launch_pyr_batch(b->pyr, c->stream);
     if (e == hipSuccess) e = launch_batch_init(a, c->stream);
+    if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, c->stream);
   }
-  for (int s = 0; s < b->steps && e == hipSuccess; ++s)
+  const int steps = first ? b->steps : b->contSteps;
+  for (int s = 0; s < steps && e == hipSuccess; ++s)
     e = launch_batch_step(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->h_remaining, b->d_remaining, sizeof(int), hipMemcpyDeviceToHost, c->stream);
@@ -207,7 +218,8 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   if (const char* e = std::getenv("LSDG_BATCH_ACC")) b->acc = std::atoi(e) ? BATCH_ACC_KAHAN : BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_PPT")) b->ppt = std::max(1, std::min(64, std::atoi(e)));
   if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));
-  if (const char* e = std::getenv("LSDG_BATCH_STEPS")) b->steps = std::max(2, std::min(512, std::atoi(e) & ~1));
+  if (const char* e = std::getenv("LSDG_BATCH_STEPS")) b->steps = std::max(0, std::min(512, std::atoi(e) & ~1));
+  if (const char* e = std::getenv("LSDG_BATCH_JOB_POINTS")) b->jobMaxPoints = std::max(0, std::atoi(e));
   b->jobs.assign(jobs, jobs + n);
   for (auto& J : b->jobs) {
     if (kind == LSDG_JOB_SE3) {
@@ -227,17 +239,35 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
     batch_free(b);
     return fail(c, LSDG_ENOMEM, "batch allocation", e);
   }
-  // frames whose pyramid the batch rebuilds
-  for (auto& J : b->jobs)
+  // pyramid data each frame is tracked on (union over its jobs), built now so
+  // the frame holds it; frames the batch rebuilds get it again on every run
+  std::vector<lsdg_frame> frames;
+  std::vector<unsigned> needs;
+  for (auto& J : b->jobs) {
+    unsigned m = fd_levels(FD_IMAGE, 0, PYR - 1) | fd_track(J.frame, J.final_level, J.start_level);
+    auto it = std::find(frames.begin(), frames.end(), J.frame);
+    if (it == frames.end()) {
+      frames.push_back(J.frame);
+      needs.push_back(m);
+    } else {
+      needs[it - frames.begin()] |= m;
+    }
     if (J.rebuild_pyramid && std::find(b->pyrFrames.begin(), b->pyrFrames.end(), J.frame) == b->pyrFrames.end())
       b->pyrFrames.push_back(J.frame);
+  }
+  for (size_t k = 0; k < frames.size(); ++k)
+    if ((st = frame_require(frames[k], needs[k])) != LSDG_OK) {
+      batch_free(b);
+      return st;
+    }
+  for (lsdg_frame f : b->pyrFrames) b->pyrNeed.push_back(needs[std::find(frames.begin(), frames.end(), f) - frames.begin()]);
   if (!b->pyrFrames.empty()) {
     size_t bytes = pyr_batch_bytes((int)b->pyrFrames.size());
     if ((e = hipMalloc(&b->d_pyr, bytes)) != hipSuccess || (e = hipHostMalloc(&b->h_pyr, bytes)) != hipSuccess) {
       batch_free(b);
       return fail(c, LSDG_ENOMEM, "batch pyramid tables", e);
     }
-    pyr_batch_tables(b->pyrFrames.data(), (int)b->pyrFrames.size(), b->h_pyr, b->d_pyr, &b->pyr);
+    pyr_batch_tables(b->pyrFrames.data(), b->pyrNeed.data(), (int)b->pyrFrames.size(), b->h_pyr, b->d_pyr, &b->pyr);
     if ((e = hipMemcpy(b->d_pyr, b->h_pyr, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
       batch_free(b);
       return fail(c, LSDG_EDEVICE, "batch pyramid tables", e);
@@ -248,7 +278,18 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
       batch_free(b);
       return st;
     }
-  if ((st = build_job_table(b)) != LSDG_OK || (st = batch_graphs(b)) != LSDG_OK) {
+  if ((st = build_job_table(b)) != LSDG_OK) {
+    batch_free(b);
+    return st;
+  }
+  if (!std::getenv("LSDG_BATCH_STEPS")) {
+    // no multi-block steps in the first graph when every level runs in k_batch_job
+    int maxPts = 0;
+    for (const JobDesc& d : b->h_jobs)
+      for (int l = 0; l < PYR; ++l) maxPts = std::max(maxPts, d.lv[l].pts.n);
+    if (maxPts <= b->jobMaxPoints) b->steps = 0;
+  }
+  if ((st = batch_graphs(b)) != LSDG_OK) {
     batch_free(b);
     return st;
   }
@@ -328,18 +369,20 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
   BatchArgs a = batch_args(b);
   if (b->pyr.nf > 0) {
     double pb = 0.0;
-    for (lsdg_frame f : b->pyrFrames) {
-      double bpx = f->precision == LSDG_FP32 ? 12.0 : 6.0;
-      pb += (double)f->w[0] * f->h[0] * 4.0;
-      for (int l = 0; l < PYR; ++l) pb += (double)f->w[l] * f->h[l] * (4.0 + bpx + 4.0);
-    }
+    for (size_t k = 0; k < b->pyrFrames.size(); ++k) pb += pyramid_bytes(b->pyrFrames[k], true, b->pyrNeed[k]);
     ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);
     CK(c, launch_pyr_batch(b->pyr, c->stream));
     prof_end(c, pm, pb);
   }
   CK(c, launch_batch_init(a, c->stream));
+  if (b->jobMaxPoints > 0) {
+    ProfMark pj = prof_begin(c, LSDG_KERNEL_BATCH_JOB);
+    CK(c, launch_batch_job(a, b->kind, b->prec, b->acc, c->stream));
+    prof_end(c, pj, 0.0);
+  }
   for (int round = 0; round <= 64; ++round) {
-    for (int s = 0; s < b->steps; ++s) {
+    const int steps = round == 0 ? b->steps : b->contSteps;
+    for (int s = 0; s < steps; ++s) {
       ProfMark pm = prof_begin(c, LSDG_KERNEL_BATCH_EVAL);
       CK(c, launch_batch_eval(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream));
       prof_end(c, pm, 0.0);
@@ -377,32 +420,40 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
       CK(c, hipGraphLaunch(b->cont, c->stream));
       CK(c, hipStreamSynchronize(c->stream));
     }
-    if (launches > 0 && b->steps < 128) {
-      // the graph was too short for this workload: size the next run's graph
-      // to the steps this one needed (LM steps = evaluations of the longest job)
-      CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
-      int need = 0;
-      for (int j = 0; j < b->n; ++j) need = std::max(need, b->h_st[j].evals);
-      int steps = std::min(128, ((need + 2) + 1) & ~1);
-      if (steps > b->steps) {
-        b->steps = steps;
-        lsdg_status st = batch_graphs(b);
-        if (st != LSDG_OK) return st;
-      }
+    // Size the first graph's multi-block steps to what this run needed (the
+    // most evaluations one job ran at levels above jobMaxPoints), when it was
+    // too short or more than a few steps too long.
+    CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
+    int need = 0;
+    for (int j = 0; j < b->n; ++j) {
+      int e = 0;
+      for (int l = 0; l < PYR; ++l)
+        if (b->h_jobs[j].lv[l].pts.n > b->jobMaxPoints) e += b->h_st[j].evalsAt[l];
+      need = std::max(need, e);
+    }
+    const int steps = std::min(128, (need + 1) & ~1);
+    if ((launches > 0 && steps > b->steps) || steps + 4 < b->steps) {
+      b->steps = steps;
+      lsdg_status st = batch_graphs(b);
+      if (st != LSDG_OK) return st;
     }
   }
   if (results || c->prof_on) {
-    CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
+    if (c->prof_on) CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
     if (results)
       for (int j = 0; j < b->n; ++j) fill_result(b, j, b->h_st[j], &results[j]);
     if (c->prof_on) {
-      // algorithmic bytes of every evaluation the steps ran
-      double bytes = 0.0;
+      // algorithmic bytes of every evaluation, split by the kernel that ran it
+      double bytesSplit = 0.0, bytesJob = 0.0;
       for (int j = 0; j < b->n; ++j)
-        for (int l = 0; l < PYR; ++l)
-          bytes += b->h_st[j].evalsAt[l] *
-                   eval_bytes(b->h_jobs[j].lv[l].pts.n, b->jobs[j].frame, l, b->kind == LSDG_JOB_SIM3);
-      c->prof[LSDG_KERNEL_BATCH_EVAL].bytes += bytes;
+        for (int l = 0; l < PYR; ++l) {
+          const int np = b->h_jobs[j].lv[l].pts.n;
+          const double eb = b->h_st[j].evalsAt[l] * eval_bytes(np, b->jobs[j].frame, l, b->kind == LSDG_JOB_SIM3);
+          if (np > b->jobMaxPoints) bytesSplit += eb;
+          else bytesJob += eb;
+        }
+      c->prof[LSDG_KERNEL_BATCH_EVAL].bytes += bytesSplit;
+      c->prof[LSDG_KERNEL_BATCH_JOB].bytes += bytesJob;
     }
   }
   return LSDG_OK;

@@ -7,6 +7,15 @@ namespace lsdg {
 // Records `what` (and the HIP error) as the context's last error; returns st.
 lsdg_status fail(lsdg_ctx c, lsdg_status st, const char* what, hipError_t e = hipSuccess);
 lsdg_status frame_depth_from_level0(lsdg_frame f);
+// Builds the pyramid data in `need` (fd_bit flags) that the frame does not
+// hold yet, on the context stream (Frame::require).
+lsdg_status frame_require(lsdg_frame f, unsigned need);
+// The (I, gx, gy) image a tracker samples at levels lo..hi: FP32 float4 or
+// the packed reduced-precision copy.
+unsigned fd_track(lsdg_frame f, int lo, int hi);
+// Algorithmic bytes of a pyramid build (image stage if `images`, gradient
+// stage for the fd_bit data in `grads`).
+double pyramid_bytes(lsdg_frame f, bool images, unsigned grads);
 lsdg_status frame_ensure_was_good(lsdg_frame f);
 lsdg_status ref_points(lsdg_ref r, int level);
 Xf pose_xf(const Pose& T, bool scaled);

@@ -13,6 +13,10 @@
 
 using namespace lsdg;
 
+// Keyframe data the depth map reads: level-0 gradients (epipolar checks) and
+// max-gradients (observe / fill-holes gates).
+static constexpr unsigned KF_DATA = fd_bit(FD_GRAD, 0) | fd_bit(FD_MAXG, 0);
+
 static void free_dm(lsdg_depthmap dm) {
   if (dm->d_pool) (void)hipFree(dm->d_pool);
   if (dm->d_refs) (void)hipFree(dm->d_refs);
@@ -250,6 +254,8 @@ extern "C" lsdg_status lsdg_depth_update(lsdg_depthmap dm, const lsdg_frame* ref
   CK(c, hipMemcpyAsync(dm->d_refs, dm->h_refs, sizeof(StereoRef) * n, hipMemcpyHostToDevice, c->stream));
   CK(c, hipMemcpyAsync(dm->d_refById, dm->h_refById, sizeof(int) * byId.size(), hipMemcpyHostToDevice, c->stream));
   const lsdg_params& P = c->P;
+  st = frame_require(kf, KF_DATA);
+  if (st != LSDG_OK) return st;
   ProfMark pm = prof_begin(c, LSDG_KERNEL_DEPTH);
   CK(c, launch_observe(dm, P, dm->d_refs, dm->d_refById, (int)byId.size(), offset, 0, n - 1,
                        (float)kf->numFramesTrackedOnThis, (float)kf->numMappedOnThis, c->stream));
@@ -277,6 +283,8 @@ extern "C" lsdg_status lsdg_depth_create_keyframe(lsdg_depthmap dm, lsdg_frame n
   t2p.s = 1.0;
   Pose oldToNew = pose_inv(t2p);
   const uint8_t* wg = (nkf->parentId == dm->kf->id && nkf->wasGoodAllocated) ? nkf->d_wasGood : nullptr;
+  st = frame_require(nkf, KF_DATA);
+  if (st != LSDG_OK) return st;
   ProfMark pm = prof_begin(c, LSDG_KERNEL_DEPTH);
   CK(c, launch_propagate(dm, P, nkf, pose_xf(oldToNew, false), wg, c->stream));
   dm->kf = nkf;
@@ -303,6 +311,8 @@ extern "C" lsdg_status lsdg_depth_finalize_keyframe(lsdg_depthmap dm) {
   if (!dm) return LSDG_EINVAL;
   lsdg_ctx c = dm->ctx;
   if (!dm->valid || !dm->kf) return fail(c, LSDG_ESTATE, "depth map has no active keyframe");
+  lsdg_status st0 = frame_require(dm->kf, KF_DATA);
+  if (st0 != LSDG_OK) return st0;
   CK(c, launch_fill_holes(dm, c->P, c->stream));
   CK(c, launch_regularize(dm, c->P, false, c->P.val_sum_min_for_keep, c->stream));
   lsdg_status st = set_depth(dm, dm->kf);

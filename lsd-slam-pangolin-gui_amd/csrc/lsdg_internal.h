@@ -27,6 +27,16 @@ constexpr int NS_SE3 = 40;
 constexpr int NS_SIM3 = 48;
 constexpr int NS_MAX = 48;
 
+// Pyramid data of a frame, one validity bit per (kind, level): the
+// reference's Frame::require() flags (IMAGE, GRADIENTS, MAX_GRADIENTS), built
+// on first use.  FD_GRADH is the packed FP16/BF16 (gx, gy, I) copy that
+// reduced-precision frames track on.
+enum FrameData : unsigned { FD_IMAGE = 0, FD_GRAD = PYR, FD_GRADH = 2 * PYR, FD_MAXG = 3 * PYR };
+constexpr unsigned fd_bit(unsigned kind, int l) { return 1u << (kind + (unsigned)l); }
+constexpr unsigned fd_levels(unsigned kind, int lo, int hi) {
+  return lo > hi ? 0u : (fd_bit(kind, lo) | fd_levels(kind, lo + 1, hi));
+}
+
 enum Se3Slot { S3_A = 0, S3_B = 21, S3_ERR = 27, S3_RESUNW = 28, S3_SIGNED = 29, S3_USAGE = 30, S3_SXX = 31,
                S3_SYY = 32, S3_SX = 33, S3_SY = 34, S3_SW = 35, S3_WARPED = 36, S3_GOOD = 37, S3_BAD = 38 };
 enum Sim3Slot { SM_A = 0, SM_B = 28, SM_RESP = 35, SM_RESD = 36, SM_NTD = 37, SM_WARPED = 38, SM_USAGE = 39,
@@ -159,6 +169,7 @@ struct lsdg_frame_s {
   float* d_ivar[lsdg::PYR];
   uint8_t* d_input_u8 = nullptr;  // staging for u8 uploads (w*h)
   uint8_t* d_wasGood = nullptr;   // refPixelWasGood (level 1)
+  unsigned valid = 0;             // built pyramid data (fd_bit flags)
   bool wasGoodAllocated = false;
   bool hasDepth = false;
   // pose fields
@@ -222,12 +233,14 @@ namespace lsdg {
 
 // pyramid.hip
 hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s);
-hipError_t launch_pyr_grads(lsdg_frame f, hipStream_t s);
+// Gradients / packed copies / max-gradients of the levels named in `need`
+// (FD_GRAD, FD_GRADH, FD_MAXG bits); other levels launch no blocks.
+hipError_t launch_pyr_grads(lsdg_frame f, unsigned need, hipStream_t s);
 hipError_t launch_pyr_depth(lsdg_frame f, hipStream_t s);
 hipError_t launch_depth_normalize(float* idepth, float* ivar, int n, hipStream_t s);
 hipError_t launch_point_cloud(lsdg_ref r, int level, hipStream_t s);  // fills r->lv[level], r->num[level]
 size_t pyr_batch_bytes(int nf);
-void pyr_batch_tables(lsdg_frame const* fs, int nf, void* h, void* d, PyrBatch* out);
+void pyr_batch_tables(lsdg_frame const* fs, const unsigned* need, int nf, void* h, void* d, PyrBatch* out);
 hipError_t launch_pyr_batch(const PyrBatch& b, hipStream_t s);
 
 // track.hip

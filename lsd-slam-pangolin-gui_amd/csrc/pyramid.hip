@@ -147,8 +147,9 @@ __device__ inline void pyr_grads_block(const PyrGradArgs& p, int b) {
     g.y = 0.5f * (I[idx + w] - I[idx - w]);
     g.z = I[idx];
   }
-  p.grad[l][idx] = g;
+  if (p.grad[l]) p.grad[l][idx] = g;
   if (p.gradh[l]) p.gradh[l][idx] = pack_grad(g, p.prec);
+  if (!p.maxg[l]) return;
   float m;
   if (idx >= w + 1 && idx < w * (h - 1) - 1) {
     float g1 = vmax_at(I, idx - 1, w, h), g2 = vmax_at(I, idx, w, h);
@@ -382,21 +383,29 @@ hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_pyr_grads(lsdg_frame f, hipStream_t s) {
-  PyrGradArgs p;
+// Fills the gradient-stage arguments of frame f for the data in `need`;
+// returns the number of blocks (levels with nothing to build get none).
+static int pyr_grad_args(lsdg_frame f, unsigned need, PyrGradArgs& p) {
   int blk = 0;
   for (int l = 0; l < PYR; ++l) {
     p.img[l] = f->d_image[l];
-    p.grad[l] = f->d_grad[l];
-    p.gradh[l] = f->d_gradh[l];
-    p.maxg[l] = f->d_maxg[l];
+    p.grad[l] = (need & fd_bit(FD_GRAD, l)) ? f->d_grad[l] : nullptr;
+    p.gradh[l] = (need & fd_bit(FD_GRADH, l)) ? f->d_gradh[l] : nullptr;
+    p.maxg[l] = (need & fd_bit(FD_MAXG, l)) ? f->d_maxg[l] : nullptr;
     p.w[l] = f->w[l];
     p.h[l] = f->h[l];
     p.blk0[l] = blk;
-    blk += (f->w[l] * f->h[l] + 255) / 256;
+    if (p.grad[l] || p.gradh[l] || p.maxg[l]) blk += (f->w[l] * f->h[l] + 255) / 256;
   }
   p.blk0[PYR] = blk;
   p.prec = f->precision;
+  return blk;
+}
+
+hipError_t launch_pyr_grads(lsdg_frame f, unsigned need, hipStream_t s) {
+  PyrGradArgs p;
+  int blk = pyr_grad_args(f, need, p);
+  if (blk == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pyr_grads, dim3(blk), dim3(256), 0, s, p);
   return hipGetLastError();
 }
@@ -408,7 +417,7 @@ size_t pyr_batch_bytes(int nf) {
 
 // Writes the argument tables for frames fs[0..nf) into host memory `h` (laid
 // out as the device copy at `d`), returns the block totals of both stages.
-void pyr_batch_tables(lsdg_frame const* fs, int nf, void* h, void* d, PyrBatch* out) {
+void pyr_batch_tables(lsdg_frame const* fs, const unsigned* need, int nf, void* h, void* d, PyrBatch* out) {
   char* hb = (char*)h;
   PyrImgArgs* ia = (PyrImgArgs*)hb;
   PyrGradArgs* ga = (PyrGradArgs*)(hb + sizeof(PyrImgArgs) * nf);
@@ -420,22 +429,12 @@ void pyr_batch_tables(lsdg_frame const* fs, int nf, void* h, void* d, PyrBatch* 
     lsdg_frame f = fs[k];
     PyrImgArgs p;
     PyrGradArgs q;
-    int blk = 0;
     for (int l = 0; l < PYR; ++l) {
       p.img[l] = f->d_image[l];
       p.w[l] = f->w[l];
       p.h[l] = f->h[l];
-      q.img[l] = f->d_image[l];
-      q.grad[l] = f->d_grad[l];
-      q.gradh[l] = f->d_gradh[l];
-      q.maxg[l] = f->d_maxg[l];
-      q.w[l] = f->w[l];
-      q.h[l] = f->h[l];
-      q.blk0[l] = blk;
-      blk += (f->w[l] * f->h[l] + 255) / 256;
     }
-    q.blk0[PYR] = blk;
-    q.prec = f->precision;
+    int blk = pyr_grad_args(f, need[k], q);
     ia[k] = p;
     ga[k] = q;
     int tx = (f->w[0] + 31) / 32, ty = (f->h[0] + 31) / 32;
@@ -462,7 +461,8 @@ hipError_t launch_pyr_batch(const PyrBatch& b, hipStream_t s) {
   if (b.nf == 0) return hipSuccess;
   k_pyr_images_batch<<<dim3(b.img_blocks), dim3(256), 0, s>>>((const PyrImgArgs*)b.img_args, b.img_off, b.img_tx,
                                                                b.nf);
-  k_pyr_grads_batch<<<dim3(b.grad_blocks), dim3(256), 0, s>>>((const PyrGradArgs*)b.grad_args, b.grad_off, b.nf);
+  if (b.grad_blocks > 0)
+    k_pyr_grads_batch<<<dim3(b.grad_blocks), dim3(256), 0, s>>>((const PyrGradArgs*)b.grad_args, b.grad_off, b.nf);
   return hipGetLastError();
 }
 

@@ -69,8 +69,8 @@ __global__ __launch_bounds__(EVAL_BS) void k_se3_eval(PointsLevel pts, const voi
                                                       double* __restrict__ out) {
   Accum<ACC, NS_SE3> acc;
   acc.zero();
-  for (int i = blockIdx.x * EVAL_BS + threadIdx.x; i < pts.n; i += gridDim.x * EVAL_BS)
-    se3_point<PREC>(pts, i, img, g, x, kp, isGood, acc);
+  se3_points<PREC, EVAL_U>(pts, blockIdx.x * EVAL_BS + threadIdx.x, gridDim.x * EVAL_BS, pts.n, img, g, x, kp, isGood,
+                           acc);
   reduce_and_merge(acc, partials, counter, out);
 }
 
@@ -83,8 +83,8 @@ __global__ __launch_bounds__(EVAL_BS) void k_sim3_eval(PointsLevel pts, const vo
                                                        double* __restrict__ out) {
   Accum<ACC, NS_SIM3> acc;
   acc.zero();
-  for (int i = blockIdx.x * EVAL_BS + threadIdx.x; i < pts.n; i += gridDim.x * EVAL_BS)
-    sim3_point<PREC>(pts, i, img, fid, fvar, g, x, kp, acc);
+  sim3_points<PREC, EVAL_U>(pts, blockIdx.x * EVAL_BS + threadIdx.x, gridDim.x * EVAL_BS, pts.n, img, fid, fvar, g, x,
+                            kp, acc);
   reduce_and_merge(acc, partials, counter, out);
 }
 

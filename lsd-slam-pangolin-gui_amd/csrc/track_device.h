@@ -6,12 +6,15 @@
 // calcResidualAndBuffers / calcWeightsAndResidual / calculateWarpUpdate and the
 // Sim3Tracker counterparts) so masks and counts match the CPU restatement.
 #pragma once
+#include <type_traits>
+
 #include "lsdg_internal.h"
 
 namespace lsdg {
 
 constexpr int EVAL_BS = 256;
 constexpr int EVAL_WAVES = EVAL_BS / 64;
+constexpr int EVAL_U = 2;  // points per thread with loads in flight together (plain kernels)
 
 struct Taps {
   float gx, gy, I;
@@ -51,6 +54,77 @@ __device__ inline Taps interp3(const void* img, float x, float y, int w) {
   o.gx = w11 * a11.gx + w01 * a01.gx + w10 * a10.gx + w00 * a00.gx;
   o.gy = w11 * a11.gy + w01 * a01.gy + w10 * a10.gy + w00 * a00.gy;
   o.I = w11 * a11.I + w01 * a01.I + w10 * a10.I + w00 * a00.I;
+  return o;
+}
+
+// The four raw taps of one bilinear sample, fetched ahead of their use so a
+// thread can keep several points' loads in flight.
+template <int PREC>
+struct TapRaw {
+  using T = typename std::conditional<PREC == LSDG_FP32, float4, uint2>::type;
+  T a11, a01, a10, a00;
+  float dx, dy;
+};
+
+template <int PREC>
+__device__ inline Taps decode_tap(const typename TapRaw<PREC>::T& g) {
+  if constexpr (PREC == LSDG_FP32) {
+    return {g.x, g.y, g.z};
+  } else if constexpr (PREC == LSDG_FP16) {
+    return {__half2float(__ushort_as_half((unsigned short)(g.x & 0xFFFFu))),
+            __half2float(__ushort_as_half((unsigned short)(g.x >> 16))),
+            __half2float(__ushort_as_half((unsigned short)(g.y & 0xFFFFu)))};
+  } else {
+    return {bf16_to_f32((unsigned short)(g.x & 0xFFFFu)), bf16_to_f32((unsigned short)(g.x >> 16)),
+            bf16_to_f32((unsigned short)(g.y & 0xFFFFu))};
+  }
+}
+
+template <int PREC>
+__device__ inline void fetch_taps(const void* img, float x, float y, int w, TapRaw<PREC>& t) {
+  using T = typename TapRaw<PREC>::T;
+  const T* im = (const T*)img;
+  int ix = (int)x, iy = (int)y;
+  t.dx = x - (float)ix;
+  t.dy = y - (float)iy;
+  int b = ix + iy * w;
+  t.a11 = im[b + 1 + w];
+  t.a01 = im[b + w];
+  t.a10 = im[b + 1];
+  t.a00 = im[b];
+}
+
+// interp3's weighted sum of fetched taps (same operations, same order).
+template <int PREC>
+__device__ inline Taps blend_taps(const TapRaw<PREC>& t) {
+  float dx = t.dx, dy = t.dy;
+  float dxdy = dx * dy;
+  Taps a11 = decode_tap<PREC>(t.a11), a01 = decode_tap<PREC>(t.a01);
+  Taps a10 = decode_tap<PREC>(t.a10), a00 = decode_tap<PREC>(t.a00);
+  float w11 = dxdy, w01 = dy - dxdy, w10 = dx - dxdy, w00 = 1.0f - dx - dy + dxdy;
+  Taps o;
+  o.gx = w11 * a11.gx + w01 * a01.gx + w10 * a10.gx + w00 * a00.gx;
+  o.gy = w11 * a11.gy + w01 * a01.gy + w10 * a10.gy + w00 * a00.gy;
+  o.I = w11 * a11.I + w01 * a01.I + w10 * a10.I + w00 * a00.I;
+  return o;
+}
+
+// Projection of one reference point into the frame (the warp every tracker
+// evaluation starts with).
+struct Proj {
+  float Wx, Wy, Wz, u, v;
+  bool in;
+};
+
+__device__ inline Proj project(const float4& p, const Xf& x, const LevelGeom& g) {
+  const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
+  Proj o;
+  o.Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
+  o.Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
+  o.Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
+  o.u = (o.Wx / o.Wz) * g.fx + g.cx;
+  o.v = (o.Wy / o.Wz) * g.fy + g.cy;
+  o.in = o.u > 1.0f && o.v > 1.0f && o.u < wlim && o.v < hlim;
   return o;
 }
 
@@ -219,21 +293,17 @@ __host__ __device__ constexpr int tri7(int i, int j) { return i * 7 - i * (i - 1
 
 // One reference point of an SE3 evaluation: warp, bounds, taps, residual,
 // isGood, affine statistics, weights, Jacobian and normal-equation terms.
+// Rest of one SE3 point once its projection and taps are in hand.
 template <int PREC, class ACC>
-__device__ inline void se3_point(const PointsLevel& pts, int i, const void* __restrict__ img, const LevelGeom& g,
-                                 const Xf& x, const TrackKParams& kp, uint8_t* __restrict__ isGood, ACC& acc) {
-  const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
-  float4 p = pts.posI[i];
-  float Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
-  float Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
-  float Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
-  float u = (Wx / Wz) * g.fx + g.cx;
-  float v = (Wy / Wz) * g.fy + g.cy;
-  if (!(u > 1.0f && v > 1.0f && u < wlim && v < hlim)) {
-    if (isGood) isGood[pts.idx[i]] = 0;
+__device__ inline void se3_finish(const float4& p, float pvar, int pidx, const Proj& w, const TapRaw<PREC>& t,
+                                  const Xf& x, const LevelGeom& g, const TrackKParams& kp,
+                                  uint8_t* __restrict__ isGood, ACC& acc) {
+  const float Wx = w.Wx, Wy = w.Wy, Wz = w.Wz;
+  if (!w.in) {
+    if (isGood) isGood[pidx] = 0;
     return;
   }
-  Taps ri = interp3<PREC>(img, u, v, g.w);
+  Taps ri = blend_taps<PREC>(t);
   float c1 = kp.aff_a * p.w + kp.aff_b;
   float c2 = ri.I;
   float r = c1 - c2;
@@ -244,7 +314,7 @@ __device__ inline void se3_point(const PointsLevel& pts, int i, const void* __re
   acc.add(S3_SY, c2 * wgt);
   acc.add(S3_SW, wgt);
   bool good = r * r / (kp.max_diff_constant + kp.max_diff_grad_mult * (ri.gx * ri.gx + ri.gy * ri.gy)) < 1.0f;
-  if (isGood) isGood[pts.idx[i]] = good ? 1 : 0;
+  if (isGood) isGood[pidx] = good ? 1 : 0;
   float gx = g.fx * ri.gx, gy = g.fy * ri.gy;
   float d = 1.0f / p.z;
   if (good) {
@@ -258,7 +328,7 @@ __device__ inline void se3_point(const PointsLevel& pts, int i, const void* __re
   acc.add(S3_USAGE, dc < 1.0f ? dc : 1.0f);
   acc.add(S3_WARPED, 1.0f);
   // calcWeightsAndResidual
-  float s = kp.var_weight * pts.var[i];
+  float s = kp.var_weight * pvar;
   float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
   float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
   float drpdd = gx * g0 + gy * g1;
@@ -280,23 +350,53 @@ __device__ inline void se3_point(const PointsLevel& pts, int i, const void* __re
   acc.add(S3_ERR, r * r * W);
 }
 
-// One reference point of a Sim3 evaluation (photometric + depth residual).
+// SE3 evaluation of points i, i + stride, ... < end of one thread, U points
+// at a time: the U points' loads are issued, then their projections' taps,
+// then the U points are finished in index order (the per-thread summation
+// order of a one-point-at-a-time loop).
+template <int PREC, int U, class ACC>
+__device__ inline void se3_points(const PointsLevel& pts, int i, int stride, int end, const void* __restrict__ img,
+                                  const LevelGeom& g, const Xf& x, const TrackKParams& kp,
+                                  uint8_t* __restrict__ isGood, ACC& acc) {
+  for (; i < end; i += U * stride) {
+    float4 p[U];
+    float pv[U];
+    int pi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = i + u * stride;
+      if (k < end) {
+        p[u] = pts.posI[k];
+        pv[u] = pts.var[k];
+        pi[u] = isGood ? pts.idx[k] : 0;
+      }
+    }
+    Proj w[U];
+    TapRaw<PREC> t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i + u * stride < end) {
+        w[u] = project(p[u], x, g);
+        if (w[u].in) fetch_taps<PREC>(img, w[u].u, w[u].v, g.w, t[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i + u * stride < end) se3_finish<PREC>(p[u], pv[u], pi[u], w[u], t[u], x, g, kp, isGood, acc);
+  }
+}
+
+// Rest of one Sim3 point (photometric + depth residual) once its
+// projection, taps and the frame's depth at the rounded pixel are in hand.
 template <int PREC, class ACC>
-__device__ inline void sim3_point(const PointsLevel& pts, int i, const void* __restrict__ img,
-                                  const float* __restrict__ fid, const float* __restrict__ fvar, const LevelGeom& g,
-                                  const Xf& x, const TrackKParams& kp, ACC& acc) {
-  const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
-  float4 p = pts.posI[i];
-  float Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
-  float Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
-  float Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
-  float u = (Wx / Wz) * g.fx + g.cx;
-  float v = (Wy / Wz) * g.fy + g.cy;
-  if (!(u > 1.0f && v > 1.0f && u < wlim && v < hlim)) return;
-  Taps ri = interp3<PREC>(img, u, v, g.w);
+__device__ inline void sim3_finish(const float4& p, float pvar, float2 rg, const Proj& w, const TapRaw<PREC>& t,
+                                   float fidR, float varF, const Xf& x, const LevelGeom& g, const TrackKParams& kp,
+                                   ACC& acc) {
+  if (!w.in) return;
+  const float Wx = w.Wx, Wy = w.Wy, Wz = w.Wz;
+  Taps ri = blend_taps<PREC>(t);
   float gx, gy;
   if (kp.use_esm) {
-    float2 rg = pts.grad[i];
     float rgx = kp.roll[0] * rg.x + kp.roll[1] * rg.y;
     float rgy = kp.roll[2] * rg.x + kp.roll[3] * rg.y;
     gx = g.fx * 0.5f * (ri.gx + rgx);
@@ -314,13 +414,11 @@ __device__ inline void sim3_point(const PointsLevel& pts, int i, const void* __r
   acc.add(SM_SX, c1 * wgt);
   acc.add(SM_SY, c2 * wgt);
   acc.add(SM_SW, wgt);
-  int ir = (int)(u + 0.5f) + g.w * (int)(v + 0.5f);
-  float varF = fvar[ir];
   float refId = 1.0f / Wz;
   float d = 1.0f / p.z;
   float rd, vard;
   if (varF > 0.0f) {
-    rd = refId - fid[ir];
+    rd = refId - fidR;
     vard = varF;
   } else {
     rd = -1.0f;
@@ -330,7 +428,7 @@ __device__ inline void sim3_point(const PointsLevel& pts, int i, const void* __r
   acc.add(SM_USAGE, dc < 1.0f ? dc : 1.0f);
   acc.add(SM_WARPED, 1.0f);
   // calcSim3WeightsAndResidual
-  float s = kp.var_weight * pts.var[i];
+  float s = kp.var_weight * pvar;
   float sv = kp.var_weight * vard;
   float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
   float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
@@ -372,6 +470,47 @@ __device__ inline void sim3_point(const PointsLevel& pts, int i, const void* __r
       for (int c = a; c < 4; ++c) acc.add(SM_A + tri7(remap[a], remap[c]), v4[a] * v4[c] * wd);
       acc.add(SM_B + remap[a], -(v4[a] * rdW));
     }
+  }
+}
+
+// Sim3 evaluation of points i, i + stride, ... < end of one thread, U points
+// at a time (loads first, then taps and the frame depth at the rounded
+// projection, then the points in index order).
+template <int PREC, int U, class ACC>
+__device__ inline void sim3_points(const PointsLevel& pts, int i, int stride, int end, const void* __restrict__ img,
+                                   const float* __restrict__ fid, const float* __restrict__ fvar, const LevelGeom& g,
+                                   const Xf& x, const TrackKParams& kp, ACC& acc) {
+  for (; i < end; i += U * stride) {
+    float4 p[U];
+    float pv[U];
+    float2 rg[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = i + u * stride;
+      if (k < end) {
+        p[u] = pts.posI[k];
+        pv[u] = pts.var[k];
+        rg[u] = kp.use_esm ? pts.grad[k] : make_float2(0.f, 0.f);
+      }
+    }
+    Proj w[U];
+    TapRaw<PREC> t[U];
+    float fd[U], fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i + u * stride < end) {
+        w[u] = project(p[u], x, g);
+        if (w[u].in) {
+          fetch_taps<PREC>(img, w[u].u, w[u].v, g.w, t[u]);
+          int ir = (int)(w[u].u + 0.5f) + g.w * (int)(w[u].v + 0.5f);
+          fv[u] = fvar[ir];
+          fd[u] = fid[ir];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i + u * stride < end) sim3_finish<PREC>(p[u], pv[u], rg[u], w[u], t[u], fd[u], fv[u], x, g, kp, acc);
   }
 }
 

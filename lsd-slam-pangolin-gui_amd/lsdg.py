@@ -120,7 +120,7 @@ def hyp_struct(arrs):
     return Hypotheses(*vals)
 
 
-KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE = range(5)
+KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE, KERNEL_BATCH_JOB = range(6)
 
 
 class KernelStats(C.Structure):
@@ -181,6 +181,7 @@ def load():
         "lsdg_frame_create_u8": (i32, [vp, i32, C.POINTER(Camera), fp, i32, i32, C.POINTER(vp)]),
         "lsdg_frame_destroy": (i32, [vp]),
         "lsdg_frame_build_pyramid": (i32, [vp]),
+        "lsdg_frame_invalidate": (i32, [vp]),
         "lsdg_frame_set_depth": (i32, [vp, fp, fp]),
         "lsdg_frame_read": (i32, [vp, i32, i32, fp, C.c_size_t]),
         "lsdg_frame_set_tracking": (i32, [vp, vp, dp, f32, fp]),
@@ -365,6 +366,10 @@ class Frame:
 
     def build_pyramid(self):
         self.ctx.check(self.ctx.L.lsdg_frame_build_pyramid(self.h), "lsdg_frame_build_pyramid")
+
+    def invalidate(self):
+        """Drop the derived pyramid levels; the next reader rebuilds what it needs."""
+        self.ctx.check(self.ctx.L.lsdg_frame_invalidate(self.h), "lsdg_frame_invalidate")
 
     def set_depth(self, idepth, var):
         a, pa = _f32(idepth)

@@ -1,5 +1,9 @@
-"""Batched tracking (one persistent launch per LM step, hipGraph-replayed)
-against the plain path, per instance.
+"""Batched tracking (hipGraph-replayed) against the plain path, per instance.
+
+Every test runs in the three schedules a plan can take: every level in the
+whole-job kernel (one block per job), every level in the multi-block steps
+(tile evaluation + per-job solve launches), and the hybrid (levels up to 3000
+points in the whole-job kernel, larger ones in the steps).
 
 Bars: FP32 pyramids with FP64 sums take the plain path's LM path (poses within
 1e-9 twist, identical call counts); FP16 pyramids with compensated FP32 block
@@ -13,6 +17,18 @@ import synth
 from helpers import tracking_params, twist
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, params=["job", "steps", "hybrid"])
+def schedule(request, monkeypatch):
+    """Levels up to LSDG_BATCH_JOB_POINTS points run in the whole-job kernel
+    (read when a plan is created)."""
+    points = {"job": None, "steps": "0", "hybrid": "3000"}[request.param]
+    if points is None:
+        monkeypatch.delenv("LSDG_BATCH_JOB_POINTS", raising=False)
+    else:
+        monkeypatch.setenv("LSDG_BATCH_JOB_POINTS", points)
+    return request.param
 
 
 def _instances(n, seed, resolutions):
