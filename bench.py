@@ -277,7 +277,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched SE3 alignment: {B} x 640x480 frames per step (pyramid rebuild + LM "
                                "levels 4..1, <=20 iterations/level), ~25% semi-dense points, 2 cm / 1 deg, "
-                               "FP16 (I, gx, gy) pyramids, FP32 residuals, compensated FP32 block sums, FP64 "
+                               "FP16 (I, gx, gy) pyramids, FP32 residuals, FP64 sums, FP64 "
                                "merge and LDLT", "batch": B, "resolution": "640x480", "storage": "fp16"},
         "batched_fp32_frames_per_s": round(batched32_fps, 2),
         **out_plain,

@@ -213,8 +213,10 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   b->n = n;
   b->kind = kind;
   b->prec = prec;
-  // FP16 / BF16 pyramids run with compensated FP32 block sums
-  b->acc = prec == LSDG_FP32 ? BATCH_ACC_F64 : BATCH_ACC_KAHAN;
+  // Per-thread FP64 sums of the FP32 point terms for every pyramid precision:
+  // fewer VALU instructions per point than compensated FP32 pairs (the
+  // evaluation is issue-bound), and the plain path's summation precision.
+  b->acc = BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_ACC")) b->acc = std::atoi(e) ? BATCH_ACC_KAHAN : BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_PPT")) b->ppt = std::max(1, std::min(64, std::atoi(e)));
   if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));

@@ -109,6 +109,27 @@ __device__ inline Taps blend_taps(const TapRaw<PREC>& t) {
   return o;
 }
 
+// Division, reciprocal and square root of the per-point math.  FP32 pyramids
+// use IEEE correctly rounded operations (the CPU restatement's results, bit
+// for bit); FP16 / BF16 pyramids use the hardware reciprocal and square-root
+// approximations (about 1 ulp, far below the taps' 11- / 8-bit quantisation),
+// which cut the evaluation's VALU work by about a third.
+template <int PREC>
+__device__ inline float fdiv(float a, float b) {
+  if constexpr (PREC == LSDG_FP32) return a / b;
+  else return a * __builtin_amdgcn_rcpf(b);
+}
+template <int PREC>
+__device__ inline float frcp(float b) {
+  if constexpr (PREC == LSDG_FP32) return 1.0f / b;
+  else return __builtin_amdgcn_rcpf(b);
+}
+template <int PREC>
+__device__ inline float fsqrt(float x) {
+  if constexpr (PREC == LSDG_FP32) return sqrtf(x);
+  else return __builtin_amdgcn_sqrtf(x);
+}
+
 // Projection of one reference point into the frame (the warp every tracker
 // evaluation starts with).
 struct Proj {
@@ -116,22 +137,24 @@ struct Proj {
   bool in;
 };
 
+template <int PREC>
 __device__ inline Proj project(const float4& p, const Xf& x, const LevelGeom& g) {
   const float wlim = (float)(g.w - 2), hlim = (float)(g.h - 2);
   Proj o;
   o.Wx = x.R[0] * p.x + x.R[1] * p.y + x.R[2] * p.z + x.t[0];
   o.Wy = x.R[3] * p.x + x.R[4] * p.y + x.R[5] * p.z + x.t[1];
   o.Wz = x.R[6] * p.x + x.R[7] * p.y + x.R[8] * p.z + x.t[2];
-  o.u = (o.Wx / o.Wz) * g.fx + g.cx;
-  o.v = (o.Wy / o.Wz) * g.fy + g.cy;
+  o.u = fdiv<PREC>(o.Wx, o.Wz) * g.fx + g.cx;
+  o.v = fdiv<PREC>(o.Wy, o.Wz) * g.fy + g.cy;
   o.in = o.u > 1.0f && o.v > 1.0f && o.u < wlim && o.v < hlim;
   return o;
 }
 
 // -v = dr/dxi of the photometric residual (SE3Tracker::calculateWarpUpdate).
+template <int PREC>
 __device__ inline void jac6(float px, float py, float pz, float gx, float gy, float* v) {
-  float z = 1.0f / pz;
-  float z_sqr = 1.0f / (pz * pz);
+  float z = frcp<PREC>(pz);
+  float z_sqr = frcp<PREC>(pz * pz);
   v[0] = z * gx;
   v[1] = z * gy;
   v[2] = (-px * z_sqr) * gx + (-py * z_sqr) * gy;
@@ -307,16 +330,16 @@ __device__ inline void se3_finish(const float4& p, float pvar, int pidx, const P
   float c1 = kp.aff_a * p.w + kp.aff_b;
   float c2 = ri.I;
   float r = c1 - c2;
-  float wgt = fabsf(r) < 5.0f ? 1.0f : 5.0f / fabsf(r);
+  float wgt = fabsf(r) < 5.0f ? 1.0f : fdiv<PREC>(5.0f, fabsf(r));
   acc.add(S3_SXX, c1 * c1 * wgt);
   acc.add(S3_SYY, c2 * c2 * wgt);
   acc.add(S3_SX, c1 * wgt);
   acc.add(S3_SY, c2 * wgt);
   acc.add(S3_SW, wgt);
-  bool good = r * r / (kp.max_diff_constant + kp.max_diff_grad_mult * (ri.gx * ri.gx + ri.gy * ri.gy)) < 1.0f;
+  bool good = fdiv<PREC>(r * r, kp.max_diff_constant + kp.max_diff_grad_mult * (ri.gx * ri.gx + ri.gy * ri.gy)) < 1.0f;
   if (isGood) isGood[pidx] = good ? 1 : 0;
   float gx = g.fx * ri.gx, gy = g.fy * ri.gy;
-  float d = 1.0f / p.z;
+  float d = frcp<PREC>(p.z);
   if (good) {
     acc.add(S3_RESUNW, r * r);
     acc.add(S3_SIGNED, r);
@@ -324,21 +347,21 @@ __device__ inline void se3_finish(const float4& p, float pvar, int pidx, const P
   } else {
     acc.add(S3_BAD, 1.0f);
   }
-  float dc = p.z / Wz;
+  float dc = fdiv<PREC>(p.z, Wz);
   acc.add(S3_USAGE, dc < 1.0f ? dc : 1.0f);
   acc.add(S3_WARPED, 1.0f);
   // calcWeightsAndResidual
   float s = kp.var_weight * pvar;
-  float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
-  float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
+  float g0 = fdiv<PREC>(x.t[0] * Wz - x.t[2] * Wx, Wz * Wz * d);
+  float g1 = fdiv<PREC>(x.t[1] * Wz - x.t[2] * Wy, Wz * Wz * d);
   float drpdd = gx * g0 + gy * g1;
-  float w_p = 1.0f / (kp.noise2 + s * drpdd * drpdd);
-  float wr = fabsf(r * sqrtf(w_p));
-  float wh = fabsf(wr < kp.huber ? 1.0f : kp.huber / wr);
+  float w_p = frcp<PREC>(kp.noise2 + s * drpdd * drpdd);
+  float wr = fabsf(r * fsqrt<PREC>(w_p));
+  float wh = fabsf(wr < kp.huber ? 1.0f : fdiv<PREC>(kp.huber, wr));
   float W = wh * w_p;
   // calculateWarpUpdate
   float jv[6];
-  jac6(Wx, Wy, Wz, gx, gy, jv);
+  jac6<PREC>(Wx, Wy, Wz, gx, gy, jv);
   int k = 0;
 #pragma unroll
   for (int a = 0; a < 6; ++a)
@@ -376,7 +399,7 @@ __device__ inline void se3_points(const PointsLevel& pts, int i, int stride, int
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (i + u * stride < end) {
-        w[u] = project(p[u], x, g);
+        w[u] = project<PREC>(p[u], x, g);
         if (w[u].in) fetch_taps<PREC>(img, w[u].u, w[u].v, g.w, t[u]);
       }
     }
@@ -408,14 +431,14 @@ __device__ inline void sim3_finish(const float4& p, float pvar, float2 rg, const
   float c1 = kp.aff_a * p.w + kp.aff_b;
   float c2 = ri.I;
   float rp = c1 - c2;
-  float wgt = fabsf(rp) < 2.0f ? 1.0f : 2.0f / fabsf(rp);
+  float wgt = fabsf(rp) < 2.0f ? 1.0f : fdiv<PREC>(2.0f, fabsf(rp));
   acc.add(SM_SXX, c1 * c1 * wgt);
   acc.add(SM_SYY, c2 * c2 * wgt);
   acc.add(SM_SX, c1 * wgt);
   acc.add(SM_SY, c2 * wgt);
   acc.add(SM_SW, wgt);
-  float refId = 1.0f / Wz;
-  float d = 1.0f / p.z;
+  float refId = frcp<PREC>(Wz);
+  float d = frcp<PREC>(p.z);
   float rd, vard;
   if (varF > 0.0f) {
     rd = refId - fidR;
@@ -424,22 +447,22 @@ __device__ inline void sim3_finish(const float4& p, float pvar, float2 rg, const
     rd = -1.0f;
     vard = -1.0f;
   }
-  float dc = p.z / Wz;
+  float dc = fdiv<PREC>(p.z, Wz);
   acc.add(SM_USAGE, dc < 1.0f ? dc : 1.0f);
   acc.add(SM_WARPED, 1.0f);
   // calcSim3WeightsAndResidual
   float s = kp.var_weight * pvar;
   float sv = kp.var_weight * vard;
-  float g0 = (x.t[0] * Wz - x.t[2] * Wx) / (Wz * Wz * d);
-  float g1 = (x.t[1] * Wz - x.t[2] * Wy) / (Wz * Wz * d);
-  float g2 = (Wz - x.t[2]) / (Wz * Wz * d);
+  float g0 = fdiv<PREC>(x.t[0] * Wz - x.t[2] * Wx, Wz * Wz * d);
+  float g1 = fdiv<PREC>(x.t[1] * Wz - x.t[2] * Wy, Wz * Wz * d);
+  float g2 = fdiv<PREC>(Wz - x.t[2], Wz * Wz * d);
   float drpdd = gx * g0 + gy * g1;
-  float w_p = 1.0f / (kp.noise2 + s * drpdd * drpdd);
-  float w_d = 1.0f / (sv + g2 * g2 * s);
-  float wrd = fabsf(rd * sqrtf(w_d));
-  float wrp = fabsf(rp * sqrtf(w_p));
+  float w_p = frcp<PREC>(kp.noise2 + s * drpdd * drpdd);
+  float w_d = frcp<PREC>(sv + g2 * g2 * s);
+  float wrd = fabsf(rd * fsqrt<PREC>(w_d));
+  float wrp = fabsf(rp * fsqrt<PREC>(w_p));
   float wabs = sv > 0.0f ? wrd + wrp : wrp;
-  float wh = fabsf(wabs < kp.huber ? 1.0f : kp.huber / wabs);
+  float wh = fabsf(wabs < kp.huber ? 1.0f : fdiv<PREC>(kp.huber, wabs));
   if (sv > 0.0f) {
     acc.add(SM_RESD, wh * w_d * rd * rd);
     acc.add(SM_NTD, 1.0f);
@@ -450,9 +473,9 @@ __device__ inline void sim3_finish(const float4& p, float pvar, float2 rg, const
   // calcSim3LGS: LGS6 over (v, w) and LGS4 over (v3, w1, w2, sigma), merged
   // into the 7x7 upper triangle at rows / columns {2, 3, 4, 6}.
   float jv[6];
-  jac6(Wx, Wy, Wz, gx, gy, jv);
-  float z = 1.0f / Wz;
-  float z_sqr = 1.0f / (Wz * Wz);
+  jac6<PREC>(Wx, Wy, Wz, gx, gy, jv);
+  float z = frcp<PREC>(Wz);
+  float z_sqr = frcp<PREC>(Wz * Wz);
   float v4[4] = {z_sqr, z_sqr * Wy, -z_sqr * Wx, z};
 #pragma unroll
   for (int a = 0; a < 6; ++a)
@@ -499,7 +522,7 @@ __device__ inline void sim3_points(const PointsLevel& pts, int i, int stride, in
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (i + u * stride < end) {
-        w[u] = project(p[u], x, g);
+        w[u] = project<PREC>(p[u], x, g);
         if (w[u].in) {
           fetch_taps<PREC>(img, w[u].u, w[u].v, g.w, t[u]);
           int ir = (int)(w[u].u + 0.5f) + g.w * (int)(w[u].v + 0.5f);
@@ -546,7 +569,7 @@ __device__ inline void adjoint_point(const PointsLevel& pts, int i, const void* 
   acc.add(7, 1.0f);
   float Wr = wh * w_p * r;
   float jv[6];
-  jac6(Wx, Wy, Wz, gx, gy, jv);
+  jac6<PREC>(Wx, Wy, Wz, gx, gy, jv);
 #pragma unroll
   for (int a = 0; a < 6; ++a) acc.add(a, -(Wr * jv[a]));
   grad_d[i] = -Wr * drpdd;

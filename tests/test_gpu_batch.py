@@ -6,8 +6,8 @@ whole-job kernel (one block per job), every level in the multi-block steps
 points in the whole-job kernel, larger ones in the steps).
 
 Bars: FP32 pyramids with FP64 sums take the plain path's LM path (poses within
-1e-9 twist, identical call counts); FP16 pyramids with compensated FP32 block
-sums stay within 1e-5 of the plain FP16 path; replays are bit-identical."""
+1e-9 twist, identical call counts); FP16 pyramids stay within 1e-5 of the
+plain FP16 path; replays are bit-identical."""
 import numpy as np
 import pytest
 
@@ -19,11 +19,11 @@ from helpers import tracking_params, twist
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["job", "steps", "hybrid"])
+@pytest.fixture(autouse=True, params=["job", "steps", "hybrid", "default"])
 def schedule(request, monkeypatch):
     """Levels up to LSDG_BATCH_JOB_POINTS points run in the whole-job kernel
     (read when a plan is created)."""
-    points = {"job": None, "steps": "0", "hybrid": "3000"}[request.param]
+    points = {"job": "1000000000", "steps": "0", "hybrid": "3000", "default": None}[request.param]
     if points is None:
         monkeypatch.delenv("LSDG_BATCH_JOB_POINTS", raising=False)
     else:
