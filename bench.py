@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the batched FP16 leg (for kernel profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="also run a plain device copy of MB MiB (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -207,6 +209,14 @@ def main():
 
     params = tracking_params(20)
     ctx = lsdg.Context(0, params)
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     insts = [synth.tracking_instance(1001 + 17 * i) for i in range(args.batch)]
     B = args.batch
 

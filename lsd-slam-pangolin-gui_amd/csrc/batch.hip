@@ -508,30 +508,23 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
 }
 
 // ------------------------------------------------------------ launchers
-hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s) {
-  const dim3 grid(a.njobs), block(BATCH_JOB_BS);
-  if (kind == LSDG_JOB_SIM3) {
-    if (acc == BATCH_ACC_KAHAN) {
-      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
-      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
-      else k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_KAHAN><<<grid, block, 0, s>>>(a);
-    } else {
-      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_F64><<<grid, block, 0, s>>>(a);
-      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_F64><<<grid, block, 0, s>>>(a);
-      else k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_F64><<<grid, block, 0, s>>>(a);
-    }
-  } else {
-    if (acc == BATCH_ACC_KAHAN) {
-      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
-      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_KAHAN><<<grid, block, 0, s>>>(a);
-      else k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_KAHAN><<<grid, block, 0, s>>>(a);
-    } else {
-      if (prec == LSDG_FP16) k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_F64><<<grid, block, 0, s>>>(a);
-      else if (prec == LSDG_BF16) k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_F64><<<grid, block, 0, s>>>(a);
-      else k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_F64><<<grid, block, 0, s>>>(a);
-    }
-  }
-  return hipGetLastError();
+hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s, const KTimer* t) {
+  auto k = kind == LSDG_JOB_SIM3
+               ? (acc == BATCH_ACC_KAHAN
+                      ? (prec == LSDG_FP16   ? k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_KAHAN>
+                         : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_KAHAN>
+                                             : k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_KAHAN>)
+                      : (prec == LSDG_FP16   ? k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_F64>
+                         : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_F64>
+                                             : k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_F64>))
+               : (acc == BATCH_ACC_KAHAN
+                      ? (prec == LSDG_FP16   ? k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_KAHAN>
+                         : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_KAHAN>
+                                             : k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_KAHAN>)
+                      : (prec == LSDG_FP16   ? k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_F64>
+                         : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_F64>
+                                             : k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_F64>));
+  return launch_k(k, dim3(a.njobs), dim3(BATCH_JOB_BS), s, t, a);
 }
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
@@ -539,31 +532,29 @@ hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int KIND, int PREC>
-static void eval_prec(const BatchArgs& a, int parity, int grid, int acc, hipStream_t s) {
-  if (acc == BATCH_ACC_KAHAN)
-    k_batch_eval<KIND, PREC, ACC_KAHAN><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
-  else
-    k_batch_eval<KIND, PREC, ACC_F64><<<dim3(grid), dim3(BATCH_BS), 0, s>>>(a, parity);
+hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s,
+                             const KTimer* t) {
+  auto k = kind == LSDG_JOB_SIM3
+               ? (acc == BATCH_ACC_KAHAN
+                      ? (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SIM3, LSDG_FP16, ACC_KAHAN>
+                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SIM3, LSDG_BF16, ACC_KAHAN>
+                                             : k_batch_eval<LSDG_JOB_SIM3, LSDG_FP32, ACC_KAHAN>)
+                      : (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SIM3, LSDG_FP16, ACC_F64>
+                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SIM3, LSDG_BF16, ACC_F64>
+                                             : k_batch_eval<LSDG_JOB_SIM3, LSDG_FP32, ACC_F64>))
+               : (acc == BATCH_ACC_KAHAN
+                      ? (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SE3, LSDG_FP16, ACC_KAHAN>
+                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SE3, LSDG_BF16, ACC_KAHAN>
+                                             : k_batch_eval<LSDG_JOB_SE3, LSDG_FP32, ACC_KAHAN>)
+                      : (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SE3, LSDG_FP16, ACC_F64>
+                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SE3, LSDG_BF16, ACC_F64>
+                                             : k_batch_eval<LSDG_JOB_SE3, LSDG_FP32, ACC_F64>));
+  return launch_k(k, dim3(grid), dim3(BATCH_BS), s, t, a, parity);
 }
 
-hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s) {
-  if (kind == LSDG_JOB_SIM3) {
-    if (prec == LSDG_FP16) eval_prec<LSDG_JOB_SIM3, LSDG_FP16>(a, parity, grid, acc, s);
-    else if (prec == LSDG_BF16) eval_prec<LSDG_JOB_SIM3, LSDG_BF16>(a, parity, grid, acc, s);
-    else eval_prec<LSDG_JOB_SIM3, LSDG_FP32>(a, parity, grid, acc, s);
-  } else {
-    if (prec == LSDG_FP16) eval_prec<LSDG_JOB_SE3, LSDG_FP16>(a, parity, grid, acc, s);
-    else if (prec == LSDG_BF16) eval_prec<LSDG_JOB_SE3, LSDG_BF16>(a, parity, grid, acc, s);
-    else eval_prec<LSDG_JOB_SE3, LSDG_FP32>(a, parity, grid, acc, s);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s) {
-  if (kind == LSDG_JOB_SIM3) k_batch_solve<LSDG_JOB_SIM3><<<dim3(a.njobs), dim3(BATCH_SOLVE_BS), 0, s>>>(a, parity);
-  else k_batch_solve<LSDG_JOB_SE3><<<dim3(a.njobs), dim3(BATCH_SOLVE_BS), 0, s>>>(a, parity);
-  return hipGetLastError();
+hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s, const KTimer* t) {
+  auto k = kind == LSDG_JOB_SIM3 ? k_batch_solve<LSDG_JOB_SIM3> : k_batch_solve<LSDG_JOB_SE3>;
+  return launch_k(k, dim3(a.njobs), dim3(BATCH_SOLVE_BS), s, t, a, parity);
 }
 
 hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s) {

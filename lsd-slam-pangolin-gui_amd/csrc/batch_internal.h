@@ -68,9 +68,12 @@ struct BatchArgs {
 };
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s);
+// Launches with an optional kernel timer (see launch_k).
+hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s,
+                            const KTimer* t = nullptr);
+hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s,
+                             const KTimer* t = nullptr);
+hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s, const KTimer* t = nullptr);
 hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
-hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
-hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s);
-hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s);
 
 }  // namespace lsdg

@@ -592,9 +592,9 @@ static lsdg_status se3_eval_sync(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int lvl, 
   TrackKParams kp = track_kparams(c->P, false);
   kp.aff_a = aa;
   kp.aff_b = ab;
-  ProfMark pm = prof_begin(c, LSDG_KERNEL_EVAL);
-  CK(c, launch_se3_eval(c, pl, f, lvl, pose_xf(T, false), kp, d_mask, c->d_result));
-  prof_end(c, pm, eval_bytes(pl.n, f, lvl, false));
+  KTimer kt = prof_timer(c);
+  CK(c, launch_se3_eval(c, pl, f, lvl, pose_xf(T, false), kp, d_mask, c->d_result, &kt));
+  prof_timer_done(c, LSDG_KERNEL_EVAL, kt, eval_bytes(pl.n, f, lvl, false));
   CK(c, hipStreamSynchronize(c->stream));
   std::memcpy(S, c->h_result, sizeof(double) * NS_SE3);
   return LSDG_OK;
@@ -805,9 +805,9 @@ static lsdg_status sim3_eval_sync(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int lvl,
   kp.aff_a = aa;
   kp.aff_b = ab;
   esm_roll(T, kp.roll);
-  ProfMark pm = prof_begin(c, LSDG_KERNEL_EVAL);
-  CK(c, launch_sim3_eval(c, pl, f, lvl, pose_xf(T, true), kp, c->d_result));
-  prof_end(c, pm, eval_bytes(pl.n, f, lvl, true));
+  KTimer kt = prof_timer(c);
+  CK(c, launch_sim3_eval(c, pl, f, lvl, pose_xf(T, true), kp, c->d_result, &kt));
+  prof_timer_done(c, LSDG_KERNEL_EVAL, kt, eval_bytes(pl.n, f, lvl, true));
   CK(c, hipStreamSynchronize(c->stream));
   std::memcpy(S, c->h_result, sizeof(double) * NS_SIM3);
   return LSDG_OK;

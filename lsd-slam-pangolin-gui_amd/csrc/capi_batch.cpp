@@ -378,19 +378,19 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
   }
   CK(c, launch_batch_init(a, c->stream));
   if (b->jobMaxPoints > 0) {
-    ProfMark pj = prof_begin(c, LSDG_KERNEL_BATCH_JOB);
-    CK(c, launch_batch_job(a, b->kind, b->prec, b->acc, c->stream));
-    prof_end(c, pj, 0.0);
+    KTimer kt = prof_timer(c);
+    CK(c, launch_batch_job(a, b->kind, b->prec, b->acc, c->stream, &kt));
+    prof_timer_done(c, LSDG_KERNEL_BATCH_JOB, kt, 0.0);
   }
   for (int round = 0; round <= 64; ++round) {
     const int steps = round == 0 ? b->steps : b->contSteps;
     for (int s = 0; s < steps; ++s) {
-      ProfMark pm = prof_begin(c, LSDG_KERNEL_BATCH_EVAL);
-      CK(c, launch_batch_eval(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream));
-      prof_end(c, pm, 0.0);
-      ProfMark ps = prof_begin(c, LSDG_KERNEL_BATCH_SOLVE);
-      CK(c, launch_batch_solve(a, b->kind, s & 1, c->stream));
-      prof_end(c, ps, 0.0);
+      KTimer ke = prof_timer(c);
+      CK(c, launch_batch_eval(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream, &ke));
+      prof_timer_done(c, LSDG_KERNEL_BATCH_EVAL, ke, 0.0);
+      KTimer ks = prof_timer(c);
+      CK(c, launch_batch_solve(a, b->kind, s & 1, c->stream, &ks));
+      prof_timer_done(c, LSDG_KERNEL_BATCH_SOLVE, ks, 0.0);
     }
     CK(c, hipMemcpyAsync(b->h_remaining, b->d_remaining, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     CK(c, hipStreamSynchronize(c->stream));

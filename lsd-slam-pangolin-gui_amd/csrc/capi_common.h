@@ -33,6 +33,10 @@ ProfMark prof_begin(lsdg_ctx c, int kind);
 void prof_end(lsdg_ctx c, ProfMark& m, double bytes);
 // Synchronises the stream and folds finished event pairs into the totals.
 lsdg_status prof_resolve(lsdg_ctx c);
+// A kernel's own start / stop events (for launch_k) when profiling is on,
+// null events otherwise; prof_timer_done files the pair under `kind`.
+KTimer prof_timer(lsdg_ctx c);
+void prof_timer_done(lsdg_ctx c, int kind, const KTimer& t, double bytes);
 
 }  // namespace lsdg
 

@@ -3,6 +3,7 @@
 // buffers) and the kernel launchers the C-ABI layer calls.
 #pragma once
 #include <hip/hip_fp16.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -101,6 +102,22 @@ struct PyrBatch {
   const int* grad_off;
   int img_blocks, grad_blocks, nf;
 };
+
+// Start / stop events of one timed kernel launch (both null: untimed).
+struct KTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Launches kernel f on stream s.  A timed launch goes through
+// hipExtLaunchKernel, whose events carry the kernel's own start and end
+// timestamps (the figures rocprofv3 reports), not the stream's around it.
+template <typename... KArgs, typename... Args>
+inline hipError_t launch_k(void (*f)(KArgs...), dim3 grid, dim3 block, hipStream_t s, const KTimer* t,
+                           Args... args) {
+  if (t && t->a) hipExtLaunchKernelGGL(f, grid, block, 0, s, t->a, t->b, 0, static_cast<KArgs>(args)...);
+  else f<<<grid, block, 0, s>>>(static_cast<KArgs>(args)...);
+  return hipGetLastError();
+}
 
 struct PointsLevel {
   const float4* posI;  // x, y, z, I
@@ -245,9 +262,9 @@ hipError_t launch_pyr_batch(const PyrBatch& b, hipStream_t s);
 
 // track.hip
 hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
-                           const TrackKParams& kp, uint8_t* d_isgood, double* d_out);
+                           const TrackKParams& kp, uint8_t* d_isgood, double* d_out, const KTimer* t = nullptr);
 hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
-                            const TrackKParams& kp, double* d_out);
+                            const TrackKParams& kp, double* d_out, const KTimer* t = nullptr);
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                               const TrackKParams& kp, float* d_grad_d, double* d_out);
 

@@ -1,5 +1,7 @@
-// lsdg_ctx_profile / lsdg_ctx_kernel_stats: hipEvent pairs recorded on the
-// context's stream around each launch, folded into per-kind totals.
+// lsdg_ctx_profile / lsdg_ctx_kernel_stats: hipEvent pairs per timed launch
+// (single kernels: the kernel's own start / stop events through
+// hipExtLaunchKernel; multi-kernel sequences: events recorded on the stream
+// around them), folded into per-kind totals.
 #include <cstring>
 
 #include "capi_common.h"
@@ -31,6 +33,24 @@ void lsdg::prof_end(lsdg_ctx c, ProfMark& m, double bytes) {
   hipEvent_t b = take_event(c);
   if (!b || hipEventRecord(b, c->stream) != hipSuccess) return;
   c->prof_pending.push_back({m.kind, m.a, b, bytes});
+  if (c->prof_pending.size() > 4096) (void)prof_resolve(c);
+}
+
+KTimer lsdg::prof_timer(lsdg_ctx c) {
+  KTimer t;
+  if (!c->prof_on) return t;
+  t.a = take_event(c);
+  t.b = t.a ? take_event(c) : nullptr;
+  if (!t.b) {
+    if (t.a) c->prof_events.push_back(t.a);
+    t.a = nullptr;
+  }
+  return t;
+}
+
+void lsdg::prof_timer_done(lsdg_ctx c, int kind, const KTimer& t, double bytes) {
+  if (!t.a || !t.b) return;
+  c->prof_pending.push_back({kind, t.a, t.b, bytes});
   if (c->prof_pending.size() > 4096) (void)prof_resolve(c);
 }
 

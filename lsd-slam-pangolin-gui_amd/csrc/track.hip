@@ -117,45 +117,26 @@ static inline const void* level_image(lsdg_frame f, int level) {
 }
 
 hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
-                           const TrackKParams& kp, uint8_t* d_isgood, double* d_out) {
+                           const TrackKParams& kp, uint8_t* d_isgood, double* d_out, const KTimer* t) {
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
-  int nb = eval_blocks(c, pts.n);
+  const dim3 grid(eval_blocks(c, pts.n)), block(EVAL_BS);
   const void* img = level_image(f, level);
-  switch (f->precision) {
-    case LSDG_FP16:
-      k_se3_eval<LSDG_FP16, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, g, x, kp, d_isgood,
-                         c->d_partials, c->d_counter, d_out);
-      break;
-    case LSDG_BF16:
-      k_se3_eval<LSDG_BF16, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, g, x, kp, d_isgood,
-                         c->d_partials, c->d_counter, d_out);
-      break;
-    default:
-      k_se3_eval<LSDG_FP32, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, g, x, kp, d_isgood,
-                         c->d_partials, c->d_counter, d_out);
-  }
-  return hipGetLastError();
+  auto k = f->precision == LSDG_FP16 ? k_se3_eval<LSDG_FP16, ACC_F64>
+           : f->precision == LSDG_BF16 ? k_se3_eval<LSDG_BF16, ACC_F64>
+                                       : k_se3_eval<LSDG_FP32, ACC_F64>;
+  return launch_k(k, grid, block, c->stream, t, pts, img, g, x, kp, d_isgood, c->d_partials, c->d_counter, d_out);
 }
 
 hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
-                            const TrackKParams& kp, double* d_out) {
+                            const TrackKParams& kp, double* d_out, const KTimer* t) {
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
-  int nb = eval_blocks(c, pts.n);
+  const dim3 grid(eval_blocks(c, pts.n)), block(EVAL_BS);
   const void* img = level_image(f, level);
-  switch (f->precision) {
-    case LSDG_FP16:
-      k_sim3_eval<LSDG_FP16, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, f->d_idepth[level],
-                         f->d_ivar[level], g, x, kp, c->d_partials, c->d_counter, d_out);
-      break;
-    case LSDG_BF16:
-      k_sim3_eval<LSDG_BF16, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, f->d_idepth[level],
-                         f->d_ivar[level], g, x, kp, c->d_partials, c->d_counter, d_out);
-      break;
-    default:
-      k_sim3_eval<LSDG_FP32, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, f->d_idepth[level],
-                         f->d_ivar[level], g, x, kp, c->d_partials, c->d_counter, d_out);
-  }
-  return hipGetLastError();
+  auto k = f->precision == LSDG_FP16 ? k_sim3_eval<LSDG_FP16, ACC_F64>
+           : f->precision == LSDG_BF16 ? k_sim3_eval<LSDG_BF16, ACC_F64>
+                                       : k_sim3_eval<LSDG_FP32, ACC_F64>;
+  return launch_k(k, grid, block, c->stream, t, pts, img, (const float*)f->d_idepth[level],
+                  (const float*)f->d_ivar[level], g, x, kp, c->d_partials, c->d_counter, d_out);
 }
 
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
