@@ -31,6 +31,10 @@ import posemath  # noqa: E402
 import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_batch_eval launch on the headline workload, from the
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy kernel
+# (profiles/k_batch_eval_se3_640x480_x64_fp16.md); re-measured when the kernel changes.
+EVAL_TRAFFIC_BYTES = 20956108
 
 
 def tracking_params(its=20, crop=None):
@@ -295,7 +299,7 @@ def main():
         "depth_updates_per_s": round(depth["updates_per_s"], 2),
         "depth_update_ms": round(depth["ms_per_update"], 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": EVAL_TRAFFIC_BYTES,
                      "kernel": "k_batch_eval (fused residual + weights + LGS over all jobs' tiles, FP16 taps)",
                      "avg_launch_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 3),
                      "launches": ks["launches"],
