@@ -168,6 +168,26 @@ def _max3x3(g):
     return m2
 
 
+def sim3_instance(seed, cam=None, max_trans=0.10, max_rot_deg=5.0, scale_range=(0.9, 1.1), fraction=0.25,
+                  noise=2.0):
+    """Keyframe-to-keyframe Sim3 alignment (the constraint search's job): two
+    keyframes of one scene, each with semi-dense inverse depth on its
+    `fraction` highest-gradient pixels; the second keyframe's depth map is in
+    its own scale, depth / s.  gt is frameToRef (qx, qy, qz, qw, tx, ty, tz, s):
+    X_ref = s R X_frame + t."""
+    cam = cam or camera_1080()
+    sc = Scene(seed)
+    a_img, a_dep = sc.render(IDENTITY7, cam)
+    pose = sample_pose(seed * 7919 + 31, max_trans, max_rot_deg, uniform=True)
+    s = float(uniform_field(seed * 7919 + 32, 1, scale_range[0], scale_range[1])[0])
+    b_img, b_dep = sc.render(pose, cam, noise, seed * 7919 + 33)
+    a_id, a_var, a_frac, _ = semi_dense_depth(a_img, a_dep, seed * 7919 + 34, fraction=fraction)
+    b_id, b_var, b_frac, _ = semi_dense_depth(b_img, (b_dep / s).astype(np.float32), seed * 7919 + 35,
+                                              fraction=fraction)
+    return dict(cam=cam, a_img=a_img, a_idepth=a_id, a_var=a_var, b_img=b_img, b_idepth=b_id, b_var=b_var,
+                gt=np.r_[pose, s], fraction=(a_frac, b_frac), seed=seed)
+
+
 RESOLUTIONS = [(320, 240), (640, 480), (1280, 960)]
 
 

@@ -128,3 +128,30 @@ def test_batch_sim3_matches_plain(ctx):
         assert rig <= 1e-9 and sc <= 1e-9, (i, rig, sc)
         H, Hp = np.array(r[i].stats.hessian[:]), np.array(st.hessian[:])
         assert np.abs(H - Hp).max() <= 1e-9 * np.abs(Hp).max()
+
+
+@pytest.mark.parametrize("precision", [lsdg.FP32, lsdg.FP16])
+def test_batch_sim3_1920x1080(ctx, schedule, precision):
+    """Batched Sim3 at 1920x1080: 4 keyframe pairs, two jobs each (shared
+    frames), against the plain Sim3 path of the same precision."""
+    insts = [synth.sim3_instance(1005 + k) for k in range(4)]
+    items = []
+    for i, d in enumerate(insts):
+        a = ctx.frame(2 * i, d["cam"], d["a_img"], precision)
+        a.set_depth(d["a_idepth"], d["a_var"])
+        b = ctx.frame(2 * i + 1, d["cam"], d["b_img"], precision)
+        b.set_depth(d["b_idepth"], d["b_var"])
+        items.append((a, b, ctx.ref(a)))
+    ident = list(synth.IDENTITY7) + [1.0]
+    plain = [ctx.sim3_track(ref, b, ident, 4, 1) for a, b, ref in items]
+    jobs = [lsdg.Job(ref.h, b.h, lsdg.pose8(ident), lsdg.JOB_SIM3, 4, 1, 0) for a, b, ref in items for _ in range(2)]
+    bt = ctx.batch(jobs)
+    res = bt.run()
+    bt.destroy()
+    for j, r in enumerate(res):
+        pose, st, status = plain[j // 2]
+        assert r.status == status
+        rig, sc = posemath.sim3_error(list(r.frame_to_ref), pose)
+        tol = 1e-9 if precision == lsdg.FP32 else 1e-5
+        assert rig <= tol and sc <= tol, (j, rig, sc)
+    assert list(res[0].frame_to_ref) == list(res[1].frame_to_ref)

@@ -238,3 +238,49 @@ def test_sim3_evaluate_and_track_parity(ctx, converged):
     if converged:
         H, H64 = np.array(st.hessian[:]), np.array(o64[1].hessian[:])
         assert np.abs(H - H64).max() <= 1e-4 * np.abs(H64).max()
+
+
+def test_se3_track_1280x960(ctx):
+    """Large-frame SE3 alignment (1280x960, ~300k keyframe points, 3 cm / 2
+    deg): same LM path as the FP32-term / FP64-sum restatement, within 1e-5
+    of the FP32-sum restatement; FP16 pyramids within 1e-4 of it."""
+    inst = synth.tracking_instance(1004, cam=synth.camera(1280, 960), trans_m=0.03, rot_deg=2.0, threshold=None,
+                                   fraction=0.25)
+    (kf, fr, ref), (pose, st, status), res = _track_all(ctx, inst, tracking_params(20))
+    assert ref.num_points(0) > 250000
+    assert status == res[2][2] == lsdg.OK
+    assert list(st.num_residual_calls) == list(res[2][1].num_residual_calls)
+    assert twist(pose, res[2][0]) <= 1e-9
+    assert twist(pose, res[0][0]) <= 1e-5
+    assert twist(pose, inst["gt"]) < 1e-2
+    _, (p16, st16, s16), _ = _track_all(ctx, inst, tracking_params(20), lsdg.FP16)
+    assert s16 == lsdg.OK and twist(p16, res[0][0]) <= 1e-4
+
+
+def _sim3_pair(sys_, d, precision=None):
+    f32 = (lambda a: a.astype(np.float32)) if isinstance(sys_, pyoracle.Oracle) else (lambda a: a)
+    kw = {} if precision is None else {"precision": precision}
+    a = sys_.frame(0, d["cam"], f32(d["a_img"]), **kw)
+    a.set_depth(d["a_idepth"], d["a_var"])
+    b = sys_.frame(1, d["cam"], f32(d["b_img"]), **kw)
+    b.set_depth(d["b_idepth"], d["b_var"])
+    return a, b, sys_.ref(a)
+
+
+def test_sim3_track_1920x1080(ctx):
+    """Keyframe-to-keyframe Sim3 at 1920x1080 (~500k points per keyframe,
+    scale drift in U[0.9, 1.1], up to 10 cm / 5 deg): the plain GPU path takes
+    the FP32-term / FP64-sum restatement's LM path (pose and scale within
+    1e-9, same call counts)."""
+    import posemath
+    d = synth.sim3_instance(1005)
+    a, b, ref = _sim3_pair(ctx, d)
+    assert ref.num_points(0) > 400000
+    out, st, status = ctx.sim3_track(ref, b, list(synth.IDENTITY7) + [1.0], 4, 1)
+    o = pyoracle.Oracle(2)
+    oa, ob, oref = _sim3_pair(o, d)
+    oout, ost, ostatus = o.sim3_track(oref, ob, list(synth.IDENTITY7) + [1.0], 4, 1)
+    assert status == ostatus == lsdg.OK
+    assert list(st.num_residual_calls) == list(ost.num_residual_calls)
+    rig, sc = posemath.sim3_error(out, oout)
+    assert rig <= 1e-9 and sc <= 1e-9, (rig, sc)
