@@ -79,6 +79,35 @@ class Tracking:
         return res
 
 
+def sim3_leg(ctx, steps, warmup, pairs=16, per_pair=8):
+    """Keyframe-to-keyframe Sim3 at 1920x1080 (~500k points per keyframe,
+    scale drift U[0.9, 1.1], up to 10 cm / 5 deg, FP16 pyramids, default LM
+    settings): 16 keyframe pairs x 8 jobs per batch, and the same pairs
+    through the plain path."""
+    ident = list(synth.IDENTITY7) + [1.0]
+    items = []
+    for k in range(pairs):
+        d = synth.sim3_instance(1005 + k)
+        a = ctx.frame(2 * k, d["cam"], d["a_img"], lsdg.FP16)
+        a.set_depth(d["a_idepth"], d["a_var"])
+        b = ctx.frame(2 * k + 1, d["cam"], d["b_img"], lsdg.FP16)
+        b.set_depth(d["b_idepth"], d["b_var"])
+        items.append((a, b, ctx.ref(a)))
+    for a, b, ref in items:
+        for l in range(1, 5):
+            ref.num_points(l)
+    jobs = [lsdg.Job(ref.h, b.h, lsdg.pose8(ident), lsdg.JOB_SIM3, 4, 1, 0) for a, b, ref in items
+            for _ in range(per_pair)]
+    bt = ctx.batch(jobs)
+    t = timed(ctx, bt.run, steps, warmup)
+    batched = len(jobs) * steps / t
+    t = timed(ctx, lambda: [ctx.sim3_track(ref, b, ident, 4, 1) for a, b, ref in items], 1, 1)
+    plain = pairs / t
+    bt.destroy()
+    return {"batched_alignments_per_s": round(batched, 1), "plain_alignments_per_s": round(plain, 1),
+            "jobs_per_batch": len(jobs)}
+
+
 def timed(ctx, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -273,6 +302,8 @@ def main():
     t = timed(ctx, tm.batch_step, args.steps, args.warmup)
     mixed_aps = 64 * args.steps / t
 
+    sim3 = sim3_leg(ctx, args.steps, args.warmup)
+
     depth, dinst = depth_leg(ctx, args.steps, args.warmup)
     ctx.set_params(params)
 
@@ -296,6 +327,7 @@ def main():
         "batched_fp32_frames_per_s": round(batched32_fps, 2),
         **out_plain,
         "mixed_resolution_batch_alignments_per_s": round(mixed_aps, 2),
+        "sim3_1920x1080": sim3,
         "depth_updates_per_s": round(depth["updates_per_s"], 2),
         "depth_update_ms": round(depth["ms_per_update"], 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

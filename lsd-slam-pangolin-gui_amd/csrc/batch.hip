@@ -44,27 +44,88 @@ __device__ inline void set_eval_pose(JobState& s, const Pose& T, bool sim3) {
   if (sim3) pose_roll_f(T, s.roll);
 }
 
+// The LM step runs on the 64 lanes of the block's first wave: its scalar
+// control flow is uniform over the wave (every lane computes the same values
+// from the same LDS state and writes the same values back), and the
+// independent divisions and trigonometric calls of the solve are spread over
+// the lanes.  `sc` is LM_SCRATCH doubles of LDS.
+constexpr int LM_SCRATCH = 64;
+
+__device__ inline int lm_lane() { return (int)(threadIdx.x & 63); }
+
+// Orders the wave's LDS writes before its later LDS reads.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* remaining) {
   s.status = status;
   *ntNext = 0;
-  atomicSub(remaining, 1);
+  if (lm_lane() == 0) atomicSub(remaining, 1);
+}
+
+// se3_exp (lsdg_lie.h) with sin / cos of theta / 2 and theta evaluated on
+// four lanes at once (same arguments, same functions, same results).
+__device__ inline Pose se3_exp_wave(const double* xi, double* sc) {
+  const double* w = xi + 3;
+  const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  const double th = sqrt(th2);
+  if (th >= 1e-10) {
+    const int lane = lm_lane();
+    if (lane < 2) sc[lane] = sin(lane == 0 ? 0.5 * th : th);
+    else if (lane < 4) sc[lane] = cos(lane == 2 ? 0.5 * th : th);
+    wave_sync();
+  }
+  Pose r;
+  double k, c;
+  if (th < 1e-10) {
+    k = 0.5 - th2 / 48.0;
+    c = 1.0 - th2 / 8.0;
+  } else {
+    k = sc[0] / th;
+    c = sc[2];
+  }
+  r.q[0] = k * w[0]; r.q[1] = k * w[1]; r.q[2] = k * w[2]; r.q[3] = c;
+  quat_normalize(r.q);
+  double bb, cc;
+  if (th < 1e-5) {
+    bb = 0.5 - th2 / 24.0;
+    cc = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    bb = (1.0 - sc[3]) / th2;
+    cc = (th - sc[1]) / (th2 * th);
+  }
+  apply_abc(1.0, bb, cc, w, xi, r.t);
+  r.s = 1.0;
+  return r;
 }
 
 // Solve (A diag*(1+lambda)) inc = b from the stored LGS and propose Tn = exp(inc) T.
 // Returns false when a Sim3 step diverges (|inc|^2 not in [0, 1)).
 template <int KIND>
-__device__ inline bool solve_and_propose(JobState& s) {
+__device__ inline bool solve_and_propose(JobState& s, double* sc) {
   constexpr int N = KIND == LSDG_JOB_SIM3 ? 7 : 6;
+  constexpr int NA = N * (N + 1) / 2;
   double A[N * N], b[N], inc[N];
-  double n = s.lgsN;
+  const double n = s.lgsN;
+  // the normalisation's NA + N divisions, one per lane
+  {
+    const int lane = lm_lane();
+    if (lane < NA) sc[lane] = s.lgsA[lane] / n;
+    else if (lane < NA + N) sc[lane] = -s.lgsB[lane - NA] / n;
+    wave_sync();
+  }
   int k = 0;
   for (int a = 0; a < N; ++a)
     for (int c = a; c < N; ++c) {
-      double v = s.lgsA[k++] / n;
+      double v = sc[k++];
       A[N * a + c] = v;
       A[N * c + a] = v;
     }
-  for (int a = 0; a < N; ++a) b[a] = -s.lgsB[a] / n;
+  for (int a = 0; a < N; ++a) b[a] = sc[NA + a];
+  wave_sync();
   for (int a = 0; a < N; ++a) A[(N + 1) * a] *= 1.0 + (double)s.lambda;
   ldlt_solve<N>(A, b, inc);
   s.incTry++;
@@ -78,7 +139,7 @@ __device__ inline bool solve_and_propose(JobState& s) {
     if (!(dd >= 0 && dd < 1)) return false;
     s.Tn = pose_mul(sim3_exp(inc), s.T);
   } else {
-    s.Tn = pose_mul(se3_exp(inc), s.T);
+    s.Tn = pose_mul(se3_exp_wave(inc, sc), s.T);
   }
   s.phase = 1;
   set_eval_pose(s, s.Tn, KIND == LSDG_JOB_SIM3);
@@ -130,7 +191,7 @@ __device__ inline void begin_level(JobState& s, const JobDesc& jd, const lsdg_pa
 
 // One LM step of an SE3 job, after the evaluation whose sums are S.
 __device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, const lsdg_params& P, int* ntNext,
-                            int* remaining) {
+                            int* remaining, double* sc) {
   const int lvl = s.level;
   const JobLevel& L = jd.lv[lvl];
   s.evals++;
@@ -163,7 +224,7 @@ __device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, con
     store_lgs<LSDG_JOB_SE3>(s, S);
     s.numWarp[lvl]++;
     s.incTry = 0;
-    solve_and_propose<LSDG_JOB_SE3>(s);
+    solve_and_propose<LSDG_JOB_SE3>(s, sc);
     *ntNext = L.ntiles;
     return;
   }
@@ -181,7 +242,7 @@ __device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, con
     store_lgs<LSDG_JOB_SE3>(s, S);
     s.numWarp[lvl]++;
     s.incTry = 0;
-    solve_and_propose<LSDG_JOB_SE3>(s);
+    solve_and_propose<LSDG_JOB_SE3>(s, sc);
     *ntNext = L.ntiles;
     return;
   }
@@ -190,7 +251,7 @@ __device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, con
     return;
   }
   s.lambda = s.lambda == 0.0f ? 0.2f : s.lambda * pow_int(P.lambda_fail_fac[lvl], s.incTry);
-  solve_and_propose<LSDG_JOB_SE3>(s);
+  solve_and_propose<LSDG_JOB_SE3>(s, sc);
   *ntNext = L.ntiles;
 }
 
@@ -207,7 +268,7 @@ __device__ inline void sim3_res(const double* S, double* r) {
 
 // One LM step of a Sim3 job.
 __device__ void lm_step_sim3(JobState& s, const JobDesc& jd, const double* S, const lsdg_params& P, int* ntNext,
-                             int* remaining) {
+                             int* remaining, double* sc) {
   const int lvl = s.level;
   const JobLevel& L = jd.lv[lvl];
   s.evals++;
@@ -238,7 +299,7 @@ __device__ void lm_step_sim3(JobState& s, const JobDesc& jd, const double* S, co
     s.upToDate = 1;
     s.numWarp[lvl]++;
     s.incTry = 0;
-    if (!solve_and_propose<LSDG_JOB_SIM3>(s)) {
+    if (!solve_and_propose<LSDG_JOB_SIM3>(s, sc)) {
       for (int k = 0; k < 28; ++k) s.lgsA[k] = 0.0;
       finish_job(s, 2, ntNext, remaining);
       return;
@@ -268,7 +329,7 @@ __device__ void lm_step_sim3(JobState& s, const JobDesc& jd, const double* S, co
     }
     s.lambda = s.lambda == 0.0f ? 0.2f : s.lambda * pow_int(P.lambda_fail_fac[lvl], s.incTry);
   }
-  if (!solve_and_propose<LSDG_JOB_SIM3>(s)) {
+  if (!solve_and_propose<LSDG_JOB_SIM3>(s, sc)) {
     for (int k = 0; k < 28; ++k) s.lgsA[k] = 0.0;
     finish_job(s, 2, ntNext, remaining);
     return;
@@ -403,6 +464,7 @@ __global__ __launch_bounds__(BATCH_SOLVE_BS) void k_batch_solve(BatchArgs a, int
   __shared__ JobState sst;
   __shared__ JobDesc sjd;
   __shared__ int next;
+  __shared__ double lmsc[LM_SCRATCH];
   const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nj = a.njobs;
   const int n = a.ntiles[parity * nj + j];
@@ -435,11 +497,11 @@ __global__ __launch_bounds__(BATCH_SOLVE_BS) void k_batch_solve(BatchArgs a, int
     S[tid] = v;
   }
   __syncthreads();
-  if (tid == 0) {
+  if (tid < 64) {
     int nx = 0;
-    if (KIND == LSDG_JOB_SIM3) lm_step_sim3(sst, sjd, S, a.P, &nx, a.remaining);
-    else lm_step_se3(sst, sjd, S, a.P, &nx, a.remaining);
-    next = nx;
+    if (KIND == LSDG_JOB_SIM3) lm_step_sim3(sst, sjd, S, a.P, &nx, a.remaining, lmsc);
+    else lm_step_se3(sst, sjd, S, a.P, &nx, a.remaining, lmsc);
+    if (tid == 0) next = nx;
   }
   __syncthreads();
   block_copy(a.st + j, &sst, BATCH_SOLVE_BS);
@@ -463,6 +525,7 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
   __shared__ JobDesc jd;
   __shared__ double sh[WAVES][NS];
   __shared__ double S[NS];
+  __shared__ double lmsc[LM_SCRATCH];
   const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   block_copy(&s, a.st + j, BATCH_JOB_BS);
   block_copy(&jd, a.jobs + j, BATCH_JOB_BS);
@@ -496,10 +559,10 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
       S[tid] = v;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
       int next = 0;
-      if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, S, a.P, &next, a.remaining);
-      else lm_step_se3(s, jd, S, a.P, &next, a.remaining);
+      if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, S, a.P, &next, a.remaining, lmsc);
+      else lm_step_se3(s, jd, S, a.P, &next, a.remaining, lmsc);
     }
     __syncthreads();
   }

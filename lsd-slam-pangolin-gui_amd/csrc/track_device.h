@@ -201,23 +201,24 @@ struct Accum<ACC_F64, NS> {
     static_assert(NS <= 64, "reduce-scatter holds one value per lane");
     const int lane = threadIdx.x & 63;
     double a[32];
-    {
-      const bool hi = (lane & 32) != 0;
+    // offset 32: v_permlane32_swap exchanges the upper half-wave of `lo` with
+    // the lower half-wave of `up`, so lo + up is the pair sum of value i on
+    // lanes 0-31 and of value i + 32 on lanes 32-63 (no lane selects)
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        if (i < NS) {
-          double lo = v[i];
-          double up = (i + 32 < NS) ? v[i + 32] : 0.0;
-          double send = hi ? lo : up;
-          double keep = hi ? up : lo;
-          a[i] = keep + __shfl_xor(send, 32);
-        } else {
-          a[i] = 0.0;
-        }
+    for (int i = 0; i < 32; ++i) {
+      if (i < NS) {
+        const double lo = v[i];
+        const double up = (i + 32 < NS) ? v[i + 32] : 0.0;
+        a[i] = swap_sum(lo, up, 32);
+      } else {
+        a[i] = 0.0;
       }
     }
+    // offset 16: v_permlane16_swap does the same between odd and even rows
 #pragma unroll
-    for (int h = 16; h >= 1; h >>= 1) {
+    for (int i = 0; i < 16; ++i) a[i] = swap_sum(a[i], a[i + 16], 16);
+#pragma unroll
+    for (int h = 8; h >= 1; h >>= 1) {
       const bool hi = (lane & h) != 0;
 #pragma unroll
       for (int i = 0; i < h; ++i) {
@@ -227,6 +228,27 @@ struct Accum<ACC_F64, NS> {
       }
     }
     return a[0];
+  }
+
+ private:
+  // lo + up after swapping lo's upper half (offset 32) or odd rows (offset
+  // 16) with up's lower half / even rows, 32 bits at a time.
+  __device__ static inline double swap_sum(double lo, double up, int offset) {
+    const unsigned long long l = (unsigned long long)__double_as_longlong(lo);
+    const unsigned long long u = (unsigned long long)__double_as_longlong(up);
+    unsigned l0 = (unsigned)l, l1 = (unsigned)(l >> 32), u0 = (unsigned)u, u1 = (unsigned)(u >> 32);
+    if (offset == 32) {
+      auto r0 = __builtin_amdgcn_permlane32_swap(l0, u0, false, false);
+      auto r1 = __builtin_amdgcn_permlane32_swap(l1, u1, false, false);
+      l0 = r0[0]; u0 = r0[1]; l1 = r1[0]; u1 = r1[1];
+    } else {
+      auto r0 = __builtin_amdgcn_permlane16_swap(l0, u0, false, false);
+      auto r1 = __builtin_amdgcn_permlane16_swap(l1, u1, false, false);
+      l0 = r0[0]; u0 = r0[1]; l1 = r1[0]; u1 = r1[1];
+    }
+    const double nl = __longlong_as_double((long long)(((unsigned long long)l1 << 32) | l0));
+    const double nu = __longlong_as_double((long long)(((unsigned long long)u1 << 32) | u0));
+    return nl + nu;
   }
 };
 
