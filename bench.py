@@ -74,9 +74,14 @@ class Tracking:
         self.batch = self.ctx.batch(jobs)
 
     def batch_step(self):
-        res = self.batch.run()
-        self.poses = [np.array(r.frame_to_ref[:7]) for r in res]
-        return res
+        """One batch run; the C call copies every job's pose and statistics
+        into the result array (host memory) before it returns."""
+        self.results = self.batch.run()
+        return self.results
+
+    def collect(self):
+        self.poses = [np.array(r.frame_to_ref[:7]) for r in self.results]
+        return self.poses
 
 
 def sim3_leg(ctx, steps, warmup, pairs=16, per_pair=8):
@@ -260,6 +265,7 @@ def main():
     batched16_fps = B * args.steps / t
     ms_per_step = 1e3 * t / args.steps
     res16 = t16.batch_step()
+    t16.collect()
     achieved, ks = kernel_roofline(ctx, t16.batch_step, lsdg.KERNEL_BATCH_EVAL)
     solve = ctx.kernel_stats(lsdg.KERNEL_BATCH_SOLVE)
     jobk = ctx.kernel_stats(lsdg.KERNEL_BATCH_JOB)
@@ -279,7 +285,7 @@ def main():
     t = timed(ctx, t32.batch_step, args.steps, args.warmup)
     batched32_fps = B * args.steps / t
     t32.batch_step()
-    poses32 = list(t32.poses)
+    poses32 = list(t32.collect())
 
     out_plain = {}
     if not args.no_plain:

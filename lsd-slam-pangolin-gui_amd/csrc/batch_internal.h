@@ -7,7 +7,7 @@ namespace lsdg {
 
 constexpr int MAX_BATCH_JOBS = 2048;
 constexpr int BATCH_ACC_F64 = 0;
-constexpr int BATCH_ACC_KAHAN = 1;
+constexpr int BATCH_ACC_F32 = 1;
 // Whole-job kernel: block size and the bound on the evaluations one job may
 // run inside it (the LM loop ends long before; the bound makes every block's
 // exit unconditional).

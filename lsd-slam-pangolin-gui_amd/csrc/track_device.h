@@ -164,13 +164,62 @@ __device__ inline void jac6(float px, float py, float pz, float gx, float gy, fl
 }
 
 // Per-thread accumulators of the per-point FP32 terms.
-//  ACC_F64:   each FP32 term is widened and summed in FP64 (the parity mode:
-//             the same terms as the CPU restatement, summed to ~1e-16).
-//  ACC_KAHAN: FP32 sums with an error-free TwoSum compensation term; the wave
-//             and block trees combine (hi, lo) pairs in FP32, and FP64 is used
-//             only when the block's pair is written out for the cross-block merge.
+//  ACC_F64: each FP32 term is widened and summed in FP64 (the parity mode:
+//           the same terms as the CPU restatement, summed to ~1e-16).
+//  ACC_F32: each thread sums its few points (a tile gives it 4) in FP32;
+//           the wave, block and cross-block trees run in FP64.  Half the
+//           accumulator registers and VALU work of ACC_F64, for the
+//           reduced-precision pyramids, whose taps carry 11 or 8 bits.
 constexpr int ACC_F64 = 0;
-constexpr int ACC_KAHAN = 1;
+constexpr int ACC_F32 = 1;
+
+// lo + up after swapping lo's upper half-wave (offset 32) or odd rows
+// (offset 16) with up's lower half-wave / even rows, 32 bits at a time
+// (v_permlane32_swap / v_permlane16_swap).
+__device__ inline double swap_sum(double lo, double up, int offset) {
+  const unsigned long long l = (unsigned long long)__double_as_longlong(lo);
+  const unsigned long long u = (unsigned long long)__double_as_longlong(up);
+  unsigned l0 = (unsigned)l, l1 = (unsigned)(l >> 32), u0 = (unsigned)u, u1 = (unsigned)(u >> 32);
+  if (offset == 32) {
+    auto r0 = __builtin_amdgcn_permlane32_swap(l0, u0, false, false);
+    auto r1 = __builtin_amdgcn_permlane32_swap(l1, u1, false, false);
+    l0 = r0[0]; u0 = r0[1]; l1 = r1[0]; u1 = r1[1];
+  } else {
+    auto r0 = __builtin_amdgcn_permlane16_swap(l0, u0, false, false);
+    auto r1 = __builtin_amdgcn_permlane16_swap(l1, u1, false, false);
+    l0 = r0[0]; u0 = r0[1]; l1 = r1[0]; u1 = r1[1];
+  }
+  const double nl = __longlong_as_double((long long)(((unsigned long long)l1 << 32) | l0));
+  const double nu = __longlong_as_double((long long)(((unsigned long long)u1 << 32) | u0));
+  return nl + nu;
+}
+
+// Reduce-scatter over the wave (NS <= 64): afterwards lane l holds the wave
+// total of value l (0 for l >= NS).  Halving butterfly: at offset o each lane
+// keeps one half of its values and receives the other half from lane ^ o.
+// Offsets 32 and 16 use the permlane swaps (the sum of the swapped pair is
+// already the kept half: no lane selects); offsets 8..1 shuffle.
+template <int NS>
+__device__ inline double wave_reduce_scatter_f64(const double* v) {
+  static_assert(NS <= 64, "reduce-scatter holds one value per lane");
+  const int lane = threadIdx.x & 63;
+  double a[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) a[i] = i < NS ? swap_sum(v[i], (i + 32 < NS) ? v[i + 32] : 0.0, 32) : 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = swap_sum(a[i], a[i + 16], 16);
+#pragma unroll
+  for (int h = 8; h >= 1; h >>= 1) {
+    const bool hi = (lane & h) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      double send = hi ? a[i] : a[i + h];
+      double keep = hi ? a[i + h] : a[i];
+      a[i] = keep + __shfl_xor(send, h);
+    }
+  }
+  return a[0];
+}
 
 template <int MODE, int NS>
 struct Accum;
@@ -193,146 +242,34 @@ struct Accum<ACC_F64, NS> {
     }
   }
   __device__ inline double value(int k) const { return v[k]; }
-  // Reduce-scatter over the wave (NS <= 64): afterwards lane l holds the wave
-  // total of value l (0 for l >= NS).  Halving butterfly: at offset o each
-  // lane keeps one half of its values and sends the other half to lane ^ o,
-  // 63 shuffles in all instead of 6 * NS.
-  __device__ inline double wave_reduce_scatter() const {
-    static_assert(NS <= 64, "reduce-scatter holds one value per lane");
-    const int lane = threadIdx.x & 63;
-    double a[32];
-    // offset 32: v_permlane32_swap exchanges the upper half-wave of `lo` with
-    // the lower half-wave of `up`, so lo + up is the pair sum of value i on
-    // lanes 0-31 and of value i + 32 on lanes 32-63 (no lane selects)
-#pragma unroll
-    for (int i = 0; i < 32; ++i) {
-      if (i < NS) {
-        const double lo = v[i];
-        const double up = (i + 32 < NS) ? v[i + 32] : 0.0;
-        a[i] = swap_sum(lo, up, 32);
-      } else {
-        a[i] = 0.0;
-      }
-    }
-    // offset 16: v_permlane16_swap does the same between odd and even rows
-#pragma unroll
-    for (int i = 0; i < 16; ++i) a[i] = swap_sum(a[i], a[i + 16], 16);
-#pragma unroll
-    for (int h = 8; h >= 1; h >>= 1) {
-      const bool hi = (lane & h) != 0;
-#pragma unroll
-      for (int i = 0; i < h; ++i) {
-        double send = hi ? a[i] : a[i + h];
-        double keep = hi ? a[i + h] : a[i];
-        a[i] = keep + __shfl_xor(send, h);
-      }
-    }
-    return a[0];
-  }
-
- private:
-  // lo + up after swapping lo's upper half (offset 32) or odd rows (offset
-  // 16) with up's lower half / even rows, 32 bits at a time.
-  __device__ static inline double swap_sum(double lo, double up, int offset) {
-    const unsigned long long l = (unsigned long long)__double_as_longlong(lo);
-    const unsigned long long u = (unsigned long long)__double_as_longlong(up);
-    unsigned l0 = (unsigned)l, l1 = (unsigned)(l >> 32), u0 = (unsigned)u, u1 = (unsigned)(u >> 32);
-    if (offset == 32) {
-      auto r0 = __builtin_amdgcn_permlane32_swap(l0, u0, false, false);
-      auto r1 = __builtin_amdgcn_permlane32_swap(l1, u1, false, false);
-      l0 = r0[0]; u0 = r0[1]; l1 = r1[0]; u1 = r1[1];
-    } else {
-      auto r0 = __builtin_amdgcn_permlane16_swap(l0, u0, false, false);
-      auto r1 = __builtin_amdgcn_permlane16_swap(l1, u1, false, false);
-      l0 = r0[0]; u0 = r0[1]; l1 = r1[0]; u1 = r1[1];
-    }
-    const double nl = __longlong_as_double((long long)(((unsigned long long)l1 << 32) | l0));
-    const double nu = __longlong_as_double((long long)(((unsigned long long)u1 << 32) | u0));
-    return nl + nu;
-  }
+  __device__ inline double wave_reduce_scatter() const { return wave_reduce_scatter_f64<NS>(v); }
 };
 
-// Error-free sum of two floats: hi + lo == a + b exactly (round-to-nearest,
-// no contraction).
-__device__ inline void two_sum(float a, float b, float& hi, float& lo) {
-  hi = a + b;
-  float bp = hi - a;
-  lo = (a - (hi - bp)) + (b - bp);
-}
-
 template <int NS>
-struct Accum<ACC_KAHAN, NS> {
-  float s[NS], c[NS];
+struct Accum<ACC_F32, NS> {
+  float v[NS];
   __device__ inline void zero() {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) s[k] = c[k] = 0.f;
+    for (int k = 0; k < NS; ++k) v[k] = 0.f;
   }
-  __device__ inline void add(int k, float x) {
-    float hi, lo;
-    two_sum(s[k], x, hi, lo);
-    s[k] = hi;
-    c[k] += lo;
-  }
+  __device__ inline void add(int k, float x) { v[k] += x; }
   __device__ inline void wave_reduce() {
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      float a = s[k], b = c[k];
+      double a = (double)v[k];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        float a2 = __shfl_xor(a, off), b2 = __shfl_xor(b, off);
-        float hi, lo;
-        two_sum(a, a2, hi, lo);
-        a = hi;
-        b = b + b2 + lo;
-      }
-      s[k] = a;
-      c[k] = b;
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+      v[k] = (float)a;
     }
   }
-  __device__ inline double value(int k) const { return (double)s[k] + (double)c[k]; }
-  // Reduce-scatter of the (hi, lo) pairs over the wave; lane l returns value l.
+  __device__ inline double value(int k) const { return (double)v[k]; }
   __device__ inline double wave_reduce_scatter() const {
-    static_assert(NS <= 64, "reduce-scatter holds one value per lane");
-    const int lane = threadIdx.x & 63;
-    float as[32], ac[32];
-    {
-      const bool hi = (lane & 32) != 0;
+    double d[NS];
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        if (i < NS) {
-          float los = s[i], loc = c[i];
-          float ups = (i + 32 < NS) ? s[i + 32] : 0.f, upc = (i + 32 < NS) ? c[i + 32] : 0.f;
-          float ss = hi ? los : ups, sc = hi ? loc : upc;
-          float ks = hi ? ups : los, kc = hi ? upc : loc;
-          float rs = __shfl_xor(ss, 32), rc = __shfl_xor(sc, 32);
-          float h2, l2;
-          two_sum(ks, rs, h2, l2);
-          as[i] = h2;
-          ac[i] = kc + rc + l2;
-        } else {
-          as[i] = 0.f;
-          ac[i] = 0.f;
-        }
-      }
-    }
-#pragma unroll
-    for (int h = 16; h >= 1; h >>= 1) {
-      const bool hi = (lane & h) != 0;
-#pragma unroll
-      for (int i = 0; i < h; ++i) {
-        float ss = hi ? as[i] : as[i + h], sc = hi ? ac[i] : ac[i + h];
-        float ks = hi ? as[i + h] : as[i], kc = hi ? ac[i + h] : ac[i];
-        float rs = __shfl_xor(ss, h), rc = __shfl_xor(sc, h);
-        float h2, l2;
-        two_sum(ks, rs, h2, l2);
-        as[i] = h2;
-        ac[i] = kc + rc + l2;
-      }
-    }
-    return (double)as[0] + (double)ac[0];
+    for (int k = 0; k < NS; ++k) d[k] = (double)v[k];
+    return wave_reduce_scatter_f64<NS>(d);
   }
 };
-
 // Index of (i, j), i <= j, in the row-major upper triangle of a 7x7 matrix.
 __host__ __device__ constexpr int tri7(int i, int j) { return i * 7 - i * (i - 1) / 2 + (j - i); }
 
