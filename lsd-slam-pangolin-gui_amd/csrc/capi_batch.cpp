@@ -29,6 +29,12 @@ struct lsdg_batch_s {
   int acc = BATCH_ACC_F64;
   int ppt = 4;
   int grid = 1024;
+  // Independent chains: the jobs are split into `chains` groups whose LM
+  // steps run as parallel graph branches, so one group's latency-bound solve
+  // overlaps another group's evaluation.
+  int chains = 1;
+  std::vector<hipStream_t> side;  // streams of chains 1..chains-1 during capture
+  std::vector<hipEvent_t> forkJoin;
   int steps = 16;  // multi-block LM steps in the first graph (even, may be 0)
   int contSteps = 16;  // LM steps per continuation graph (even, > 0)
   // Levels of up to this many points run in k_batch_job (12 points per
@@ -65,6 +71,8 @@ static void batch_free(lsdg_batch b) {
   if (b->d_st) (void)hipFree(b->d_st);
   if (b->d_ntiles) (void)hipFree(b->d_ntiles);
   if (b->d_counters) (void)hipFree(b->d_counters);
+  for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : b->forkJoin) (void)hipEventDestroy(ev);
   if (b->d_remaining) (void)hipFree(b->d_remaining);
   if (b->h_remaining) (void)hipHostFree(b->h_remaining);
   if (b->d_partials) (void)hipFree(b->d_partials);
@@ -124,15 +132,21 @@ static lsdg_status build_job_table(lsdg_batch b) {
   return LSDG_OK;
 }
 
-static BatchArgs batch_args(lsdg_batch b) {
+// First job of chain c (chain c holds jobs [chain_begin(c), chain_begin(c + 1))).
+static int chain_begin(lsdg_batch b, int c) { return (int)((long long)c * b->n / b->chains); }
+
+// Kernel arguments of chain c: its jobs, control blocks, tile counts
+// ([2][jobs of the chain]) and remaining-jobs counter.
+static BatchArgs batch_args(lsdg_batch b, int c) {
+  const int j0 = chain_begin(b, c), j1 = chain_begin(b, c + 1);
   BatchArgs a;
-  a.jobs = b->d_jobs;
-  a.st = b->d_st;
-  a.njobs = b->n;
-  a.ntiles = b->d_ntiles;
-  a.counters = b->d_counters;
+  a.jobs = b->d_jobs + j0;
+  a.st = b->d_st + j0;
+  a.njobs = j1 - j0;
+  a.ntiles = b->d_ntiles + 2 * j0;
+  a.counters = b->d_counters + j0;
   a.partials = b->d_partials;
-  a.remaining = b->d_remaining;
+  a.remaining = b->d_remaining + c;
   a.ppt = b->ppt;
   a.jobMaxPoints = b->jobMaxPoints;
   a.P = b->ctx->P;
@@ -140,23 +154,34 @@ static BatchArgs batch_args(lsdg_batch b) {
   return a;
 }
 
-// Captures one graph: optional pyramid rebuild and init, then `steps` LM
-// steps starting at parity 0, then the remaining-jobs counter to host memory.
+// Captures one graph: optional pyramid rebuild; then per chain, as parallel
+// branches: init and whole-job kernel (first graph only), `steps` LM steps
+// from parity 0 and the chain's remaining-jobs counter to host memory.
 static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec_t* exec) {
   lsdg_ctx c = b->ctx;
-  BatchArgs a = batch_args(b);
   CK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  if (first) {
-    if (b->pyr.nf > 0) e = launch_pyr_batch(b->pyr, c->stream);
-    if (e == hipSuccess) e = launch_batch_init(a, c->stream);
-    if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, c->stream);
-  }
+  if (first && b->pyr.nf > 0) e = launch_pyr_batch(b->pyr, c->stream);
+  // fork: every side stream starts after the main stream's work so far
+  if (e == hipSuccess && b->chains > 1) e = hipEventRecord(b->forkJoin[0], c->stream);
+  for (int k = 1; k < b->chains && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->side[k - 1], b->forkJoin[0], 0);
   const int steps = first ? b->steps : b->contSteps;
-  for (int s = 0; s < steps && e == hipSuccess; ++s)
-    e = launch_batch_step(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(b->h_remaining, b->d_remaining, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  for (int k = 0; k < b->chains && e == hipSuccess; ++k) {
+    hipStream_t st = k == 0 ? c->stream : b->side[k - 1];
+    BatchArgs a = batch_args(b, k);
+    if (first) {
+      e = launch_batch_init(a, st);
+      if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, st);
+    }
+    for (int s = 0; s < steps && e == hipSuccess; ++s)
+      e = launch_batch_step(a, b->kind, b->prec, b->acc, s & 1, b->grid, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->h_remaining + k, b->d_remaining + k, sizeof(int), hipMemcpyDeviceToHost, st);
+  }
+  // join
+  for (int k = 1; k < b->chains && e == hipSuccess; ++k) {
+    e = hipEventRecord(b->forkJoin[k], b->side[k - 1]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, b->forkJoin[k], 0);
+  }
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -223,9 +248,12 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   b->acc = BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_ACC")) b->acc = std::atoi(e) ? BATCH_ACC_F32 : BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_PPT")) b->ppt = std::max(1, std::min(64, std::atoi(e)));
-  if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));
   if (const char* e = std::getenv("LSDG_BATCH_STEPS")) b->steps = std::max(0, std::min(512, std::atoi(e) & ~1));
   if (const char* e = std::getenv("LSDG_BATCH_JOB_POINTS")) b->jobMaxPoints = std::max(0, std::atoi(e));
+  b->chains = n >= 16 ? 2 : 1;
+  if (const char* e = std::getenv("LSDG_BATCH_CHAINS")) b->chains = std::max(1, std::min(8, std::min(n, std::atoi(e))));
+  b->grid = 1024 / b->chains;  // evaluation blocks per chain
+  if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));
   b->jobs.assign(jobs, jobs + n);
   for (auto& J : b->jobs) {
     if (kind == LSDG_JOB_SE3) {
@@ -244,6 +272,21 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
       (e = hipHostMalloc(&b->h_remaining, 64, hipHostMallocDefault)) != hipSuccess) {
     batch_free(b);
     return fail(c, LSDG_ENOMEM, "batch allocation", e);
+  }
+  for (int k = 0; k < b->chains; ++k) {
+    hipEvent_t ev = nullptr;
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+      batch_free(b);
+      return fail(c, LSDG_EDEVICE, "batch events", e);
+    }
+    b->forkJoin.push_back(ev);
+    if (k == 0) continue;
+    hipStream_t st = nullptr;
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) {
+      batch_free(b);
+      return fail(c, LSDG_EDEVICE, "batch streams", e);
+    }
+    b->side.push_back(st);
   }
   // pyramid data each frame is tracked on (union over its jobs), built now so
   // the frame holds it; frames the batch rebuilds get it again on every run
@@ -372,7 +415,6 @@ static void fill_result(lsdg_batch b, int j, const JobState& s, lsdg_job_result*
 // around every LM-step kernel (and the pyramid rebuild) for lsdg_ctx_profile.
 static lsdg_status batch_run_profiled(lsdg_batch b) {
   lsdg_ctx c = b->ctx;
-  BatchArgs a = batch_args(b);
   if (b->pyr.nf > 0) {
     double pb = 0.0;
     for (size_t k = 0; k < b->pyrFrames.size(); ++k) pb += pyramid_bytes(b->pyrFrames[k], true, b->pyrNeed[k]);
@@ -380,27 +422,39 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
     CK(c, launch_pyr_batch(b->pyr, c->stream));
     prof_end(c, pm, pb);
   }
-  CK(c, launch_batch_init(a, c->stream));
-  if (b->jobMaxPoints > 0) {
-    KTimer kt = prof_timer(c);
-    CK(c, launch_batch_job(a, b->kind, b->prec, b->acc, c->stream, &kt));
-    prof_timer_done(c, LSDG_KERNEL_BATCH_JOB, kt, 0.0);
-  }
-  for (int round = 0; round <= 64; ++round) {
-    const int steps = round == 0 ? b->steps : b->contSteps;
-    for (int s = 0; s < steps; ++s) {
-      KTimer ke = prof_timer(c);
-      CK(c, launch_batch_eval(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream, &ke));
-      prof_timer_done(c, LSDG_KERNEL_BATCH_EVAL, ke, 0.0);
-      KTimer ks = prof_timer(c);
-      CK(c, launch_batch_solve(a, b->kind, s & 1, c->stream, &ks));
-      prof_timer_done(c, LSDG_KERNEL_BATCH_SOLVE, ks, 0.0);
+  // chains one after the other on the context stream
+  for (int k = 0; k < b->chains; ++k) {
+    BatchArgs a = batch_args(b, k);
+    CK(c, launch_batch_init(a, c->stream));
+    if (b->jobMaxPoints > 0) {
+      KTimer kt = prof_timer(c);
+      CK(c, launch_batch_job(a, b->kind, b->prec, b->acc, c->stream, &kt));
+      prof_timer_done(c, LSDG_KERNEL_BATCH_JOB, kt, 0.0);
     }
-    CK(c, hipMemcpyAsync(b->h_remaining, b->d_remaining, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    CK(c, hipStreamSynchronize(c->stream));
-    if (*b->h_remaining <= 0) return prof_resolve(c);
+    bool done = false;
+    for (int round = 0; round <= 64 && !done; ++round) {
+      const int steps = round == 0 ? b->steps : b->contSteps;
+      for (int s = 0; s < steps; ++s) {
+        KTimer ke = prof_timer(c);
+        CK(c, launch_batch_eval(a, b->kind, b->prec, b->acc, s & 1, b->grid, c->stream, &ke));
+        prof_timer_done(c, LSDG_KERNEL_BATCH_EVAL, ke, 0.0);
+        KTimer ks = prof_timer(c);
+        CK(c, launch_batch_solve(a, b->kind, s & 1, c->stream, &ks));
+        prof_timer_done(c, LSDG_KERNEL_BATCH_SOLVE, ks, 0.0);
+      }
+      CK(c, hipMemcpyAsync(b->h_remaining + k, b->d_remaining + k, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      CK(c, hipStreamSynchronize(c->stream));
+      done = b->h_remaining[k] <= 0;
+    }
+    if (!done) return fail(c, LSDG_ESTATE, "batched LM did not finish");
   }
-  return fail(c, LSDG_ESTATE, "batched LM did not finish");
+  return prof_resolve(c);
+}
+
+static int remaining_jobs(lsdg_batch b) {
+  int r = 0;
+  for (int k = 0; k < b->chains; ++k) r += b->h_remaining[k];
+  return r;
 }
 
 extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
@@ -421,7 +475,7 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
     CK(c, hipGraphLaunch(b->first, c->stream));
     CK(c, hipStreamSynchronize(c->stream));
     int launches = 0;
-    while (*b->h_remaining > 0) {
+    while (remaining_jobs(b) > 0) {
       if (++launches > 64) return fail(c, LSDG_ESTATE, "batched LM did not finish within 64 continuation graphs");
       CK(c, hipGraphLaunch(b->cont, c->stream));
       CK(c, hipStreamSynchronize(c->stream));

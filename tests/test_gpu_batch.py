@@ -1,9 +1,11 @@
 """Batched tracking (hipGraph-replayed) against the plain path, per instance.
 
-Every test runs in the three schedules a plan can take: every level in the
+Every test runs in the schedules a plan can take: every level in the
 whole-job kernel (one block per job), every level in the multi-block steps
-(tile evaluation + per-job solve launches), and the hybrid (levels up to 3000
-points in the whole-job kernel, larger ones in the steps).
+(tile evaluation + per-job solve launches), the hybrid (levels up to 3000
+points in the whole-job kernel, larger ones in the steps), the defaults, and
+the hybrid with the jobs split over 3 chains that step as parallel graph
+branches.
 
 Bars: FP32 pyramids with FP64 sums take the plain path's LM path (poses within
 1e-9 twist, identical call counts); FP16 pyramids stay within 1e-5 of the
@@ -19,15 +21,18 @@ from helpers import tracking_params, twist
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["job", "steps", "hybrid", "default"])
+@pytest.fixture(autouse=True, params=["job", "steps", "hybrid", "default", "chains"])
 def schedule(request, monkeypatch):
-    """Levels up to LSDG_BATCH_JOB_POINTS points run in the whole-job kernel
-    (read when a plan is created)."""
-    points = {"job": "1000000000", "steps": "0", "hybrid": "3000", "default": None}[request.param]
-    if points is None:
-        monkeypatch.delenv("LSDG_BATCH_JOB_POINTS", raising=False)
-    else:
-        monkeypatch.setenv("LSDG_BATCH_JOB_POINTS", points)
+    """Levels up to LSDG_BATCH_JOB_POINTS points run in the whole-job kernel;
+    LSDG_BATCH_CHAINS groups of jobs step as parallel graph branches (both
+    read when a plan is created)."""
+    points, chains = {"job": ("1000000000", "1"), "steps": ("0", "1"), "hybrid": ("3000", "1"),
+                      "default": (None, None), "chains": ("3000", "3")}[request.param]
+    for var, val in (("LSDG_BATCH_JOB_POINTS", points), ("LSDG_BATCH_CHAINS", chains)):
+        if val is None:
+            monkeypatch.delenv(var, raising=False)
+        else:
+            monkeypatch.setenv(var, val)
     return request.param
 
 
