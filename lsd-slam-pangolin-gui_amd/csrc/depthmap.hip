@@ -6,6 +6,7 @@
 //   k_fill_holes   regularizeDepthMapFillHoles; the reference's integral image
 //                  of validity is replaced by the equal integer 5x5 box sum.
 //   k_regularize   regularizeDepthMap(removeOcclusions, validityTH), 5x5 gather.
+//                  Both stage a 16x16 tile plus its 2-pixel halo in LDS.
 //   k_set_depth    Frame::setDepth (level 0; the pyramid follows).
 //   k_prop_*       propagateDepth as a deterministic scatter: project every
 //                  source, count per destination, scan, bucket, then one
@@ -485,32 +486,67 @@ __global__ __launch_bounds__(256) void k_observe(ObsArgs a, lsdg_params P) {
 }
 
 // ---------------------------------------------------------- fill holes
+// A 16x16 block of pixels and its 2-pixel halo of input hypotheses
+// (validity flag, validity counter, idepth / variance quadruple) in LDS;
+// positions outside the image read as invalid.
+struct HaloTile {
+  static constexpr int N = 20;
+  uint8_t valid[N][N];
+  int val[N][N];
+  float4 dv[N][N];
+};
+
+__device__ inline void load_halo(HaloTile& t, const HypBuf& in, int w, int h) {
+  const int x0 = blockIdx.x * 16 - 2, y0 = blockIdx.y * 16 - 2;
+  for (int k = threadIdx.x; k < HaloTile::N * HaloTile::N; k += blockDim.x) {
+    const int ty = k / HaloTile::N, tx = k - ty * HaloTile::N;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= 0 && x < w && y >= 0 && y < h) {
+      const int j = x + y * w;
+      const uint8_t v = in.valid[j];
+      t.valid[ty][tx] = v;
+      t.val[ty][tx] = in.val[j];
+      t.dv[ty][tx] = in.dv[j];
+    } else {
+      t.valid[ty][tx] = 0;
+      t.val[ty][tx] = 0;
+      t.dv[ty][tx] = make_float4(0.f, 1.f, 0.f, 1.f);
+    }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_fill_holes(DepthGeom g, HypBuf in, HypBuf out, const float* __restrict__ maxg,
                                                     lsdg_params P) {
+  __shared__ HaloTile t;
   const int w = g.w, h = g.h;
-  int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  load_halo(t, in, w, h);
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
   if (x >= w || y >= h) return;
-  int idx = x + y * w;
-  Hyp d = load_hyp(in, idx);
+  const int idx = x + y * w;
+  const int cx = lx + 2, cy = ly + 2;  // this pixel in the tile
+  Hyp d;
+  d.valid = t.valid[cy][cx] != 0;
+  d.bl = in.bl[idx];
+  d.nsid = in.nsid[idx];
+  d.val = t.val[cy][cx];
+  d.dv = t.dv[cy][cx];
   if (x >= 3 && x < w - 2 && y >= 3 && y < h - 2 && !d.valid && !(maxg[idx] < P.min_abs_grad_decrease)) {
     // 5x5 box sum of the validity of valid pixels (the reference reads it off
     // an integral image; the integer sum is the same)
     int val = 0;
-    for (int yy = y - 2; yy <= y + 2; ++yy)
-      for (int xx = x - 2; xx <= x + 2; ++xx) {
-        int j = xx + yy * w;
-        if (in.valid[j]) val += in.val[j];
-      }
+    for (int yy = -2; yy <= 2; ++yy)
+      for (int xx = -2; xx <= 2; ++xx)
+        if (t.valid[cy + yy][cx + xx]) val += t.val[cy + yy][cx + xx];
     if ((d.bl >= P.min_blacklist && val > P.val_sum_min_for_create) || val > P.val_sum_min_for_unblacklist) {
       float sumId = 0.0f, sumIv = 0.0f;
-      for (int yy = y - 2; yy <= y + 2; ++yy)
-        for (int xx = x - 2; xx <= x + 2; ++xx) {
-          int j = xx + yy * w;
-          if (!in.valid[j]) continue;
-          float4 s = in.dv[j];
-          sumId += s.x / s.y;
-          sumIv += 1.0f / s.y;
+      for (int yy = -2; yy <= 2; ++yy)
+        for (int xx = -2; xx <= 2; ++xx) {
+          if (!t.valid[cy + yy][cx + xx]) continue;
+          const float4 sv = t.dv[cy + yy][cx + xx];
+          sumId += sv.x / sv.y;
+          sumIv += 1.0f / sv.y;
         }
       float obs = sumId / sumIv;
       obs = unzero(obs);
@@ -523,31 +559,38 @@ __global__ __launch_bounds__(256) void k_fill_holes(DepthGeom g, HypBuf in, HypB
 // ---------------------------------------------------------- regularize
 __global__ __launch_bounds__(256) void k_regularize(DepthGeom g, HypBuf in, HypBuf out, lsdg_params P, int removeOcc,
                                                     int validityTH) {
+  __shared__ HaloTile t;
   const int w = g.w, h = g.h;
-  int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  load_halo(t, in, w, h);
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
   if (x >= w || y >= h) return;
-  int idx = x + y * w;
-  Hyp d = load_hyp(in, idx);
+  const int idx = x + y * w;
+  const int cx = lx + 2, cy = ly + 2;
+  Hyp d;
+  d.valid = t.valid[cy][cx] != 0;
+  d.bl = in.bl[idx];
+  d.nsid = in.nsid[idx];
+  d.val = t.val[cy][cx];
+  d.dv = t.dv[cy][cx];
   if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2 && d.valid) {
     const float regDistVar = 0.075f * 0.075f * P.depth_smoothing_factor * P.depth_smoothing_factor;
     float sum = 0.0f, val_sum = 0.0f, sumIvar = 0.0f;
     int nOcc = 0, nNot = 0;
     for (int dx = -2; dx <= 2; ++dx)
       for (int dy = -2; dy <= 2; ++dy) {
-        int j = idx + dx + dy * w;
-        if (!in.valid[j]) continue;
-        float4 s = in.dv[j];
-        float diff = s.x - d.dv.x;
-        if (P.diff_fac_smoothing * diff * diff > s.y + d.dv.y) {
-          if (removeOcc && s.x > d.dv.x) nOcc++;
+        if (!t.valid[cy + dy][cx + dx]) continue;
+        const float4 sv = t.dv[cy + dy][cx + dx];
+        float diff = sv.x - d.dv.x;
+        if (P.diff_fac_smoothing * diff * diff > sv.y + d.dv.y) {
+          if (removeOcc && sv.x > d.dv.x) nOcc++;
           continue;
         }
-        val_sum += (float)in.val[j];
+        val_sum += (float)t.val[cy + dy][cx + dx];
         if (removeOcc) nNot++;
         float distFac = (float)(dx * dx + dy * dy) * regDistVar;
-        float ivar = 1.0f / (s.y + distFac);
-        sum += s.x * ivar;
+        float ivar = 1.0f / (sv.y + distFac);
+        sum += sv.x * ivar;
         sumIvar += ivar;
       }
     if (val_sum < (float)validityTH) {
