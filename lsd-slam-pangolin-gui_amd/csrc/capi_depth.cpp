@@ -63,7 +63,8 @@ extern "C" lsdg_status lsdg_depthmap_create(lsdg_ctx c, const lsdg_camera* cam, 
   }
   size_t o_obs = take(n), o_dest = take(n * 4), o_iv = take(n * 8), o_cnt = take((n + 1) * 4),
          o_fill = take(n * 4), o_bucket = take(n * 4), o_blk = take((nb + 1) * 4), o_red = take(2 * 1024 * 8),
-         o_rc = take(64);
+         o_rc = take(64), o_work = take(obs_work_slots(g.w, g.h) * 16),
+         o_wc = take(OBS_COUNT_BYTES);
   hipError_t e = hipMalloc(&dm->d_pool, total);
   if (e != hipSuccess) {
     free_dm(dm);
@@ -86,6 +87,8 @@ extern "C" lsdg_status lsdg_depthmap_create(lsdg_ctx c, const lsdg_camera* cam, 
   dm->d_scanblk = (int*)(b + o_blk);
   dm->d_red = (double*)(b + o_red);
   dm->d_red_counter = (unsigned*)(b + o_rc);
+  dm->d_work = (int4*)(b + o_work);
+  dm->d_work_count = (int*)(b + o_wc);
   if ((e = hipMemsetAsync(dm->d_pool, 0, total, c->stream)) != hipSuccess ||
       (e = hipHostMalloc(&dm->h_red, 64, hipHostMallocMapped)) != hipSuccess ||
       (e = hipHostGetDevicePointer((void**)&dm->d_red_out, dm->h_red, 0)) != hipSuccess ||
@@ -251,13 +254,11 @@ extern "C" lsdg_status lsdg_depth_update(lsdg_depthmap dm, const lsdg_frame* ref
     r.id = f->id;
   }
   for (size_t k = 0; k < byId.size(); ++k) dm->h_refById[k] = byId[k];
-  CK(c, hipMemcpyAsync(dm->d_refs, dm->h_refs, sizeof(StereoRef) * n, hipMemcpyHostToDevice, c->stream));
-  CK(c, hipMemcpyAsync(dm->d_refById, dm->h_refById, sizeof(int) * byId.size(), hipMemcpyHostToDevice, c->stream));
   const lsdg_params& P = c->P;
   st = frame_require(kf, KF_DATA);
   if (st != LSDG_OK) return st;
   ProfMark pm = prof_begin(c, LSDG_KERNEL_DEPTH);
-  CK(c, launch_observe(dm, P, dm->d_refs, dm->d_refById, (int)byId.size(), offset, 0, n - 1,
+  CK(c, launch_observe(dm, P, dm->h_refs, n, dm->h_refById, (int)byId.size(), offset, 0, n - 1,
                        (float)kf->numFramesTrackedOnThis, (float)kf->numMappedOnThis, c->stream));
   CK(c, launch_fill_holes(dm, P, c->stream));
   CK(c, launch_regularize(dm, P, false, P.val_sum_min_for_keep, c->stream));

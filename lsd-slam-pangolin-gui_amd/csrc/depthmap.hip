@@ -1,8 +1,10 @@
 // DepthMap (semi-dense inverse-depth filter) on gfx950, one thread per pixel.
 //
-//   k_observe      DepthMap::observeDepthRow -> observeDepthCreate / Update ->
+//   k_observe_*    DepthMap::observeDepthRow -> observeDepthCreate / Update ->
 //                  makeAndCheckEPL -> doLineStereo, in place (each pixel only
 //                  touches its own hypothesis); writes a per-pixel outcome code.
+//                  A gate pass compacts the pixels that search an epipolar
+//                  line into a work list; a stereo pass runs one thread per item.
 //   k_fill_holes   regularizeDepthMapFillHoles; the reference's integral image
 //                  of validity is replaced by the equal integer 5x5 box sum.
 //   k_regularize   regularizeDepthMap(removeOcclusions, validityTH), 5x5 gather.
@@ -24,11 +26,13 @@
 
 namespace lsdg {
 
+typedef __attribute__((address_space(1))) const float GlobalFloat;
+
 __device__ inline float dinterp(const float* __restrict__ m, float x, float y, int w) {
   int ix = (int)x, iy = (int)y;
   float dx = x - (float)ix, dy = y - (float)iy;
   float dxdy = dx * dy;
-  const float* bp = m + ix + iy * w;
+  GlobalFloat* bp = (GlobalFloat*)(m + ix + iy * w);
   return dxdy * bp[1 + w] + (dy - dxdy) * bp[w] + (dx - dxdy) * bp[1] + (1.0f - dx - dy + dxdy) * bp[0];
 }
 
@@ -42,6 +46,41 @@ __device__ inline float2 dinterp42(const float4* __restrict__ g, float x, float 
   float w11 = dxdy, w01 = dy - dxdy, w10 = dx - dxdy, w00 = 1.0f - dx - dy + dxdy;
   return make_float2(w11 * a.x + w01 * b.x + w10 * c.x + w00 * d.x, w11 * a.y + w01 * b.y + w10 * c.y + w00 * d.y);
 }
+
+// The four texels dinterp reads at (x, y), as two 8-byte loads; the index is
+// clamped to the image so that positions outside it (never blended) stay in
+// bounds, and is unchanged for every position dinterp may read.
+// The image pointer reaches the kernel through device memory (a reference
+// frame's descriptor), so it is cast to the global address space: generic
+// (flat) loads would make every wait drain all outstanding loads.
+typedef float Texel2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef __attribute__((address_space(1))) const Texel2* GlobalTexel2;
+struct Quad {
+  Texel2 t, b;  // (I(ix, iy), I(ix + 1, iy)), (I(ix, iy + 1), I(ix + 1, iy + 1))
+};
+__device__ inline Quad fetch_quad(const float* __restrict__ m, float x, float y, int w, int h) {
+  int ix = (int)x, iy = (int)y;
+  ix = min(max(ix, 0), w - 2);
+  iy = min(max(iy, 0), h - 2);
+  const float* bp = m + ix + iy * w;
+  Quad q;
+  q.t = *(GlobalTexel2)bp;
+  q.b = *(GlobalTexel2)(bp + w);
+  return q;
+}
+// dinterp's bilinear blend of fetched texels (same expression and order).
+__device__ inline float blend_quad(const Quad& q, float x, float y) {
+  int ix = (int)x, iy = (int)y;
+  float dx = x - (float)ix, dy = y - (float)iy;
+  float dxdy = dx * dy;
+  return dxdy * q.b.y + (dy - dxdy) * q.b.x + (dx - dxdy) * q.t.y + (1.0f - dx - dy + dxdy) * q.t.x;
+}
+
+#ifndef LSDG_STEREO_AHEAD
+#define LSDG_STEREO_AHEAD 8
+#endif
+constexpr int STEREO_AHEAD = LSDG_STEREO_AHEAD;
+static_assert(STEREO_AHEAD % 2 == 0, "the search loop reads its step parity from the unrolled index");
 
 __device__ inline float unzero(float v) {
   return v < 0.0f ? (v > -1e-10f ? -1e-10f : v) : (v < 1e-10f ? 1e-10f : v);
@@ -223,11 +262,34 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
   float e1A = NAN, e1B = NAN, e2A = NAN, e2B = NAN, e3A = NAN, e3B = NAN, e4A = NAN, e4B = NAN, e5A = NAN, e5B = NAN;
   int loopBest = -1, loopSecond = -1;
   const int maxSteps = P.stereo_max_steps;
-  while ((((incx < 0.0f) == (cpx > pClose[0]) && (incy < 0.0f) == (cpy > pClose[1])) || loop == 0) &&
-         loop < maxSteps) {
-    val_p2 = dinterp(refImg, cpx + 2.0f * incx, cpy + 2.0f * incy, width);
+  // The texels of the next STEREO_AHEAD steps' samples are in flight while a
+  // step is evaluated: a lead copy of the position chain (the same float
+  // additions, hence the same positions) fetches each sample STEREO_AHEAD
+  // steps early.  Lead fetches past the line's end are clamped to the image
+  // and never used.
+  float lx = cpx, ly = cpy;
+  Quad ring[STEREO_AHEAD];
+#pragma unroll
+  for (int k = 0; k < STEREO_AHEAD; ++k) {
+    ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+    lx += incx;
+    ly += incy;
+  }
+  // The while condition of the reference loop, without short-circuit
+  // branches; loop parity equals k's (STEREO_AHEAD is even).
+  const bool negx = incx < 0.0f, negy = incy < 0.0f;
+  const float closeX = pClose[0], closeY = pClose[1];
+  for (;;) {
+#pragma unroll
+  for (int k = 0; k < STEREO_AHEAD; ++k) {
+    const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
+    if (!(go & (loop < maxSteps))) goto searched;
+    val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
+    ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+    lx += incx;
+    ly += incy;
     float ee = 0.0f;
-    if (loop % 2 == 0) {
+    if ((k & 1) == 0) {
       e1A = val_p2 - rv_p2; ee += e1A * e1A;
       e2A = val_p1 - rv_p1; ee += e2A * e2A;
       e3A = val - rv; ee += e3A * e3A;
@@ -272,6 +334,8 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
     cpy += incy;
     loop++;
   }
+  }
+searched:
   if (best_err > 4.0f * P.max_error_stereo) return -3.0f;
   if ((float)abs(loopBest - loopSecond) > 1.0f && P.min_distance_error_stereo * best_err > second_err) return -2.0f;
 
@@ -341,6 +405,11 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
   return best_err;
 }
 
+// Reference frames of one update: in the kernel arguments when they fit
+// (the common case of a few frames), else in device memory.
+constexpr int OBS_INLINE_REFS = 4;
+constexpr int OBS_INLINE_BYID = 32;
+
 struct ObsArgs {
   DepthGeom g;
   HypBuf cur;
@@ -348,8 +417,10 @@ struct ObsArgs {
   const float* kfImg;
   const float* kfMaxg;
   const float4* kfGrad;
-  const StereoRef* refs;
+  const StereoRef* refs;  // device copies when not inline
   const int* refById;
+  StereoRef inlRefs[OBS_INLINE_REFS];
+  int inlById[OBS_INLINE_BYID];
   int nRefById;
   int refByIdOffset;
   int oldestIdx, newestIdx;
@@ -358,80 +429,161 @@ struct ObsArgs {
   float numTracked, numMapped;
 };
 
+template <bool INL>
+__device__ inline const StereoRef& obs_ref(const ObsArgs& a, int i) {
+  if constexpr (INL) return a.inlRefs[i];
+  else return a.refs[i];
+}
+template <bool INL>
+__device__ inline int obs_by_id(const ObsArgs& a, int k) {
+  if constexpr (INL) return a.inlById[k];
+  else return a.refById[k];
+}
+
 __device__ inline bool was_good_gate(const StereoRef& r, int x, int y, int w) {
-  if (r.wasGood && !r.wasGood[(x >> 1) + (w >> 1) * (y >> 1)]) return false;
+  typedef __attribute__((address_space(1))) const uint8_t GlobalByte;
+  if (r.wasGood && !((GlobalByte*)r.wasGood)[(x >> 1) + (w >> 1) * (y >> 1)]) return false;
   return true;
 }
 
-__global__ __launch_bounds__(256) void k_observe(ObsArgs a, lsdg_params P) {
+// observeDepthRow in two passes.  The gate pass runs the per-pixel tests of
+// observeDepth / observeDepthCreate / observeDepthUpdate up to and including
+// makeAndCheckEPL and writes the outcome of every pixel that stops there;
+// pixels that go on to doLineStereo are appended to a work list.  The stereo
+// pass runs one thread per work item, so waves hold only pixels that search
+// an epipolar line, and creates (long lines over the full inverse-depth
+// range) are not mixed with updates (short lines around the prior).  Each
+// pixel still reads and writes only its own hypothesis, so the item order
+// does not matter.
+//
+// Appends are aggregated per block, and the blocks are spread over OBS_SEGS
+// list segments with one counter pair each (gate block b appends to segment
+// b % OBS_SEGS): same-address device atomics serialise, so one counter for
+// the whole grid would cost more than the search itself.  Segment s holds
+// creates from its front and updates from its back; its capacity is the
+// pixel count of its blocks, so it cannot overflow.
+constexpr int OBS_SEGS = 16;
+constexpr int OBS_COUNTER_STRIDE = 32;  // ints between counters: one 128-byte line each
+
+__device__ inline int lane_rank(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// One thread per pixel of the whole map (pixels outside [3, w-3) x [3, h-3)
+// get LSDG_OBS_NONE).  Appends go to the counter set `count`; the other set,
+// `other`, is cleared for the next update (the two alternate, so no clearing
+// pass runs between updates).
+template <bool INL>
+__global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, int4* __restrict__ work,
+                                                      int* __restrict__ count, int* __restrict__ other, int segcap) {
   const int w = a.g.w, h = a.g.h;
-  int x = 3 + blockIdx.x * 16 + (threadIdx.x & 15);
-  int y = 3 + blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= w - 3 || y >= h - 3) return;
-  int idx = x + y * w;
-  ObsCtx o{a.g, a.kfImg, a.kfGrad};
-  Hyp t = load_hyp(a.cur, idx);
-  float mg = a.kfMaxg[idx];
-  uint8_t code = LSDG_OBS_NONE;
-  bool has = t.valid;
-  if (has && mg < P.min_abs_grad_decrease) {
-    a.cur.valid[idx] = 0;
-    a.obs[idx] = LSDG_OBS_LOWGRAD_INVALIDATE;
-    return;
-  }
-  if (mg < P.min_abs_grad_create || t.bl < P.min_blacklist) {
-    a.obs[idx] = LSDG_OBS_NONE;
-    return;
-  }
-  if (!has) {
-    // observeDepthCreate
-    const StereoRef& ref = a.refs[a.kfReactivated ? a.newestIdx : a.oldestIdx];
-    float epx, epy, rid, rvar, repl;
-    if (!was_good_gate(ref, x, y, w)) {
-      code = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
-    } else if (!make_and_check_epl(o, P, x, y, ref, &epx, &epy)) {
-      code = LSDG_OBS_EPL_FAIL;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2 * OBS_SEGS)
+    other[threadIdx.x * OBS_COUNTER_STRIDE] = 0;
+  int kind = 0;  // 1 observeDepthCreate search, 2 observeDepthUpdate search
+  int idx = 0, ri = 0;
+  float epx = 0.0f, epy = 0.0f;
+  if (x < w && y < h && (x < 3 || x >= w - 3 || y < 3 || y >= h - 3)) {
+    a.obs[x + y * w] = LSDG_OBS_NONE;
+  } else if (x < w && y < h) {
+    idx = x + y * w;
+    ObsCtx o{a.g, a.kfImg, a.kfGrad};
+    const bool has = a.cur.valid[idx] != 0;
+    const int bl = a.cur.bl[idx];
+    const float mg = a.kfMaxg[idx];
+    uint8_t code = LSDG_OBS_NONE;
+    if (has && mg < P.min_abs_grad_decrease) {
+      a.cur.valid[idx] = 0;
+      code = LSDG_OBS_LOWGRAD_INVALIDATE;
+    } else if (mg < P.min_abs_grad_create || bl < P.min_blacklist) {
+      code = LSDG_OBS_NONE;
     } else {
-      float err = line_stereo(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
-      if (err == -3.0f || err == -2.0f) {
-        t.bl--;
-        a.cur.bl[idx] = t.bl;
-      }
-      if (err < 0.0f) {
-        code = (uint8_t)(LSDG_OBS_STEREO_FAIL_BASE - (int)err);
-      } else if (rvar > P.max_var) {
-        code = LSDG_OBS_CREATE_VAR_TOO_BIG;
+      bool go = true;
+      if (!has) {
+        ri = a.kfReactivated ? a.newestIdx : a.oldestIdx;
+      } else if (!a.kfReactivated) {
+        const int k = (int)a.cur.nsid[idx] - a.refByIdOffset;
+        if (k >= a.nRefById) {
+          code = LSDG_OBS_SKIP_ALREADY_GOOD;
+          go = false;
+        } else {
+          ri = k < 0 ? a.oldestIdx : obs_by_id<INL>(a, k);
+        }
       } else {
-        rid = unzero(rid);
-        store_hyp(a.cur, idx, make_hyp(rid, rvar, P.validity_counter_initial_observe));
-        code = LSDG_OBS_CREATED;
+        ri = a.newestIdx;
       }
+      if (go) {
+        const StereoRef& ref = obs_ref<INL>(a, ri);
+        if (!was_good_gate(ref, x, y, w)) code = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
+        else if (!make_and_check_epl(o, P, x, y, ref, &epx, &epy)) code = LSDG_OBS_EPL_FAIL;
+        else kind = has ? 2 : 1;
+      }
+    }
+    if (kind == 0) a.obs[idx] = code;
+  }
+  // block-wide ranks: per-wave ballots, then one atomic pair per block
+  __shared__ int wcount[4][2], base[2];
+  const int wave = threadIdx.x >> 6;
+  const unsigned long long mc = __ballot(kind == 1), mu = __ballot(kind == 2);
+  if (__lane_id() == 0) {
+    wcount[wave][0] = __popcll(mc);
+    wcount[wave][1] = __popcll(mu);
+  }
+  __syncthreads();
+  const int seg = (blockIdx.x + blockIdx.y * gridDim.x) % OBS_SEGS;
+  if (threadIdx.x < 2) {
+    const int k = threadIdx.x;
+    const int tot = wcount[0][k] + wcount[1][k] + wcount[2][k] + wcount[3][k];
+    base[k] = tot ? atomicAdd(&count[(2 * seg + k) * OBS_COUNTER_STRIDE], tot) : 0;
+  }
+  __syncthreads();
+  if (kind == 0) return;
+  int r = kind == 1 ? lane_rank(mc) : lane_rank(mu);
+  for (int v = 0; v < wave; ++v) r += wcount[v][kind - 1];
+  const int4 item = make_int4(idx, ri, __float_as_int(epx), __float_as_int(epy));
+  int4* segw = work + (size_t)seg * segcap;
+  if (kind == 1) segw[base[0] + r] = item;
+  else segw[segcap - 1 - (base[1] + r)] = item;
+}
+
+// Grid: OBS_SEGS x (segcap / 256) blocks, blockIdx.y the segment.
+template <bool INL>
+__global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
+                                                        const int* __restrict__ count, int segcap) {
+  const int seg = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int nc = count[(2 * seg) * OBS_COUNTER_STRIDE], nu = count[(2 * seg + 1) * OBS_COUNTER_STRIDE];
+  bool create;
+  if (i < nc) create = true;
+  else if (i >= segcap - nu && i < segcap) create = false;
+  else return;
+  const int4 item = work[(size_t)seg * segcap + i];
+  const int w = a.g.w;
+  const int idx = item.x, x = idx % w, y = idx / w;
+  const float epx = __int_as_float(item.z), epy = __int_as_float(item.w);
+  const StereoRef& ref = obs_ref<INL>(a, item.y);
+  ObsCtx o{a.g, a.kfImg, a.kfGrad};
+  uint8_t code;
+  if (create) {
+    // observeDepthCreate
+    float rid, rvar, repl;
+    float err = line_stereo(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
+    if (err == -3.0f || err == -2.0f) a.cur.bl[idx] = a.cur.bl[idx] - 1;
+    if (err < 0.0f) {
+      code = (uint8_t)(LSDG_OBS_STEREO_FAIL_BASE - (int)err);
+    } else if (rvar > P.max_var) {
+      code = LSDG_OBS_CREATE_VAR_TOO_BIG;
+    } else {
+      rid = unzero(rid);
+      store_hyp(a.cur, idx, make_hyp(rid, rvar, P.validity_counter_initial_observe));
+      code = LSDG_OBS_CREATED;
     }
     a.obs[idx] = code;
     return;
   }
   // observeDepthUpdate
-  int ri;
-  if (!a.kfReactivated) {
-    int k = (int)t.nsid - a.refByIdOffset;
-    if (k >= a.nRefById) {
-      a.obs[idx] = LSDG_OBS_SKIP_ALREADY_GOOD;
-      return;
-    }
-    ri = k < 0 ? a.oldestIdx : a.refById[k];
-  } else {
-    ri = a.newestIdx;
-  }
-  const StereoRef& ref = a.refs[ri];
-  if (!was_good_gate(ref, x, y, w)) {
-    a.obs[idx] = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
-    return;
-  }
-  float epx, epy;
-  if (!make_and_check_epl(o, P, x, y, ref, &epx, &epy)) {
-    a.obs[idx] = LSDG_OBS_EPL_FAIL;
-    return;
-  }
+  Hyp t = load_hyp(a.cur, idx);
   float sv = sqrtf(t.dv.w);
   float min_id = t.dv.z - sv * P.stereo_epl_var_fac;
   float max_id = t.dv.z + sv * P.stereo_epl_var_fac;
@@ -463,6 +615,7 @@ __global__ __launch_bounds__(256) void k_observe(ObsArgs a, lsdg_params P) {
     store_hyp(a.cur, idx, t);
     code = LSDG_OBS_INCONSISTENT;
   } else {
+    const float mg = a.kfMaxg[idx];
     float id_var = t.dv.y * P.succ_var_inc_fac;
     float wgt = rvar / (rvar + id_var);
     float new_id = (1.0f - wgt) * rid + wgt * t.dv.x;
@@ -470,8 +623,8 @@ __global__ __launch_bounds__(256) void k_observe(ObsArgs a, lsdg_params P) {
     id_var = id_var * wgt;
     if (id_var < t.dv.y) t.dv.y = id_var;
     t.val += P.validity_counter_inc;
-    float cap = (float)P.validity_counter_max + mg * (float)P.validity_counter_max_variable / 255.0f;
-    if ((float)t.val > cap) t.val = (int)cap;
+    float cap_v = (float)P.validity_counter_max + mg * (float)P.validity_counter_max_variable / 255.0f;
+    if ((float)t.val > cap_v) t.val = (int)cap_v;
     if (repl < P.min_epl_length_crop) {
       float inc = a.numTracked / (a.numMapped + 5.0f);
       if (inc < 3.0f) inc = 3.0f;
@@ -874,10 +1027,17 @@ __global__ void k_hyp_prior(HypBuf cur, int n, const float* __restrict__ id, con
 // ------------------------------------------------------------ launchers
 static inline dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }
 
-hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* d_refs, const int* d_refById,
-                          int nRefById, int refByIdOffset, int oldestIdx, int newestIdx, float numTracked,
-                          float numMapped, hipStream_t s) {
-  ObsArgs a;
+static_assert(OBS_COUNT_BYTES >= 2 * 2 * OBS_SEGS * OBS_COUNTER_STRIDE * sizeof(int), "observe counter sets");
+
+size_t obs_work_slots(int w, int h) {
+  const size_t bx = (size_t)(w + 15) / 16 + 1, by = (size_t)(h + 15) / 16 + 1;
+  return ((bx * by + OBS_SEGS - 1) / OBS_SEGS) * OBS_SEGS * 256;
+}
+
+hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* h_refs, int nRefs,
+                          const int* h_refById, int nRefById, int refByIdOffset, int oldestIdx, int newestIdx,
+                          float numTracked, float numMapped, hipStream_t s) {
+  ObsArgs a{};
   a.g = dm->g;
   a.cur = dm->buf[dm->cur];
   a.obs = dm->d_obs;
@@ -885,8 +1045,6 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
   a.kfImg = kf->d_image[0];
   a.kfMaxg = kf->d_maxg[0];
   a.kfGrad = kf->d_grad[0];
-  a.refs = d_refs;
-  a.refById = d_refById;
   a.nRefById = nRefById;
   a.refByIdOffset = refByIdOffset;
   a.oldestIdx = oldestIdx;
@@ -895,10 +1053,36 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
   a.kfId = kf->id;
   a.numTracked = numTracked;
   a.numMapped = numMapped;
-  (void)hipMemsetAsync(dm->d_obs, 0, (size_t)dm->g.w * dm->g.h, s);
-  int iw = dm->g.w - 6, ih = dm->g.h - 6;
-  if (iw <= 0 || ih <= 0) return hipSuccess;
-  k_observe<<<grid16(iw, ih), dim3(256), 0, s>>>(a, P);
+  const bool inl = nRefs <= OBS_INLINE_REFS && nRefById <= OBS_INLINE_BYID;
+  hipError_t e = hipSuccess;
+  if (inl) {
+    for (int i = 0; i < nRefs; ++i) a.inlRefs[i] = h_refs[i];
+    for (int k = 0; k < nRefById; ++k) a.inlById[k] = h_refById[k];
+  } else {
+    // h_refs / h_refById are pinned staging buffers the caller keeps until the stream drains
+    e = hipMemcpyAsync(dm->d_refs, h_refs, sizeof(StereoRef) * nRefs, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dm->d_refById, h_refById, sizeof(int) * nRefById, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    a.refs = dm->d_refs;
+    a.refById = dm->d_refById;
+  }
+  const dim3 gg = grid16(dm->g.w, dm->g.h);
+  // a segment takes every OBS_SEGS-th gate block: 256 slots per block
+  const int segBlocks = (int)((gg.x * gg.y + OBS_SEGS - 1) / OBS_SEGS);
+  const int segcap = segBlocks * 256;
+  if ((size_t)segcap * OBS_SEGS > obs_work_slots(dm->g.w, dm->g.h)) return hipErrorInvalidValue;
+  const int set = 2 * OBS_SEGS * OBS_COUNTER_STRIDE;
+  int* count = dm->d_work_count + dm->obsParity * set;
+  int* other = dm->d_work_count + (1 - dm->obsParity) * set;
+  dm->obsParity ^= 1;
+  if (inl) {
+    k_observe_gate<true><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
+    k_observe_stereo<true><<<dim3(segBlocks, OBS_SEGS), dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
+  } else {
+    k_observe_gate<false><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
+    k_observe_stereo<false><<<dim3(segBlocks, OBS_SEGS), dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
+  }
   return hipGetLastError();
 }
 

@@ -127,21 +127,6 @@ struct PointsLevel {
   int n;
 };
 
-// depthmap.hip
-hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* d_refs, const int* d_refById,
-                          int nRefById, int refByIdOffset, int oldestIdx, int newestIdx, float numTracked,
-                          float numMapped, hipStream_t s);
-hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s);  // swaps buffers
-hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th,
-                             hipStream_t s);  // swaps buffers
-hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s);
-hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame nkf, const Xf& oldToNew,
-                            const uint8_t* d_wasGood, hipStream_t s);  // swaps buffers
-hipError_t launch_mean_idepth(lsdg_depthmap dm, hipStream_t s);  // h_red[0] = count, h_red[1] = sum
-hipError_t launch_rescale(lsdg_depthmap dm, float f, hipStream_t s);
-hipError_t launch_hyp_prior(lsdg_depthmap dm, const float* d_id, const float* d_var, int validity, int gt,
-                            float gtVar, hipStream_t s);
-
 }  // namespace lsdg
 
 // ------------------------------------------------------------ C-ABI objects
@@ -229,6 +214,9 @@ struct lsdg_depthmap_s {
   int refs_cap = 0;
   int* d_refById = nullptr;
   int refById_cap = 0;
+  int4* d_work = nullptr;    // epipolar-search work items, one slot per pixel
+  int* d_work_count = nullptr;  // two sets of per-segment (create, update) item counters
+  int obsParity = 0;            // counter set of the next observe pass
   lsdg::StereoRef* h_refs = nullptr;  // pinned staging
   int* h_refById = nullptr;
   // propagation scratch
@@ -269,9 +257,15 @@ hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, 
                               const TrackKParams& kp, float* d_grad_d, double* d_out);
 
 // depthmap.hip
-hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* d_refs, const int* d_refById,
-                          int nRefById, int refByIdOffset, int oldestIdx, int newestIdx, float numTracked,
-                          float numMapped, hipStream_t s);
+// Observe work list: slots (int4 each) for a w x h depth map, and the bytes of
+// its two alternating sets of segment counters.
+constexpr size_t OBS_COUNT_BYTES = 8192;
+size_t obs_work_slots(int w, int h);
+// h_refs / h_refById: host arrays (pinned: copied asynchronously when they do
+// not fit in the kernel arguments).
+hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* h_refs, int nRefs,
+                          const int* h_refById, int nRefById, int refByIdOffset, int oldestIdx, int newestIdx,
+                          float numTracked, float numMapped, hipStream_t s);
 hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s);  // swaps buffers
 hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th,
                              hipStream_t s);  // swaps buffers

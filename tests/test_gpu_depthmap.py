@@ -40,13 +40,13 @@ def _diff(a, b):
     return out
 
 
-def _scenario(sys_, d, masks=None, n_refs=1):
-    """prior on the keyframe -> update with reference frame(s) -> createKeyFrame."""
+def _scenario(sys_, d, masks=None, n_refs=1, ids=(1, 3, 4), updates=1):
+    """prior on the keyframe -> update(s) with reference frame(s) -> createKeyFrame."""
     cam = d["cam"]
     f32 = (lambda a: a.astype(np.float32)) if isinstance(sys_, pyoracle.Oracle) else (lambda a: a)
     kf = sys_.frame(0, cam, f32(d["kf_img"]))
     refs = []
-    ids = [1, 3, 4][:n_refs]
+    ids = list(ids)[:n_refs]
     for k, fid in enumerate(ids):
         img = d["ref_img"] if k == 0 else d["kf2_img"]
         pose = d["ref_pose"] if k == 0 else d["kf2_pose"]
@@ -58,7 +58,8 @@ def _scenario(sys_, d, masks=None, n_refs=1):
     dm = sys_.depthmap(cam)
     dm.set_prior(kf, d["prior_idepth"], d["prior_var"], 20)
     kf.set_counters(7, 2)
-    dm.update(refs)
+    for u in range(updates):
+        dm.update(refs[u % len(refs):] if updates > 1 else refs)
     h1 = dm.read()
     kfd = (kf.read(lsdg.WHAT_IDEPTH, 0), kf.read(lsdg.WHAT_IDEPTH_VAR, 2))
     dm.create_keyframe(kf2)
@@ -105,23 +106,41 @@ def test_update_and_create_keyframe_bit_exact(ctx, depth_inst, crop):
     assert abs(g[4][1][7] - o0[4][1][7]) <= 1e-4 * abs(o0[4][1][7])
 
 
-def test_update_with_tracking_masks_and_several_refs(ctx, depth_inst):
+@pytest.mark.parametrize("ids", [(1, 3, 4), (1, 40, 41)], ids=["inline", "byid-in-device-memory"])
+def test_update_with_tracking_masks_and_several_refs(ctx, depth_inst, ids):
     """refPixelWasGood gates (from a seeded mask), three reference frames with
-    id gaps (referenceFrameByID), and the propagation gate on the new keyframe's mask."""
+    id gaps (referenceFrameByID), and the propagation gate on the new keyframe's
+    mask.  The second id set spans more frame ids than the kernel arguments
+    hold, so the reference table goes through device memory."""
     w1, h1 = 320, 240
     rng = np.random.default_rng(5)
     masks = [(rng.random((h1, w1)) > 0.1).astype(np.uint8) for _ in range(3)]
     p = _params(100)
     ctx.set_params(p)
     try:
-        g = _scenario(ctx, depth_inst, masks=masks, n_refs=3)
+        g = _scenario(ctx, depth_inst, masks=masks, n_refs=3, ids=ids)
     finally:
         ctx.set_params(lsdg.default_params())
-    o2 = _scenario(pyoracle.Oracle(2, params=p), depth_inst, masks=masks, n_refs=3)
-    o0 = _scenario(pyoracle.Oracle(0, params=p), depth_inst, masks=masks, n_refs=3)
+    o2 = _scenario(pyoracle.Oracle(2, params=p), depth_inst, masks=masks, n_refs=3, ids=ids)
+    o0 = _scenario(pyoracle.Oracle(0, params=p), depth_inst, masks=masks, n_refs=3, ids=ids)
     assert not _diff(g[0], o0[0]), _diff(g[0], o0[0])
     assert np.bincount(g[0]["observe_code"], minlength=15)[lsdg.OBS["SKIP_NOT_GOOD_TRACKED"]] > 100
     assert not _diff(g[1], o2[1]), _diff(g[1], o2[1])
+
+
+def test_consecutive_updates_bit_exact(ctx, depth_inst):
+    """Three updates in a row on one depth map (the work-list counters
+    alternate between calls), with the reference set shrinking each time."""
+    p = _params(100)
+    ctx.set_params(p)
+    try:
+        g = _scenario(ctx, depth_inst, n_refs=3, updates=3)
+    finally:
+        ctx.set_params(lsdg.default_params())
+    o0 = _scenario(pyoracle.Oracle(0, params=p), depth_inst, n_refs=3, updates=3)
+    assert not _diff(g[0], o0[0]), _diff(g[0], o0[0])
+    codes = np.bincount(g[0]["observe_code"], minlength=15)
+    assert codes[lsdg.OBS["UPDATED"]] > 1000
 
 
 def test_initialize_from_gt_and_finalize(ctx, depth_inst):
