@@ -59,10 +59,11 @@ struct Quad {
   Texel2 t, b;  // (I(ix, iy), I(ix + 1, iy)), (I(ix, iy + 1), I(ix + 1, iy + 1))
 };
 __device__ inline Quad fetch_quad(const float* __restrict__ m, float x, float y, int w, int h) {
-  int ix = (int)x, iy = (int)y;
-  ix = min(max(ix, 0), w - 2);
-  iy = min(max(iy, 0), h - 2);
-  const float* bp = m + ix + iy * w;
+  // the linear index, clamped to [0, w (h - 1) - 2]: unchanged for
+  // 0 <= ix <= w - 2, 0 <= iy <= h - 2 (every blended position)
+  const unsigned lin = (unsigned)(int)x + (unsigned)(int)y * (unsigned)w;
+  const unsigned off = (unsigned)min(max((int)lin, 0), w * (h - 1) - 2);
+  const float* bp = m + off;
   Quad q;
   q.t = *(GlobalTexel2)bp;
   q.b = *(GlobalTexel2)(bp + w);
@@ -70,8 +71,8 @@ __device__ inline Quad fetch_quad(const float* __restrict__ m, float x, float y,
 }
 // dinterp's bilinear blend of fetched texels (same expression and order).
 __device__ inline float blend_quad(const Quad& q, float x, float y) {
-  int ix = (int)x, iy = (int)y;
-  float dx = x - (float)ix, dy = y - (float)iy;
+  // truncf(x) == (float)(int)x over the int range
+  float dx = x - truncf(x), dy = y - truncf(y);
   float dxdy = dx * dy;
   return dxdy * q.b.y + (dy - dxdy) * q.b.x + (dx - dxdy) * q.t.y + (1.0f - dx - dy + dxdy) * q.t.x;
 }
@@ -149,10 +150,84 @@ __device__ inline bool make_and_check_epl(const ObsCtx& o, const lsdg_params& P,
   return true;
 }
 
-// doLineStereo: the matching error (>= 0) or a code -1..-4.
+// Outcome of the epipolar search over a set of steps: the smallest error
+// (first step on ties) with its neighbours' errors and error-vector dot
+// products and its position, and the smallest error of the other steps
+// (first step on ties).  Empty entries hold (INFINITY, INT_MAX): the
+// reference only records errors below INFINITY.
+struct Scan {
+  float best, second;
+  int ibest, isecond;
+  float errPre, diffPre, errPost, diffPost;
+  float bx, by;
+};
+
+__device__ inline bool scan_before(float v, int i, float u, int k) { return v < u || (v == u && i < k); }
+
+// Union of two disjoint step sets.  Over a sequence in step order the
+// reference keeps the first minimum as best and, as second, the first minimum
+// of the remaining steps; both are minima in the (error, step) order, so the
+// union's best is the better of the two bests and its second the better of
+// the winner's second and the loser's best.
+__device__ inline Scan scan_merge(const Scan& a, const Scan& b) {
+  const bool tb = scan_before(b.best, b.ibest, a.best, a.ibest);
+  Scan r = tb ? b : a;
+  const Scan& l = tb ? a : b;
+  if (scan_before(l.best, l.ibest, r.second, r.isecond)) {
+    r.second = l.best;
+    r.isecond = l.ibest;
+  }
+  return r;
+}
+
+// Butterfly over the G lanes of a group; every lane ends with the union.
+template <int G>
+__device__ inline Scan scan_group(Scan s) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) {
+    Scan o;
+    o.best = __shfl_xor(s.best, m, G);
+    o.second = __shfl_xor(s.second, m, G);
+    o.ibest = __shfl_xor(s.ibest, m, G);
+    o.isecond = __shfl_xor(s.isecond, m, G);
+    o.errPre = __shfl_xor(s.errPre, m, G);
+    o.diffPre = __shfl_xor(s.diffPre, m, G);
+    o.errPost = __shfl_xor(s.errPost, m, G);
+    o.diffPost = __shfl_xor(s.diffPost, m, G);
+    o.bx = __shfl_xor(s.bx, m, G);
+    o.by = __shfl_xor(s.by, m, G);
+    s = scan_merge(s, o);
+  }
+  return s;
+}
+
+// Advances the position chain cp_{j+1} = cp_j + inc (float additions, one
+// at a time, as the reference's loop does) from step j to step `to`.
+__device__ inline void walk(float& x, float& y, int& j, int to, float ix, float iy) {
+  while (j + 4 <= to) {
+    x += ix;
+    x += ix;
+    x += ix;
+    x += ix;
+    y += iy;
+    y += iy;
+    y += iy;
+    y += iy;
+    j += 4;
+  }
+  while (j < to) {
+    x += ix;
+    y += iy;
+    ++j;
+  }
+}
+
+// doLineStereo: the matching error (>= 0) or a code -1..-4.  Called by the G
+// lanes of a group (lane g) with the same arguments; all return the same.
+template <int G>
 __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, float v, float epxn, float epyn,
                              float min_idepth, float prior_idepth, float max_idepth, const StereoRef& ref,
-                             float& r_idepth, float& r_var, float& r_epl) {
+                             float& r_idepth, float& r_var, float& r_epl, int g) {
   const int width = o.g.w, height = o.g.h;
   const float* kfImg = o.kfImg;
   const float* refImg = ref.img;
@@ -247,95 +322,181 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
     if (pClose[0] <= B || pClose[0] >= Wb || pClose[1] <= B || pClose[1] >= Hb || newLen < 8.0f) return -1.0f;
   }
 
-  float cpx = pFar[0], cpy = pFar[1];
-  float val_m2 = dinterp(refImg, cpx - 2.0f * incx, cpy - 2.0f * incy, width);
-  float val_m1 = dinterp(refImg, cpx - incx, cpy - incy, width);
-  float val = dinterp(refImg, cpx, cpy, width);
-  float val_p1 = dinterp(refImg, cpx + incx, cpy + incy, width);
-  float val_p2;
-  int loop = 0;
-  float best_x = -1.0f, best_y = -1.0f;
-  float best_err = INFINITY, second_err = INFINITY;
-  float best_errPre = NAN, best_errPost = NAN, best_DiffPre = NAN, best_DiffPost = NAN;
-  bool bestWasLast = false;
-  float eeLast = -1.0f;
-  float e1A = NAN, e1B = NAN, e2A = NAN, e2B = NAN, e3A = NAN, e3B = NAN, e4A = NAN, e4B = NAN, e5A = NAN, e5B = NAN;
-  int loopBest = -1, loopSecond = -1;
+  // Epipolar search.  With G > 1 the G lanes of a group split the steps into
+  // chunks (lane g takes [base + g C, base + (g + 1) C) of every round);
+  // each lane runs the reference loop body on its chunk from the state the
+  // sequential loop would have there, and the group merges the lanes'
+  // results (scan_merge).  The positions come from the same chain of float
+  // additions as the sequential loop (walk), so every sample is the same.
+  const float cp0x = pFar[0], cp0y = pFar[1];
+  const float init_m2 = dinterp(refImg, cp0x - 2.0f * incx, cp0y - 2.0f * incy, width);
+  const float init_m1 = dinterp(refImg, cp0x - incx, cp0y - incy, width);
+  const float init_0 = dinterp(refImg, cp0x, cp0y, width);
+  const float init_p1 = dinterp(refImg, cp0x + incx, cp0y + incy, width);
   const int maxSteps = P.stereo_max_steps;
-  // The texels of the next STEREO_AHEAD steps' samples are in flight while a
-  // step is evaluated: a lead copy of the position chain (the same float
-  // additions, hence the same positions) fetches each sample STEREO_AHEAD
-  // steps early.  Lead fetches past the line's end are clamped to the image
-  // and never used.
-  float lx = cpx, ly = cpy;
-  Quad ring[STEREO_AHEAD];
-#pragma unroll
-  for (int k = 0; k < STEREO_AHEAD; ++k) {
-    ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
-    lx += incx;
-    ly += incy;
-  }
-  // The while condition of the reference loop, without short-circuit
-  // branches; loop parity equals k's (STEREO_AHEAD is even).
   const bool negx = incx < 0.0f, negy = incy < 0.0f;
   const float closeX = pClose[0], closeY = pClose[1];
-  for (;;) {
+  int C = maxSteps;
+  if (G > 1) {
+    // chunk length from the step count to pClose (rounds continue if short)
+    const float nx = incx != 0.0f ? (closeX - cp0x) / incx : INFINITY;
+    const float ny = incy != 0.0f ? (closeY - cp0y) / incy : INFINITY;
+    const float n = fminf(fmaxf(fminf(nx, ny), 0.0f), (float)maxSteps);
+    C = max(((int)n + 3 + G - 1) / G, 6);  // >= 6: a lane's next chunk starts past its warm-up
+    C += C & 1;                            // chunks start on even steps: the e-vector parity is k's
+  }
+  Scan sc;
+  sc.best = INFINITY;
+  sc.second = INFINITY;
+  sc.ibest = INT_MAX;
+  sc.isecond = INT_MAX;
+  sc.errPre = NAN;
+  sc.diffPre = NAN;
+  sc.errPost = NAN;
+  sc.diffPost = NAN;
+  sc.bx = -1.0f;
+  sc.by = -1.0f;
+  float wx = cp0x, wy = cp0y;  // walker at cp_{wj}
+  int wj = 0;
+  for (int base = 0;; base += G * C) {
+    const int j0 = base + g * C, j1 = j0 + C;
+    bool lookValid = false;  // step j1 is on the line
+    if (j0 < maxSteps) {
+      // window V_{j0-3} .. V_{j0+1}; V_m for m >= 2 is sampled at cp_{m-2} + 2 inc
+      float W[5];
+      walk(wx, wy, wj, j0 - 5, incx, incy);
 #pragma unroll
-  for (int k = 0; k < STEREO_AHEAD; ++k) {
-    const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
-    if (!(go & (loop < maxSteps))) goto searched;
-    val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
-    ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
-    lx += incx;
-    ly += incy;
-    float ee = 0.0f;
-    if ((k & 1) == 0) {
-      e1A = val_p2 - rv_p2; ee += e1A * e1A;
-      e2A = val_p1 - rv_p1; ee += e2A * e2A;
-      e3A = val - rv; ee += e3A * e3A;
-      e4A = val_m1 - rv_m1; ee += e4A * e4A;
-      e5A = val_m2 - rv_m2; ee += e5A * e5A;
-    } else {
-      e1B = val_p2 - rv_p2; ee += e1B * e1B;
-      e2B = val_p1 - rv_p1; ee += e2B * e2B;
-      e3B = val - rv; ee += e3B * e3B;
-      e4B = val_m1 - rv_m1; ee += e4B * e4B;
-      e5B = val_m2 - rv_m2; ee += e5B * e5B;
-    }
-    if (ee < best_err) {
-      second_err = best_err;
-      loopSecond = loopBest;
-      best_err = ee;
-      loopBest = loop;
-      best_errPre = eeLast;
-      best_DiffPre = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-      best_errPost = -1.0f;
-      best_DiffPost = -1.0f;
-      best_x = cpx;
-      best_y = cpy;
-      bestWasLast = true;
-    } else {
-      if (bestWasLast) {
-        best_errPost = ee;
-        best_DiffPost = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-        bestWasLast = false;
+      for (int t = 0; t < 5; ++t) {
+        const int m = j0 - 3 + t;
+        if (m <= 1) {
+          W[t] = m == 1 ? init_p1 : m == 0 ? init_0 : m == -1 ? init_m1 : init_m2;
+        } else {
+          walk(wx, wy, wj, m - 2, incx, incy);
+          // clamped fetch: a chunk past the line's end may start off the image
+          W[t] = blend_quad(fetch_quad(refImg, wx + 2.0f * incx, wy + 2.0f * incy, width, height),
+                            wx + 2.0f * incx, wy + 2.0f * incy);
+        }
       }
-      if (ee < second_err) {
-        second_err = ee;
-        loopSecond = loop;
+      walk(wx, wy, wj, j0, incx, incy);
+      float cpx = wx, cpy = wy;
+      int loop = j0;
+      float val_m2 = W[1], val_m1 = W[2], val = W[3], val_p1 = W[4], val_p2;
+      float e1A = NAN, e2A = NAN, e3A = NAN, e4A = NAN, e5A = NAN;
+      float e1B = NAN, e2B = NAN, e3B = NAN, e4B = NAN, e5B = NAN;
+      float eeLast = -1.0f;
+      if (j0 > 0) {  // the step before the chunk (odd: the B registers)
+        eeLast = 0.0f;
+        e1B = W[4] - rv_p2; eeLast += e1B * e1B;
+        e2B = W[3] - rv_p1; eeLast += e2B * e2B;
+        e3B = W[2] - rv; eeLast += e3B * e3B;
+        e4B = W[1] - rv_m1; eeLast += e4B * e4B;
+        e5B = W[0] - rv_m2; eeLast += e5B * e5B;
       }
+      bool bestWasLast = false;
+      // The texels of the next STEREO_AHEAD steps' samples are in flight
+      // while a step is evaluated: a lead copy of the position chain fetches
+      // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
+      // end are clamped to the image and never used.
+      float lx = cpx, ly = cpy;
+      Quad ring[STEREO_AHEAD];
+#pragma unroll
+      for (int k = 0; k < STEREO_AHEAD; ++k) {
+        ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+        lx += incx;
+        ly += incy;
+      }
+      // Lanes leave the loop together (a wave-uniform exit every
+      // STEREO_AHEAD steps; finished lanes idle), and every step issues its
+      // fetch whether or not the lane is still searching: with per-lane exits
+      // the wave's outstanding-load count would differ between paths and the
+      // compiler would wait for all fetches at every step.
+      bool alive = true;
+      for (;;) {
+#pragma unroll
+        for (int k = 0; k < STEREO_AHEAD; ++k) {
+          // the reference's while condition, without short-circuit branches
+          const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
+          const bool act = alive & go & (loop < maxSteps);
+          val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
+          ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+          lx += incx;
+          ly += incy;
+          alive = act;
+          if (act) {
+            float ee = 0.0f;
+            if ((k & 1) == 0) {
+              e1A = val_p2 - rv_p2; ee += e1A * e1A;
+              e2A = val_p1 - rv_p1; ee += e2A * e2A;
+              e3A = val - rv; ee += e3A * e3A;
+              e4A = val_m1 - rv_m1; ee += e4A * e4A;
+              e5A = val_m2 - rv_m2; ee += e5A * e5A;
+            } else {
+              e1B = val_p2 - rv_p2; ee += e1B * e1B;
+              e2B = val_p1 - rv_p1; ee += e2B * e2B;
+              e3B = val - rv; ee += e3B * e3B;
+              e4B = val_m1 - rv_m1; ee += e4B * e4B;
+              e5B = val_m2 - rv_m2; ee += e5B * e5B;
+            }
+            if (loop == j1) {  // the next lane's first step: only the error after a best
+              if (bestWasLast) {
+                sc.errPost = ee;
+                sc.diffPost = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+              }
+              lookValid = true;
+              alive = false;
+            } else {
+              if (ee < sc.best) {
+                sc.second = sc.best;
+                sc.isecond = sc.ibest;
+                sc.best = ee;
+                sc.ibest = loop;
+                sc.errPre = eeLast;
+                sc.diffPre = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+                sc.errPost = -1.0f;
+                sc.diffPost = -1.0f;
+                sc.bx = cpx;
+                sc.by = cpy;
+                bestWasLast = true;
+              } else if (bestWasLast | (ee < sc.second)) {  // else nothing changes (best <= second)
+                if (bestWasLast) {
+                  sc.errPost = ee;
+                  sc.diffPost = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+                  bestWasLast = false;
+                }
+                if (ee < sc.second) {
+                  sc.second = ee;
+                  sc.isecond = loop;
+                }
+              }
+              eeLast = ee;
+              val_m2 = val_m1;
+              val_m1 = val;
+              val = val_p1;
+              val_p1 = val_p2;
+              cpx += incx;
+              cpy += incy;
+              loop++;
+            }
+          }
+        }
+        if (!__any(alive)) break;
+      }
+      wx = cpx;
+      wy = cpy;
+      wj = loop;
     }
-    eeLast = ee;
-    val_m2 = val_m1;
-    val_m1 = val;
-    val = val_p1;
-    val_p1 = val_p2;
-    cpx += incx;
-    cpy += incy;
-    loop++;
+    if (G == 1) break;
+    // another round while lane G-1's look-ahead step (base + G C) is on the line
+    if (!__shfl(lookValid, G - 1, G)) break;
   }
-  }
-searched:
+  if (G > 1) sc = scan_group<G>(sc);
+  float best_err = sc.best;
+  const float second_err = sc.second;
+  const int loopBest = sc.best < INFINITY ? sc.ibest : -1;
+  const int loopSecond = sc.second < INFINITY ? sc.isecond : -1;
+  const float best_errPre = sc.errPre, best_DiffPre = sc.diffPre;
+  const float best_errPost = sc.errPost, best_DiffPost = sc.diffPost;
+  float best_x = sc.bx, best_y = sc.by;
   if (best_err > 4.0f * P.max_error_stereo) return -3.0f;
   if ((float)abs(loopBest - loopSecond) > 1.0f && P.min_distance_error_stereo * best_err > second_err) return -2.0f;
 
@@ -547,41 +708,65 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
   else segw[segcap - 1 - (base[1] + r)] = item;
 }
 
-// Grid: OBS_SEGS x (segcap / 256) blocks, blockIdx.y the segment.
-template <bool INL>
-__global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
-                                                        const int* __restrict__ count, int segcap) {
+#ifndef LSDG_STEREO_GROUP
+#define LSDG_STEREO_GROUP 2
+#endif
+// Lanes per observeDepthCreate search (a power of two up to 64).
+constexpr int STEREO_GROUP = LSDG_STEREO_GROUP;
+static_assert(STEREO_GROUP >= 1 && STEREO_GROUP <= 64 && (STEREO_GROUP & (STEREO_GROUP - 1)) == 0,
+              "lanes per search");
+
+// observeDepthCreate item: G lanes (creates search the whole inverse-depth
+// range, so their lines are long); `blk` indexes segcap G / 256 blocks.
+template <bool INL, int G>
+__device__ inline void observe_create(const ObsArgs& a, const lsdg_params& P, const int4* __restrict__ work,
+                                      const int* __restrict__ count, int segcap, int blk) {
   const int seg = blockIdx.y;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int nc = count[(2 * seg) * OBS_COUNTER_STRIDE], nu = count[(2 * seg + 1) * OBS_COUNTER_STRIDE];
-  bool create;
-  if (i < nc) create = true;
-  else if (i >= segcap - nu && i < segcap) create = false;
-  else return;
+  const int i = blk * (256 / G) + threadIdx.x / G, g = threadIdx.x % G;
+  const int nc = count[(2 * seg) * OBS_COUNTER_STRIDE];
+  if (i >= nc) return;  // the whole group
   const int4 item = work[(size_t)seg * segcap + i];
+  const int w = a.g.w;
+  const int idx = item.x, x = idx % w, y = idx / w;
+  const float epx = __int_as_float(item.z), epy = __int_as_float(item.w);
+  // every create searches the same frame (item.y holds this index): read
+  // as a launch-uniform value its image pointer and products stay scalar
+  const StereoRef& ref = obs_ref<INL>(a, a.kfReactivated ? a.newestIdx : a.oldestIdx);
+  ObsCtx o{a.g, a.kfImg, a.kfGrad};
+  float rid, rvar, repl;
+  float err =
+      line_stereo<G>(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl, g);
+  if (g != 0) return;
+  uint8_t code;
+  if (err == -3.0f || err == -2.0f) a.cur.bl[idx] = a.cur.bl[idx] - 1;
+  if (err < 0.0f) {
+    code = (uint8_t)(LSDG_OBS_STEREO_FAIL_BASE - (int)err);
+  } else if (rvar > P.max_var) {
+    code = LSDG_OBS_CREATE_VAR_TOO_BIG;
+  } else {
+    rid = unzero(rid);
+    store_hyp(a.cur, idx, make_hyp(rid, rvar, P.validity_counter_initial_observe));
+    code = LSDG_OBS_CREATED;
+  }
+  a.obs[idx] = code;
+}
+
+// observeDepthUpdate item, one thread (short lines around the prior); `blk`
+// indexes segcap / 256 blocks; item i of a segment is slot segcap - 1 - i.
+template <bool INL>
+__device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, const int4* __restrict__ work,
+                                      const int* __restrict__ count, int segcap, int blk) {
+  const int seg = blockIdx.y;
+  const int i = blk * 256 + threadIdx.x;
+  const int nu = count[(2 * seg + 1) * OBS_COUNTER_STRIDE];
+  if (i >= nu) return;
+  const int4 item = work[(size_t)seg * segcap + segcap - 1 - i];
   const int w = a.g.w;
   const int idx = item.x, x = idx % w, y = idx / w;
   const float epx = __int_as_float(item.z), epy = __int_as_float(item.w);
   const StereoRef& ref = obs_ref<INL>(a, item.y);
   ObsCtx o{a.g, a.kfImg, a.kfGrad};
   uint8_t code;
-  if (create) {
-    // observeDepthCreate
-    float rid, rvar, repl;
-    float err = line_stereo(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
-    if (err == -3.0f || err == -2.0f) a.cur.bl[idx] = a.cur.bl[idx] - 1;
-    if (err < 0.0f) {
-      code = (uint8_t)(LSDG_OBS_STEREO_FAIL_BASE - (int)err);
-    } else if (rvar > P.max_var) {
-      code = LSDG_OBS_CREATE_VAR_TOO_BIG;
-    } else {
-      rid = unzero(rid);
-      store_hyp(a.cur, idx, make_hyp(rid, rvar, P.validity_counter_initial_observe));
-      code = LSDG_OBS_CREATED;
-    }
-    a.obs[idx] = code;
-    return;
-  }
   // observeDepthUpdate
   Hyp t = load_hyp(a.cur, idx);
   float sv = sqrtf(t.dv.w);
@@ -590,7 +775,7 @@ __global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P
   if (min_id < 0.0f) min_id = 0.0f;
   if (max_id > 1.0f / P.min_depth) max_id = 1.0f / P.min_depth;
   float rid = 0.0f, rvar = 0.0f, repl = 0.0f;
-  float err = line_stereo(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl);
+  float err = line_stereo<1>(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl, 0);
   float diff = rid - t.dv.z;
   if (err == -1.0f) {
     code = LSDG_OBS_STEREO_FAIL_BASE + 1;
@@ -636,6 +821,17 @@ __global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P
     code = LSDG_OBS_UPDATED;
   }
   a.obs[idx] = code;
+}
+
+// Both searches in one launch, so that the short update searches fill the
+// gaps the long create searches leave.  Grid: OBS_SEGS x (segcap G / 256 +
+// segcap / 256) blocks, blockIdx.y the segment; create blocks first.
+template <bool INL, int G>
+__global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
+                                                        const int* __restrict__ count, int segcap) {
+  const int createBlocks = segcap / 256 * G;
+  if ((int)blockIdx.x < createBlocks) observe_create<INL, G>(a, P, work, count, segcap, blockIdx.x);
+  else observe_update<INL>(a, P, work, count, segcap, blockIdx.x - createBlocks);
 }
 
 // ---------------------------------------------------------- fill holes
@@ -1076,12 +1272,13 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
   int* count = dm->d_work_count + dm->obsParity * set;
   int* other = dm->d_work_count + (1 - dm->obsParity) * set;
   dm->obsParity ^= 1;
+  const dim3 gs(segBlocks * (STEREO_GROUP + 1), OBS_SEGS);
   if (inl) {
     k_observe_gate<true><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
-    k_observe_stereo<true><<<dim3(segBlocks, OBS_SEGS), dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
+    k_observe_stereo<true, STEREO_GROUP><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
   } else {
     k_observe_gate<false><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
-    k_observe_stereo<false><<<dim3(segBlocks, OBS_SEGS), dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
+    k_observe_stereo<false, STEREO_GROUP><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
   }
   return hipGetLastError();
 }

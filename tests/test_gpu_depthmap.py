@@ -143,6 +143,26 @@ def test_consecutive_updates_bit_exact(ctx, depth_inst):
     assert codes[lsdg.OBS["UPDATED"]] > 1000
 
 
+def test_update_posterized_images_bit_exact(ctx, depth_inst):
+    """Images quantised to 9 grey levels: many exactly equal matching errors
+    along a line, so the first-minimum rules for the best and second match
+    decide the outcome."""
+    d = dict(depth_inst)
+    for k in ("kf_img", "ref_img", "kf2_img"):
+        a = d[k].astype(np.float32)
+        d[k] = np.clip(np.round(a / 32.0) * 32.0, 0, 255).astype(d[k].dtype)
+    p = _params(100)
+    ctx.set_params(p)
+    try:
+        g = _scenario(ctx, d)
+    finally:
+        ctx.set_params(lsdg.default_params())
+    o0 = _scenario(pyoracle.Oracle(0, params=p), d)
+    assert not _diff(g[0], o0[0]), _diff(g[0], o0[0])
+    codes = np.bincount(g[0]["observe_code"], minlength=15)
+    assert codes[lsdg.OBS["CREATED"]] > 100 and codes[lsdg.OBS["STEREO_FAIL_BASE"] + 2] > 100
+
+
 def test_initialize_from_gt_and_finalize(ctx, depth_inst):
     d = depth_inst
     cam = d["cam"]
