@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_batch_eval launch on the headline workload, from the
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy kernel
 # (profiles/k_batch_eval_se3_640x480_x64_fp16.md); re-measured when the kernel changes.
-EVAL_TRAFFIC_BYTES = 20956108
+EVAL_TRAFFIC_BYTES = 10134461
 
 
 def tracking_params(its=20, crop=None):
@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the batched FP16 leg (for kernel profiling)")
+    ap.add_argument("--depth-only", action="store_true",
+                    help="run only the depth-map update leg (for kernel profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -255,6 +257,13 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
+    if args.depth_only:
+        depth, _ = depth_leg(ctx, args.steps, args.warmup)
+        print(json.dumps({"depth_updates_per_s": round(depth["updates_per_s"], 2),
+                          "depth_update_ms": round(depth["ms_per_update"], 4),
+                          "depth_device_ms_per_update": round(depth["device_ms_per_update"], 4),
+                          "depth_roofline_gbs": depth["roofline_gbs"]}))
+        return
     insts = [synth.tracking_instance(1001 + 17 * i) for i in range(args.batch)]
     B = args.batch
 
@@ -336,6 +345,8 @@ def main():
         "sim3_1920x1080": sim3,
         "depth_updates_per_s": round(depth["updates_per_s"], 2),
         "depth_update_ms": round(depth["ms_per_update"], 4),
+        "depth_device_ms_per_update": round(depth["device_ms_per_update"], 4),
+        "depth_roofline_gbs": depth["roofline_gbs"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": EVAL_TRAFFIC_BYTES,
                      "kernel": "k_batch_eval (fused residual + weights + LGS over all jobs' tiles, FP16 taps)",
