@@ -342,6 +342,7 @@ lsdg_status lsdg_depth_active_keyframe(lsdg_depthmap dm, int32_t* id);
 #define LSDG_KERNEL_BATCH_EVAL 3  /* batched evaluation sweep */
 #define LSDG_KERNEL_BATCH_SOLVE 4 /* batched LM step (merge, solve, accept) */
 #define LSDG_KERNEL_BATCH_JOB 5   /* batched whole-job LM, one block per job */
+#define LSDG_KERNEL_BATCH_COOP 6  /* batched whole LM, several cooperating blocks per job */
 #define LSDG_KERNEL_KINDS 8
 
 typedef struct lsdg_kernel_stats {

@@ -60,10 +60,12 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// remaining may be null: blocks that repeat another block's LM step (the
+// cooperative kernel) leave the count to that block.
 __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* remaining) {
   s.status = status;
   *ntNext = 0;
-  if (lm_lane() == 0) atomicSub(remaining, 1);
+  if (remaining && lm_lane() == 0) atomicSub(remaining, 1);
 }
 
 // se3_exp (lsdg_lie.h) with sin / cos of theta / 2 and theta evaluated on
@@ -570,6 +572,131 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
   if (tid == 0) a.ntiles[j] = s.status == 0 ? jd.lv[s.level].ntiles : 0;
 }
 
+// Cooperative whole-LM kernel: K blocks per job (blockIdx.x = job * K + part)
+// run the job's whole LM loop together, every level included.  A level of at
+// most soloMax points is evaluated whole by every block of the job.  A larger
+// level is cut into K contiguous point ranges: each block reduces its range
+// to an FP64 row, publishes it (device-scope stores, then a release add on the
+// job's arrival counter), waits until the counter shows all K rows of this
+// iteration, and sums the K rows in part order.  In both cases every block
+// of the job then holds the same sums and runs the same LM step on its own
+// LDS copy of the control block, so no result is broadcast; part 0 alone
+// counts finished jobs and writes the control block back.  Rows are double
+// buffered by iteration parity: a block can be at most one iteration ahead of
+// the slowest block of its job.  The wait is bounded: if the K blocks of a job
+// cannot all be resident (the grid is sized from the occupancy, so this does
+// not happen on an idle device), the blocks stop and set *remaining negative,
+// which the host reports as an error.
+#ifndef LSDG_COOP_BS
+#define LSDG_COOP_BS 512
+#endif
+constexpr int COOP_BS = LSDG_COOP_BS;
+constexpr unsigned COOP_SPIN_MAX = 1u << 22;
+#ifndef LSDG_SPIN_SLEEP
+#define LSDG_SPIN_SLEEP 2
+#endif
+
+template <int KIND, int PREC>
+__global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
+  constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
+  constexpr int WAVES = COOP_BS / 64;
+  __shared__ JobState s;
+  __shared__ JobDesc jd;
+  __shared__ double sh[WAVES][NS];
+  __shared__ double S[NS];
+  __shared__ double lmsc[LM_SCRATCH];
+  __shared__ int arrived;
+  const int K = a.coopK;
+  const int j = blockIdx.x / K, part = blockIdx.x % K;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int* remaining = part == 0 ? a.remaining : nullptr;
+  double* rows = a.coopRows + (size_t)j * 2 * K * NS;
+  unsigned* arrive = a.counters + j;
+  block_copy(&s, a.st + j, COOP_BS);
+  block_copy(&jd, a.jobs + j, COOP_BS);
+  __syncthreads();
+  unsigned gen = 0;  // synchronised iterations of this job so far
+  for (int it = 0; it < BATCH_JOB_MAX_EVALS; ++it) {
+    // s is read after a barrier: the exit test is uniform over the block
+    if (s.status != 0) break;
+    const int lvl = s.level;
+    const JobLevel& L = jd.lv[lvl];
+    const int n = L.pts.n;
+    const bool split = K > 1 && n > a.soloMax;
+    const int i0 = split ? (int)((long long)n * part / K) : 0;
+    const int i1 = split ? (int)((long long)n * (part + 1) / K) : n;
+    const Xf x = s.xf;
+    TrackKParams kp = a.kp;
+    kp.aff_a = s.affA;
+    kp.aff_b = s.affB;
+    Accum<ACC_F64, NS> acc;
+    acc.zero();
+    if (KIND == LSDG_JOB_SIM3) {
+      kp.roll[0] = s.roll[0]; kp.roll[1] = s.roll[1]; kp.roll[2] = s.roll[2]; kp.roll[3] = s.roll[3];
+      sim3_points<PREC, BATCH_U>(L.pts, i0 + tid, COOP_BS, i1, L.img, L.fid, L.fvar, L.g, x, kp, acc);
+    } else {
+      uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
+      se3_points<PREC, BATCH_U>(L.pts, i0 + tid, COOP_BS, i1, L.img, L.g, x, kp, mask, acc);
+    }
+    const double wsum = acc.wave_reduce_scatter();
+    if (lane < NS) sh[wave][lane] = wsum;
+    __syncthreads();
+    if (!split) {
+      if (tid < NS) {
+        double v = 0.0;
+        for (int w = 0; w < WAVES; ++w) v += sh[w][tid];
+        S[tid] = v;
+      }
+    } else {
+      double* buf = rows + (size_t)(gen & 1) * K * NS;
+      if (tid < 64) {  // wave 0: publish this part's row, then wait for the others
+        if (tid < NS) {
+          double v = 0.0;
+          for (int w = 0; w < WAVES; ++w) v += sh[w][tid];
+          __hip_atomic_store((unsigned long long*)&buf[(size_t)part * NS + tid],
+                             (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tid == 0) {
+          __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          const unsigned target = (unsigned)K * (gen + 1);
+          unsigned spins = 0;
+          while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
+                 ++spins < COOP_SPIN_MAX)
+            __builtin_amdgcn_s_sleep(LSDG_SPIN_SLEEP);
+          arrived = spins < COOP_SPIN_MAX;
+        }
+      }
+      __syncthreads();
+      if (!arrived) {
+        if (tid == 0) __hip_atomic_store(a.remaining, -(1 << 20), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (tid < NS) {
+        double v = 0.0;
+        for (int q = 0; q < K; ++q) {
+          const unsigned long long bits = __hip_atomic_load((const unsigned long long*)&buf[(size_t)q * NS + tid],
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          v += __longlong_as_double((long long)bits);
+        }
+        S[tid] = v;
+      }
+      ++gen;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      int next = 0;
+      if (KIND == LSDG_JOB_SIM3) lm_step_sim3(s, jd, S, a.P, &next, remaining, lmsc);
+      else lm_step_se3(s, jd, S, a.P, &next, remaining, lmsc);
+    }
+    __syncthreads();
+  }
+  if (part == 0) {
+    block_copy(a.st + j, &s, COOP_BS);
+    if (tid == 0) a.ntiles[j] = s.status == 0 ? jd.lv[s.level].ntiles : 0;
+  }
+}
+
 // ------------------------------------------------------------ launchers
 hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s, const KTimer* t) {
   // FP64 per-thread sums whatever `acc` says: a thread of the whole-job block
@@ -582,6 +709,30 @@ hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hip
                                     : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_F64>
                                                         : k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_F64>);
   return launch_k(k, dim3(a.njobs), dim3(BATCH_JOB_BS), s, t, a);
+}
+
+static void (*coop_kernel(int kind, int prec))(BatchArgs) {
+  if (kind == LSDG_JOB_SIM3)
+    return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_FP16>
+           : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_BF16>
+                               : k_batch_coop<LSDG_JOB_SIM3, LSDG_FP32>;
+  return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_FP16>
+         : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_BF16>
+                             : k_batch_coop<LSDG_JOB_SE3, LSDG_FP32>;
+}
+
+hipError_t batch_coop_capacity(int kind, int prec, int* blocks) {
+  int dev = 0, perCU = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coop_kernel(kind, prec), COOP_BS, 0);
+  *blocks = e == hipSuccess ? perCU * cus : 0;
+  return e;
+}
+
+hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, hipStream_t s, const KTimer* t) {
+  return launch_k(coop_kernel(kind, prec), dim3(a.njobs * a.coopK), dim3(COOP_BS), s, t, a);
 }
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {

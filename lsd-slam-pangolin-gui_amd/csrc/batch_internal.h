@@ -63,11 +63,18 @@ struct BatchArgs {
   int* remaining;     // jobs still running
   int ppt;            // points per thread per tile
   int jobMaxPoints;   // levels with at most this many points run in k_batch_job
+  double* coopRows;   // k_batch_coop: [njobs][2][coopK] rows of NS doubles
+  int coopK;          // k_batch_coop: blocks per job
+  int soloMax;        // k_batch_coop: levels of at most this many points are not split
   lsdg_params P;
   TrackKParams kp;
 };
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s);
+// Cooperative whole-LM kernel (a.coopK blocks per job, all levels) and the
+// number of its blocks the device holds at once.
+hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, hipStream_t s, const KTimer* t = nullptr);
+hipError_t batch_coop_capacity(int kind, int prec, int* blocks);
 // Launches with an optional kernel timer (see launch_k).
 hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s,
                             const KTimer* t = nullptr);

@@ -40,6 +40,11 @@ struct lsdg_batch_s {
   // Levels of up to this many points run in k_batch_job (12 points per
   // thread of its 512-thread block); larger ones in the multi-block steps.
   int jobMaxPoints = 12 * BATCH_JOB_BS;
+  // Cooperative schedule: coopK blocks per job run every level's LM loop in
+  // one launch (0: the whole-job kernel and multi-block steps instead).
+  int coopK = 0;
+  int soloMax = 2048;
+  double* d_coop = nullptr;
   std::vector<lsdg_job> jobs;
   std::vector<int> depthVersion;
   std::vector<JobDesc> h_jobs;
@@ -76,6 +81,7 @@ static void batch_free(lsdg_batch b) {
   if (b->d_remaining) (void)hipFree(b->d_remaining);
   if (b->h_remaining) (void)hipHostFree(b->h_remaining);
   if (b->d_partials) (void)hipFree(b->d_partials);
+  if (b->d_coop) (void)hipFree(b->d_coop);
   if (b->d_pyr) (void)hipFree(b->d_pyr);
   if (b->h_pyr) (void)hipHostFree(b->h_pyr);
   delete b;
@@ -149,6 +155,9 @@ static BatchArgs batch_args(lsdg_batch b, int c) {
   a.remaining = b->d_remaining + c;
   a.ppt = b->ppt;
   a.jobMaxPoints = b->jobMaxPoints;
+  a.coopRows = b->d_coop + (size_t)j0 * 2 * std::max(b->coopK, 1) * (b->kind == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3);
+  a.coopK = b->coopK;
+  a.soloMax = b->soloMax;
   a.P = b->ctx->P;
   a.kp = track_kparams(b->ctx->P, b->kind == LSDG_JOB_SIM3);
   return a;
@@ -171,7 +180,8 @@ static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec
     BatchArgs a = batch_args(b, k);
     if (first) {
       e = launch_batch_init(a, st);
-      if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, st);
+      if (e == hipSuccess && b->coopK > 0) e = launch_batch_coop(a, b->kind, b->prec, st);
+      else if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, st);
     }
     for (int s = 0; s < steps && e == hipSuccess; ++s)
       e = launch_batch_step(a, b->kind, b->prec, b->acc, s & 1, b->grid, st);
@@ -252,6 +262,24 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   if (const char* e = std::getenv("LSDG_BATCH_JOB_POINTS")) b->jobMaxPoints = std::max(0, std::atoi(e));
   b->chains = n >= 16 ? 2 : 1;
   if (const char* e = std::getenv("LSDG_BATCH_CHAINS")) b->chains = std::max(1, std::min(8, std::min(n, std::atoi(e))));
+  // The cooperative schedule when the device holds at least two blocks per
+  // job at once (LSDG_BATCH_COOP=0 turns it off, LSDG_BATCH_COOP_K caps the
+  // blocks per job, LSDG_BATCH_SOLO sets the largest level not split).
+  {
+    bool coop = true;
+    if (const char* e = std::getenv("LSDG_BATCH_COOP")) coop = std::atoi(e) != 0;
+    int cap = 0;
+    if (coop && batch_coop_capacity(kind, prec, &cap) == hipSuccess) {
+      int K = std::min(16, cap / n);
+      if (const char* e = std::getenv("LSDG_BATCH_COOP_K")) K = std::min(K, std::max(1, std::atoi(e)));
+      if (K >= 2) {
+        b->coopK = K;
+        b->chains = 1;
+        b->jobMaxPoints = 1 << 30;
+      }
+    }
+    if (const char* e = std::getenv("LSDG_BATCH_SOLO")) b->soloMax = std::max(0, std::atoi(e));
+  }
   b->grid = 1024 / b->chains;  // evaluation blocks per chain
   if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));
   b->jobs.assign(jobs, jobs + n);
@@ -269,6 +297,7 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
       (e = hipMalloc(&b->d_ntiles, sizeof(int) * 2 * n)) != hipSuccess ||
       (e = hipMalloc(&b->d_counters, sizeof(unsigned) * n)) != hipSuccess ||
       (e = hipMalloc(&b->d_remaining, 64)) != hipSuccess ||
+      (b->coopK > 0 && (e = hipMalloc(&b->d_coop, sizeof(double) * 2 * b->coopK * NS_SIM3 * (size_t)n)) != hipSuccess) ||
       (e = hipHostMalloc(&b->h_remaining, 64, hipHostMallocDefault)) != hipSuccess) {
     batch_free(b);
     return fail(c, LSDG_ENOMEM, "batch allocation", e);
@@ -426,7 +455,11 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
   for (int k = 0; k < b->chains; ++k) {
     BatchArgs a = batch_args(b, k);
     CK(c, launch_batch_init(a, c->stream));
-    if (b->jobMaxPoints > 0) {
+    if (b->coopK > 0) {
+      KTimer kt = prof_timer(c);
+      CK(c, launch_batch_coop(a, b->kind, b->prec, c->stream, &kt));
+      prof_timer_done(c, LSDG_KERNEL_BATCH_COOP, kt, 0.0);
+    } else if (b->jobMaxPoints > 0) {
       KTimer kt = prof_timer(c);
       CK(c, launch_batch_job(a, b->kind, b->prec, b->acc, c->stream, &kt));
       prof_timer_done(c, LSDG_KERNEL_BATCH_JOB, kt, 0.0);
@@ -444,6 +477,7 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
       }
       CK(c, hipMemcpyAsync(b->h_remaining + k, b->d_remaining + k, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       CK(c, hipStreamSynchronize(c->stream));
+      if (b->h_remaining[k] < 0) return fail(c, LSDG_EDEVICE, "batched LM: cooperative blocks not co-resident");
       done = b->h_remaining[k] <= 0;
     }
     if (!done) return fail(c, LSDG_ESTATE, "batched LM did not finish");
@@ -474,6 +508,8 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
   } else {
     CK(c, hipGraphLaunch(b->first, c->stream));
     CK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < b->chains; ++k)
+      if (b->h_remaining[k] < 0) return fail(c, LSDG_EDEVICE, "batched LM: cooperative blocks not co-resident");
     int launches = 0;
     while (remaining_jobs(b) > 0) {
       if (++launches > 64) return fail(c, LSDG_ESTATE, "batched LM did not finish within 64 continuation graphs");
@@ -512,8 +548,12 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
           if (np > b->jobMaxPoints) bytesSplit += eb;
           else bytesJob += eb;
         }
-      c->prof[LSDG_KERNEL_BATCH_EVAL].bytes += bytesSplit;
-      c->prof[LSDG_KERNEL_BATCH_JOB].bytes += bytesJob;
+      if (b->coopK > 0) {
+        c->prof[LSDG_KERNEL_BATCH_COOP].bytes += bytesSplit + bytesJob;
+      } else {
+        c->prof[LSDG_KERNEL_BATCH_EVAL].bytes += bytesSplit;
+        c->prof[LSDG_KERNEL_BATCH_JOB].bytes += bytesJob;
+      }
     }
   }
   return LSDG_OK;

@@ -120,7 +120,7 @@ def hyp_struct(arrs):
     return Hypotheses(*vals)
 
 
-KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE, KERNEL_BATCH_JOB = range(6)
+KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE, KERNEL_BATCH_JOB, KERNEL_BATCH_COOP = range(7)
 
 
 class KernelStats(C.Structure):

@@ -21,14 +21,27 @@ from helpers import tracking_params, twist
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["job", "steps", "hybrid", "default", "chains"])
+SCHEDULES = {
+    # name: (LSDG_BATCH_COOP, LSDG_BATCH_JOB_POINTS, LSDG_BATCH_CHAINS, LSDG_BATCH_COOP_K, LSDG_BATCH_SOLO)
+    "job": ("0", "1000000000", "1", None, None),
+    "steps": ("0", "0", "1", None, None),
+    "hybrid": ("0", "3000", "1", None, None),
+    "chains": ("0", "3000", "3", None, None),
+    "default": (None, None, None, None, None),
+    "coop-k2": (None, None, None, "2", None),
+    "coop-split-all": (None, None, None, None, "0"),
+}
+
+
+@pytest.fixture(autouse=True, params=list(SCHEDULES))
 def schedule(request, monkeypatch):
-    """Levels up to LSDG_BATCH_JOB_POINTS points run in the whole-job kernel;
-    LSDG_BATCH_CHAINS groups of jobs step as parallel graph branches (both
-    read when a plan is created)."""
-    points, chains = {"job": ("1000000000", "1"), "steps": ("0", "1"), "hybrid": ("3000", "1"),
-                      "default": (None, None), "chains": ("3000", "3")}[request.param]
-    for var, val in (("LSDG_BATCH_JOB_POINTS", points), ("LSDG_BATCH_CHAINS", chains)):
+    """The cooperative kernel (several blocks per job, every level, levels
+    above LSDG_BATCH_SOLO points split over the blocks) unless
+    LSDG_BATCH_COOP=0; then levels up to LSDG_BATCH_JOB_POINTS points run in
+    the whole-job kernel and LSDG_BATCH_CHAINS groups of jobs step as
+    parallel graph branches (all read when a plan is created)."""
+    names = ("LSDG_BATCH_COOP", "LSDG_BATCH_JOB_POINTS", "LSDG_BATCH_CHAINS", "LSDG_BATCH_COOP_K", "LSDG_BATCH_SOLO")
+    for var, val in zip(names, SCHEDULES[request.param]):
         if val is None:
             monkeypatch.delenv(var, raising=False)
         else:
@@ -84,6 +97,25 @@ def test_batch_se3_matches_plain(ctx, precision):
         assert twist(res.frame_to_ref[:7], insts[i]["gt"]) < 2e-2
         # replay of the same graph is bit-identical
         assert list(r2[i].frame_to_ref) == list(res.frame_to_ref)
+
+
+def test_batch_schedule_kernels(ctx, schedule):
+    """The profiled run launches the kernels of the schedule the plan chose."""
+    insts = _instances(4, 2051, [(320, 240)])
+    items = _setup(ctx, insts, lsdg.FP32)
+    b = ctx.batch(_jobs(items))
+    ctx.profile(True)
+    try:
+        b.run()
+        coop = ctx.kernel_stats(lsdg.KERNEL_BATCH_COOP)["launches"]
+        job = ctx.kernel_stats(lsdg.KERNEL_BATCH_JOB)["launches"]
+    finally:
+        ctx.profile(False)
+        b.destroy()
+    if schedule.startswith("coop") or schedule == "default":
+        assert coop == 1 and job == 0
+    else:
+        assert coop == 0
 
 
 def test_batch_divergent_job(ctx):
