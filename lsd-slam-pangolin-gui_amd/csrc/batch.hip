@@ -1,5 +1,7 @@
 // Batched tracking: many frame-vs-keyframe alignments of uneven size, with no
-// host round trip between LM iterations.  Two schedules share the LM code:
+// host round trip between LM iterations.  Three schedules share the LM code:
+// the cooperative kernel (k_batch_coop, further down: several blocks per job,
+// every level in one launch) and the two below.
 //
 // Whole-job kernel (k_batch_job).  While a job's level is small, one block
 // runs the job's LM loop by itself: evaluation over the level's points,

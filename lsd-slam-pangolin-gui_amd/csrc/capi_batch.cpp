@@ -2,6 +2,9 @@
 //
 // A plan holds the device job table, one LM control block per job, the
 // per-tile partial-sum rows and a hipGraph of
+//     [batched pyramid rebuild] -> init -> k_batch_coop (every level, coopK
+//     blocks per job) -> copy of the remaining-jobs counter to host memory
+// when the device holds at least two blocks per job at once, else
 //     [batched pyramid rebuild] -> init -> k_batch_job (levels with at most
 //     jobMaxPoints points, one block per job) -> K x (k_batch_eval,
 //     k_batch_solve) for larger levels -> copy of the remaining-jobs counter
