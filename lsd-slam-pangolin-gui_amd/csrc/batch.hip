@@ -70,16 +70,20 @@ __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* rem
   if (remaining && lm_lane() == 0) atomicSub(remaining, 1);
 }
 
-// se3_exp (lsdg_lie.h) with sin / cos of theta / 2 and theta evaluated on
-// four lanes at once (same arguments, same functions, same results).
+// se3_exp (lsdg_lie.h) with sin and cos of theta / 2 (lane 0) and theta
+// (lane 1) evaluated at once, one sincos per lane.
 __device__ inline Pose se3_exp_wave(const double* xi, double* sc) {
   const double* w = xi + 3;
   const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   const double th = sqrt(th2);
   if (th >= 1e-10) {
     const int lane = lm_lane();
-    if (lane < 2) sc[lane] = sin(lane == 0 ? 0.5 * th : th);
-    else if (lane < 4) sc[lane] = cos(lane == 2 ? 0.5 * th : th);
+    if (lane < 2) {
+      double sv, cv;
+      sincos(lane == 0 ? 0.5 * th : th, &sv, &cv);
+      sc[lane] = sv;
+      sc[2 + lane] = cv;
+    }
     wave_sync();
   }
   Pose r;

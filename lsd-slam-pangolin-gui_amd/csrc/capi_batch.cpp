@@ -67,7 +67,7 @@ struct lsdg_batch_s {
   PyrBatch pyr{};
   hipGraphExec_t first = nullptr, cont = nullptr;
   hipGraph_t g_first = nullptr, g_cont = nullptr;
-  std::vector<JobState> h_st;
+  JobState* h_st = nullptr;  // pinned: every graph ends by copying the control blocks here
 };
 
 static void batch_free(lsdg_batch b) {
@@ -83,6 +83,7 @@ static void batch_free(lsdg_batch b) {
   for (hipEvent_t ev : b->forkJoin) (void)hipEventDestroy(ev);
   if (b->d_remaining) (void)hipFree(b->d_remaining);
   if (b->h_remaining) (void)hipHostFree(b->h_remaining);
+  if (b->h_st) (void)hipHostFree(b->h_st);
   if (b->d_partials) (void)hipFree(b->d_partials);
   if (b->d_coop) (void)hipFree(b->d_coop);
   if (b->d_pyr) (void)hipFree(b->d_pyr);
@@ -189,6 +190,9 @@ static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec
     for (int s = 0; s < steps && e == hipSuccess; ++s)
       e = launch_batch_step(a, b->kind, b->prec, b->acc, s & 1, b->grid, st);
     if (e == hipSuccess) e = hipMemcpyAsync(b->h_remaining + k, b->d_remaining + k, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->h_st + chain_begin(b, k), b->d_st + chain_begin(b, k), sizeof(JobState) * a.njobs,
+                         hipMemcpyDeviceToHost, st);
   }
   // join
   for (int k = 1; k < b->chains && e == hipSuccess; ++k) {
@@ -301,7 +305,8 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
       (e = hipMalloc(&b->d_counters, sizeof(unsigned) * n)) != hipSuccess ||
       (e = hipMalloc(&b->d_remaining, 64)) != hipSuccess ||
       (b->coopK > 0 && (e = hipMalloc(&b->d_coop, sizeof(double) * 2 * b->coopK * NS_SIM3 * (size_t)n)) != hipSuccess) ||
-      (e = hipHostMalloc(&b->h_remaining, 64, hipHostMallocDefault)) != hipSuccess) {
+      (e = hipHostMalloc(&b->h_remaining, 64, hipHostMallocDefault)) != hipSuccess ||
+      (e = hipHostMalloc(&b->h_st, sizeof(JobState) * n, hipHostMallocDefault)) != hipSuccess) {
     batch_free(b);
     return fail(c, LSDG_ENOMEM, "batch allocation", e);
   }
@@ -374,7 +379,6 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
     batch_free(b);
     return st;
   }
-  b->h_st.resize(n);
   *out = b;
   return LSDG_OK;
 }
@@ -522,7 +526,7 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
     // Size the first graph's multi-block steps to what this run needed (the
     // most evaluations one job ran at levels above jobMaxPoints), when it was
     // too short or more than a few steps too long.
-    CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
+    // (the graphs copied the control blocks to h_st)
     int need = 0;
     for (int j = 0; j < b->n; ++j) {
       int e = 0;
@@ -538,7 +542,7 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
     }
   }
   if (results || c->prof_on) {
-    if (c->prof_on) CK(c, hipMemcpy(b->h_st.data(), b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
+    if (c->prof_on) CK(c, hipMemcpy(b->h_st, b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
     if (results)
       for (int j = 0; j < b->n; ++j) fill_result(b, j, b->h_st[j], &results[j]);
     if (c->prof_on) {
