@@ -72,7 +72,7 @@ __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* rem
 
 // se3_exp (lsdg_lie.h) with sin and cos of theta / 2 (lane 0) and theta
 // (lane 1) evaluated at once, one sincos per lane.
-__device__ inline Pose se3_exp_wave(const double* xi, double* sc) {
+__device__ inline Pose se3_exp_wave(const double* __restrict__ xi, double* __restrict__ sc) {
   const double* w = xi + 3;
   const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   const double th = sqrt(th2);
@@ -113,7 +113,7 @@ __device__ inline Pose se3_exp_wave(const double* xi, double* sc) {
 // Solve (A diag*(1+lambda)) inc = b from the stored LGS and propose Tn = exp(inc) T.
 // Returns false when a Sim3 step diverges (|inc|^2 not in [0, 1)).
 template <int KIND>
-__device__ inline bool solve_and_propose(JobState& s, double* sc) {
+__device__ inline bool solve_and_propose(JobState& __restrict__ s, double* __restrict__ sc) {
   constexpr int N = KIND == LSDG_JOB_SIM3 ? 7 : 6;
   constexpr int NA = N * (N + 1) / 2;
   double A[N * N], b[N], inc[N];
@@ -157,7 +157,7 @@ __device__ inline bool solve_and_propose(JobState& s, double* sc) {
 // Keeps the evaluation's unnormalised LGS (SE3: 6x6 upper triangle from the
 // 7-stride sums; Sim3: 7x7) as the accepted pose's normal equations.
 template <int KIND>
-__device__ inline void store_lgs(JobState& s, const double* S) {
+__device__ inline void store_lgs(JobState& __restrict__ s, const double* __restrict__ S) {
   if (KIND == LSDG_JOB_SIM3) {
     for (int k = 0; k < 28; ++k) s.lgsA[k] = S[SM_A + k];
     for (int a = 0; a < 7; ++a) s.lgsB[a] = S[SM_B + a];
@@ -198,8 +198,8 @@ __device__ inline void begin_level(JobState& s, const JobDesc& jd, const lsdg_pa
 }
 
 // One LM step of an SE3 job, after the evaluation whose sums are S.
-__device__ void lm_step_se3(JobState& s, const JobDesc& jd, const double* S, const lsdg_params& P, int* ntNext,
-                            int* remaining, double* sc) {
+__device__ void lm_step_se3(JobState& __restrict__ s, const JobDesc& __restrict__ jd, const double* __restrict__ S,
+                            const lsdg_params& P, int* ntNext, int* remaining, double* __restrict__ sc) {
   const int lvl = s.level;
   const JobLevel& L = jd.lv[lvl];
   s.evals++;
@@ -275,8 +275,8 @@ __device__ inline void sim3_res(const double* S, double* r) {
 }
 
 // One LM step of a Sim3 job.
-__device__ void lm_step_sim3(JobState& s, const JobDesc& jd, const double* S, const lsdg_params& P, int* ntNext,
-                             int* remaining, double* sc) {
+__device__ void lm_step_sim3(JobState& __restrict__ s, const JobDesc& __restrict__ jd, const double* __restrict__ S,
+                             const lsdg_params& P, int* ntNext, int* remaining, double* __restrict__ sc) {
   const int lvl = s.level;
   const JobLevel& L = jd.lv[lvl];
   s.evals++;
