@@ -162,6 +162,42 @@ def depth_leg(ctx, steps, warmup):
     return out, d
 
 
+def export_leg(ctx, steps, warmup, cpu_seconds):
+    """Keyframe export for display at 640x480: publishKeyframe's records and
+    Keyframe::computeVbo's filter (lib/Pangolin_IOWrapper/) on the device,
+    results copied to host memory; the oracle's restatement of the same two
+    loops, one thread, beside it."""
+    d = synth.depth_instance()
+    rng = np.random.default_rng(11)
+    idp = np.where(d["prior_idepth"] > 0, 1.0 / d["kf_depth"], -1).astype(np.float32)
+    var = np.where(idp > 0, rng.uniform(1e-7, 3e-6, idp.shape), -1).astype(np.float32)
+    kf = ctx.frame(0, d["cam"], d["kf_img"])
+    kf.set_depth(idp, var)
+    p = lsdg.vbo_params(1.0)
+    for _ in range(warmup):
+        kf.keyframe_vbo(0, p)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        v = kf.keyframe_vbo(0, p)
+    t = (time.perf_counter() - t0) / steps
+    out = {"keyframe_vbo_ms": round(1e3 * t, 4), "keyframe_vbo_vertices": int(len(v))}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(0, fast=True)
+        of = o.frame(0, d["cam"], d["kf_img"].astype(np.float32))
+        of.set_depth(idp, var)
+        w, h = of.size(0)
+        cam4 = of.camera(0)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            o.compute_vbo(of.keyframe_points(0), w, h, cam4, p)
+            n += 1
+        out["cpu_keyframe_vbo_ms"] = round(1e3 * (time.perf_counter() - t0) / n, 4)
+    return out
+
+
 def cpu_tracking_baseline(insts, params, seconds):
     """The oracle's FP32 restatement of the reference CPU path (timing build,
     upstream FP32 sums), one thread, on a bounded sample of the same
@@ -333,6 +369,7 @@ def main():
 
     depth, dinst = depth_leg(ctx, args.steps, args.warmup)
     ctx.set_params(params)
+    export = export_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
 
     out = {
         "metric": "SE3 track frames/s + depth updates/s @640x480, batched vs plain, 1 GPU",
@@ -359,6 +396,7 @@ def main():
         "depth_update_ms": round(depth["ms_per_update"], 4),
         "depth_device_ms_per_update": round(depth["device_ms_per_update"], 4),
         "depth_roofline_gbs": depth["roofline_gbs"],
+        "keyframe_export_640x480": export,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES[dom_name],
                      "kernel": dom_desc,

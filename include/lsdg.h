@@ -386,6 +386,55 @@ lsdg_status lsdg_batch_destroy(lsdg_batch b);
 /* Convenience: create, run once, destroy. */
 lsdg_status lsdg_batch_track(lsdg_ctx ctx, const lsdg_job* jobs, int32_t n, lsdg_job_result* results);
 
+/* ------------------------------------------------------ keyframe export */
+
+/* InputPointDense (lib/Pangolin_IOWrapper/Keyframe.h:16-21): one per pixel. */
+typedef struct lsdg_point_dense {
+  float idepth;
+  float idepth_var;
+  uint8_t color[4];
+} lsdg_point_dense;
+
+/* Keyframe::MyVertex (lib/Pangolin_IOWrapper/Keyframe.h:48-52). */
+typedef struct lsdg_vertex {
+  float point[3];
+  uint8_t color[4];
+} lsdg_vertex;
+
+/* The constants of Keyframe::computeVbo (Keyframe.h:78-82) and the
+ * keyframe's camToWorld scale; my_sparsifyFactor (rand()-based, 1 there) is
+ * not supported. */
+typedef struct lsdg_vbo_params {
+  float scaled_th;           /* my_scaledTH = 1e-3 */
+  float abs_th;              /* my_absTH = 1e-1 */
+  float scale;               /* camToWorld.scale() */
+  int32_t min_near_support;  /* my_minNearSupport = 9 */
+} lsdg_vbo_params;
+
+void lsdg_vbo_params_default(lsdg_vbo_params* p);
+
+/* PangolinOutputIOWrapper::publishKeyframe (PangolinOutputIOWrapper.cpp:56-89):
+ * the keyframe's level-`level` idepth, idepth variance and image as
+ * w_l * h_l InputPointDense records in raster order, into host memory `out`.
+ * The image value goes to all four color bytes, truncated (values the
+ * reference's float-to-byte conversion leaves undefined are clamped to
+ * 0..255, NaN to 0).  LSDG_ESTATE when the frame has no depth (the reference
+ * then publishes an uninitialised buffer). */
+lsdg_status lsdg_keyframe_points(lsdg_frame kf, int32_t level, lsdg_point_dense* out);
+
+/* Keyframe::computeVbo (Keyframe.h:66-142) on InputPointDense records `in`
+ * (host memory, w x h, raster order) of a camera (fx, fy, cx, cy): the
+ * points that pass the variance and near-support filters, back-projected,
+ * color bytes swizzled, alpha 100, in raster order, into host memory `out`
+ * (capacity (w - 2) * (h - 2)); *count receives their number. */
+lsdg_status lsdg_points_vbo(lsdg_ctx ctx, const lsdg_point_dense* in, int32_t w, int32_t h, float fx, float fy,
+                            float cx, float cy, const lsdg_vbo_params* p, lsdg_vertex* out, int32_t* count);
+
+/* Both on the device in one call: publishKeyframe at `level`, then
+ * computeVbo with that level's camera. */
+lsdg_status lsdg_keyframe_vbo(lsdg_frame kf, int32_t level, const lsdg_vbo_params* p, lsdg_vertex* out,
+                              int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

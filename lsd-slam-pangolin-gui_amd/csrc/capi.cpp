@@ -137,6 +137,7 @@ extern "C" lsdg_status lsdg_ctx_destroy(lsdg_ctx c) {
   if (c->d_partials) (void)hipFree(c->d_partials);
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->h_result) (void)hipHostFree(c->h_result);
+  if (c->d_export) (void)hipFree(c->d_export);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return LSDG_OK;

@@ -130,6 +130,17 @@ struct PointsLevel {
 }  // namespace lsdg
 
 // ------------------------------------------------------------ C-ABI objects
+namespace lsdg {
+// Keyframe::computeVbo inputs: geometry, camera inverse (fxi = 1 / fx,
+// cxi = -cx / fx, ...) and the filter constants.
+struct VboArgs {
+  int w, h;
+  float fxi, fyi, cxi, cyi;
+  float scaledTH, absTH, scale;
+  int minNear;
+};
+}  // namespace lsdg
+
 struct lsdg_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -152,6 +163,9 @@ struct lsdg_ctx_s {
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> prof_events;
   lsdg_kernel_stats prof[LSDG_KERNEL_KINDS];
+  // keyframe export scratch (records, block offsets, vertices), grown on demand
+  void* d_export = nullptr;
+  size_t exportCap = 0;
 };
 
 struct lsdg_frame_s {
@@ -247,6 +261,13 @@ hipError_t launch_point_cloud(lsdg_ref r, int level, hipStream_t s);  // fills r
 size_t pyr_batch_bytes(int nf);
 void pyr_batch_tables(lsdg_frame const* fs, const unsigned* need, int nf, void* h, void* d, PyrBatch* out);
 hipError_t launch_pyr_batch(const PyrBatch& b, hipStream_t s);
+
+// keyframe.hip
+hipError_t launch_kf_dense(const float* id, const float* var, const float* img, int n, lsdg_point_dense* out,
+                           hipStream_t s);
+size_t vbo_block_count(int w, int h);
+// blk: vbo_block_count + 1 ints; out: capacity (w - 2) * (h - 2)
+hipError_t launch_vbo(const lsdg_point_dense* in, const VboArgs& a, int* blk, lsdg_vertex* out, hipStream_t s);
 
 // track.hip
 hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,

@@ -137,6 +137,36 @@ class JobResult(C.Structure):
     _fields_ = [("frame_to_ref", C.c_double * 8), ("stats", TrackStats), ("status", C.c_int32)]
 
 
+class PointDense(C.Structure):
+    """InputPointDense (lib/Pangolin_IOWrapper/Keyframe.h:16-21)."""
+    _fields_ = [("idepth", C.c_float), ("idepth_var", C.c_float), ("color", C.c_uint8 * 4)]
+
+
+class Vertex(C.Structure):
+    """Keyframe::MyVertex (lib/Pangolin_IOWrapper/Keyframe.h:48-52)."""
+    _fields_ = [("point", C.c_float * 3), ("color", C.c_uint8 * 4)]
+
+
+class VboParams(C.Structure):
+    _fields_ = [("scaled_th", C.c_float), ("abs_th", C.c_float), ("scale", C.c_float),
+                ("min_near_support", C.c_int32)]
+
+
+POINT_DENSE_DTYPE = np.dtype([("idepth", "<f4"), ("idepth_var", "<f4"), ("color", "u1", (4,))])
+VERTEX_DTYPE = np.dtype([("point", "<f4", (3,)), ("color", "u1", (4,))])
+
+
+def vbo_params(scale=1.0, **kw):
+    """Keyframe::computeVbo's constants (Keyframe.h:78-82) with the keyframe's
+    camToWorld scale; keyword arguments override fields."""
+    p = VboParams()
+    load().lsdg_vbo_params_default(C.byref(p))
+    p.scale = scale
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 def pose7(p):
     return (C.c_double * 7)(*[float(v) for v in list(p)[:7]])
 
@@ -217,6 +247,10 @@ def load():
         "lsdg_batch_run": (i32, [vp, C.POINTER(JobResult)]),
         "lsdg_batch_destroy": (i32, [vp]),
         "lsdg_batch_track": (i32, [vp, C.POINTER(Job), i32, C.POINTER(JobResult)]),
+        "lsdg_vbo_params_default": (None, [C.POINTER(VboParams)]),
+        "lsdg_keyframe_points": (i32, [vp, i32, fp]),
+        "lsdg_points_vbo": (i32, [vp, fp, i32, i32, f32, f32, f32, f32, C.POINTER(VboParams), fp, C.POINTER(i32)]),
+        "lsdg_keyframe_vbo": (i32, [vp, i32, C.POINTER(VboParams), fp, C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -331,6 +365,19 @@ class Context:
                                            gd.ctypes.data, n), "lsdg_energy_grad")
         return e.value, np.array(g[:]), gd
 
+    def points_vbo(self, records, w, h, cam4, params=None):
+        """Keyframe::computeVbo over InputPointDense records (w x h, raster
+        order) of a camera (fx, fy, cx, cy): the kept vertices."""
+        rec = np.ascontiguousarray(records, POINT_DENSE_DTYPE)
+        assert rec.size == w * h
+        out = np.zeros(max(w * h, 1), VERTEX_DTYPE)
+        n = C.c_int32(0)
+        p = params if params is not None else vbo_params()
+        fx, fy, cx, cy = (float(v) for v in cam4)
+        self.check(self.L.lsdg_points_vbo(self.h, rec.ctypes.data, w, h, fx, fy, cx, cy, C.byref(p), out.ctypes.data,
+                                          C.byref(n)), "lsdg_points_vbo")
+        return out[:n.value]
+
     def depthmap(self, cam):
         h = C.c_void_p()
         cc = Camera.of(cam)
@@ -366,6 +413,23 @@ class Frame:
 
     def build_pyramid(self):
         self.ctx.check(self.ctx.L.lsdg_frame_build_pyramid(self.h), "lsdg_frame_build_pyramid")
+
+    def keyframe_points(self, level=0):
+        """publishKeyframe's InputPointDense records of `level` (raster order)."""
+        w, h = self.size(level)
+        out = np.zeros(w * h, POINT_DENSE_DTYPE)
+        self.ctx.check(self.ctx.L.lsdg_keyframe_points(self.h, level, out.ctypes.data), "lsdg_keyframe_points")
+        return out
+
+    def keyframe_vbo(self, level=0, params=None):
+        """publishKeyframe + Keyframe::computeVbo on the device: the kept vertices."""
+        w, h = self.size(level)
+        out = np.zeros(max(w * h, 1), VERTEX_DTYPE)
+        n = C.c_int32(0)
+        p = params if params is not None else vbo_params()
+        self.ctx.check(self.ctx.L.lsdg_keyframe_vbo(self.h, level, C.byref(p), out.ctypes.data, C.byref(n)),
+                       "lsdg_keyframe_vbo")
+        return out[:n.value]
 
     def invalidate(self):
         """Drop the derived pyramid levels; the next reader rebuilds what it needs."""

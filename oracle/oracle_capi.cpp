@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 
+#include "keyframe_export.hpp"
 #include "lsd_oracle.hpp"
 
 namespace orc {
@@ -598,3 +599,36 @@ extern "C" void orc_ldlt6(const double* A, const double* b, double* x) { ldlt_so
 extern "C" void orc_ldlt7(const double* A, const double* b, double* x) { ldlt_solve<7>(A, b, x); }
 extern "C" float orc_interp_f32(const float* img, float x, float y, int w) { return interp(img, x, y, w); }
 
+// PangolinOutputIOWrapper::publishKeyframe's records of a frame's level.
+template <class S>
+static int keyframe_points_t(OFrame* F, int l, lsdg_point_dense* out) {
+  Frame<S>& f = fr<S>(F);
+  if (!f.hasDepth) return LSDG_ESTATE;
+  const size_t n = (size_t)f.w[l] * f.h[l];
+  std::vector<float> id(n), var(n), img(n);
+  for (size_t i = 0; i < n; ++i) {
+    id[i] = (float)f.idepth[l][i];
+    var[i] = (float)f.ivar[l][i];
+    img[i] = (float)f.image[l][i];
+  }
+  publish_points(id.data(), var.data(), img.data(), (int)n, out);
+  return 0;
+}
+extern "C" int orc_keyframe_points(void* f, int level, lsdg_point_dense* out) {
+  OFrame* F = (OFrame*)f;
+  if (level < 0 || level >= PYR) return LSDG_EINVAL;
+  return F->dbl ? keyframe_points_t<double>(F, level, out) : keyframe_points_t<float>(F, level, out);
+}
+extern "C" int orc_compute_vbo(const lsdg_point_dense* in, int w, int h, float fx, float fy, float cx, float cy,
+                               const lsdg_vbo_params* p, lsdg_vertex* out) {
+  return compute_vbo(in, w, h, fx, fy, cx, cy, *p, out);
+}
+extern "C" void orc_frame_camera(void* f, int level, float* k4) {
+  OFrame* F = (OFrame*)f;
+  if (F->dbl) {
+    k4[0] = (float)F->f64.fx[level]; k4[1] = (float)F->f64.fy[level];
+    k4[2] = (float)F->f64.cx[level]; k4[3] = (float)F->f64.cy[level];
+  } else {
+    k4[0] = F->f32.fx[level]; k4[1] = F->f32.fy[level]; k4[2] = F->f32.cx[level]; k4[3] = F->f32.cy[level];
+  }
+}

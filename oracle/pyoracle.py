@@ -69,6 +69,9 @@ def load(fast=False):
         "orc_ldlt6": (None, [dp, dp, dp]),
         "orc_ldlt7": (None, [dp, dp, dp]),
         "orc_interp_f32": (f32, [fp, f32, f32, i32]),
+        "orc_keyframe_points": (i32, [vp, i32, fp]),
+        "orc_compute_vbo": (i32, [fp, i32, i32, f32, f32, f32, f32, C.POINTER(lsdg.VboParams), fp]),
+        "orc_frame_camera": (None, [vp, i32, fp]),
     }
     for n, (r, a) in sig.items():
         fn = getattr(L, n)
@@ -94,6 +97,13 @@ class Oracle:
         self.L = load(fast)
         self.prec = prec
         self.params = params if params is not None else default_params()
+
+    def compute_vbo(self, records, w, h, cam4, params):
+        rec = np.ascontiguousarray(records, lsdg.POINT_DENSE_DTYPE)
+        out = np.zeros(max(w * h, 1), lsdg.VERTEX_DTYPE)
+        fx, fy, cx, cy = (float(v) for v in cam4)
+        n = self.L.orc_compute_vbo(rec.ctypes.data, w, h, fx, fy, cx, cy, C.byref(params), out.ctypes.data)
+        return out[:n]
 
     def frame(self, fid, cam, image):
         img = np.ascontiguousarray(image, np.float32)
@@ -159,6 +169,19 @@ class OFrame:
 
     def size(self, level):
         return self.cam["w"] >> level, self.cam["h"] >> level
+
+    def camera(self, level):
+        k = np.zeros(4, np.float32)
+        self.o.L.orc_frame_camera(self.h, level, k.ctypes.data)
+        return k
+
+    def keyframe_points(self, level=0):
+        w, h = self.size(level)
+        out = np.zeros(w * h, lsdg.POINT_DENSE_DTYPE)
+        rc = self.o.L.orc_keyframe_points(self.h, level, out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"orc_keyframe_points: {rc}")
+        return out
 
     def read(self, what, level=0, f64=False):
         w, h = self.size(level)
