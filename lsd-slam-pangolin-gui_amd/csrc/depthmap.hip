@@ -421,63 +421,50 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
           ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
           lx += incx;
           ly += incy;
-          alive = act;
-          if (act) {
-            float ee = 0.0f;
-            if ((k & 1) == 0) {
-              e1A = val_p2 - rv_p2; ee += e1A * e1A;
-              e2A = val_p1 - rv_p1; ee += e2A * e2A;
-              e3A = val - rv; ee += e3A * e3A;
-              e4A = val_m1 - rv_m1; ee += e4A * e4A;
-              e5A = val_m2 - rv_m2; ee += e5A * e5A;
-            } else {
-              e1B = val_p2 - rv_p2; ee += e1B * e1B;
-              e2B = val_p1 - rv_p1; ee += e2B * e2B;
-              e3B = val - rv; ee += e3B * e3B;
-              e4B = val_m1 - rv_m1; ee += e4B * e4B;
-              e5B = val_m2 - rv_m2; ee += e5B * e5B;
-            }
-            if (loop == j1) {  // the next lane's first step: only the error after a best
-              if (bestWasLast) {
-                sc.errPost = ee;
-                sc.diffPost = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-              }
-              lookValid = true;
-              alive = false;
-            } else {
-              if (ee < sc.best) {
-                sc.second = sc.best;
-                sc.isecond = sc.ibest;
-                sc.best = ee;
-                sc.ibest = loop;
-                sc.errPre = eeLast;
-                sc.diffPre = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-                sc.errPost = -1.0f;
-                sc.diffPost = -1.0f;
-                sc.bx = cpx;
-                sc.by = cpy;
-                bestWasLast = true;
-              } else if (bestWasLast | (ee < sc.second)) {  // else nothing changes (best <= second)
-                if (bestWasLast) {
-                  sc.errPost = ee;
-                  sc.diffPost = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-                  bestWasLast = false;
-                }
-                if (ee < sc.second) {
-                  sc.second = ee;
-                  sc.isecond = loop;
-                }
-              }
-              eeLast = ee;
-              val_m2 = val_m1;
-              val_m1 = val;
-              val = val_p1;
-              val_p1 = val_p2;
-              cpx += incx;
-              cpy += incy;
-              loop++;
-            }
+          // The step in straight-line code: every lane computes it, and the
+          // bookkeeping applies the reference's branches as selects (a lane
+          // that has left the line keeps its state).
+          float ee = 0.0f;
+          if ((k & 1) == 0) {
+            e1A = val_p2 - rv_p2; ee += e1A * e1A;
+            e2A = val_p1 - rv_p1; ee += e2A * e2A;
+            e3A = val - rv; ee += e3A * e3A;
+            e4A = val_m1 - rv_m1; ee += e4A * e4A;
+            e5A = val_m2 - rv_m2; ee += e5A * e5A;
+          } else {
+            e1B = val_p2 - rv_p2; ee += e1B * e1B;
+            e2B = val_p1 - rv_p1; ee += e2B * e2B;
+            e3B = val - rv; ee += e3B * e3B;
+            e4B = val_m1 - rv_m1; ee += e4B * e4B;
+            e5B = val_m2 - rv_m2; ee += e5B * e5B;
           }
+          const float dot = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+          const bool look = act & (loop == j1);  // the next lane's first step: only the error after a best
+          const bool step = act & (loop != j1);
+          const bool nb = step & (ee < sc.best);                 // new best
+          const bool post = act & !nb & bestWasLast;             // the step after the best
+          const bool ns = step & !nb & (ee < sc.second);         // new second
+          sc.second = nb ? sc.best : (ns ? ee : sc.second);
+          sc.isecond = nb ? sc.ibest : (ns ? loop : sc.isecond);
+          sc.best = nb ? ee : sc.best;
+          sc.ibest = nb ? loop : sc.ibest;
+          sc.errPre = nb ? eeLast : sc.errPre;
+          sc.diffPre = nb ? dot : sc.diffPre;
+          sc.errPost = nb ? -1.0f : (post ? ee : sc.errPost);
+          sc.diffPost = nb ? -1.0f : (post ? dot : sc.diffPost);
+          sc.bx = nb ? cpx : sc.bx;
+          sc.by = nb ? cpy : sc.by;
+          bestWasLast = nb | (bestWasLast & !post);
+          lookValid |= look;
+          alive = step;
+          eeLast = ee;
+          val_m2 = val_m1;
+          val_m1 = val;
+          val = val_p1;
+          val_p1 = val_p2;
+          cpx = step ? cpx + incx : cpx;
+          cpy = step ? cpy + incy : cpy;
+          loop += step ? 1 : 0;
         }
         if (!__any(alive)) break;
       }
