@@ -438,7 +438,6 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
             e4B = val_m1 - rv_m1; ee += e4B * e4B;
             e5B = val_m2 - rv_m2; ee += e5B * e5B;
           }
-          const float dot = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
           const bool look = act & (loop == j1);  // the next lane's first step: only the error after a best
           const bool step = act & (loop != j1);
           const bool nb = step & (ee < sc.best);                 // new best
@@ -446,14 +445,19 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
           const bool ns = step & !nb & (ee < sc.second);         // new second
           sc.second = nb ? sc.best : (ns ? ee : sc.second);
           sc.isecond = nb ? sc.ibest : (ns ? loop : sc.isecond);
-          sc.best = nb ? ee : sc.best;
-          sc.ibest = nb ? loop : sc.ibest;
-          sc.errPre = nb ? eeLast : sc.errPre;
-          sc.diffPre = nb ? dot : sc.diffPre;
-          sc.errPost = nb ? -1.0f : (post ? ee : sc.errPost);
-          sc.diffPost = nb ? -1.0f : (post ? dot : sc.diffPost);
-          sc.bx = nb ? cpx : sc.bx;
-          sc.by = nb ? cpy : sc.by;
+          // a best or the step after one is rare once a line is under way:
+          // its records are written under a wave-uniform branch
+          if (__any(nb | post)) {
+            const float dot = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+            sc.best = nb ? ee : sc.best;
+            sc.ibest = nb ? loop : sc.ibest;
+            sc.errPre = nb ? eeLast : sc.errPre;
+            sc.diffPre = nb ? dot : sc.diffPre;
+            sc.errPost = nb ? -1.0f : (post ? ee : sc.errPost);
+            sc.diffPost = nb ? -1.0f : (post ? dot : sc.diffPost);
+            sc.bx = nb ? cpx : sc.bx;
+            sc.by = nb ? cpy : sc.by;
+          }
           bestWasLast = nb | (bestWasLast & !post);
           lookValid |= look;
           alive = step;
