@@ -53,8 +53,8 @@ extern "C" lsdg_status lsdg_depthmap_create(lsdg_ctx c, const lsdg_camera* cam, 
     total += (bytes + 255) & ~(size_t)255;
     return o;
   };
-  size_t ob[2][5];
-  for (int k = 0; k < 2; ++k) {
+  size_t ob[3][5];
+  for (int k = 0; k < 3; ++k) {
     ob[k][0] = take(n);
     ob[k][1] = take(n * 4);
     ob[k][2] = take(n * 4);
@@ -71,7 +71,7 @@ extern "C" lsdg_status lsdg_depthmap_create(lsdg_ctx c, const lsdg_camera* cam, 
     return fail(c, LSDG_ENOMEM, "hipMalloc(depth map)", e);
   }
   char* b = (char*)dm->d_pool;
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 3; ++k) {
     dm->buf[k].valid = (uint8_t*)(b + ob[k][0]);
     dm->buf[k].bl = (int*)(b + ob[k][1]);
     dm->buf[k].nsid = (float*)(b + ob[k][2]);
@@ -112,8 +112,7 @@ extern "C" lsdg_status lsdg_depth_reset(lsdg_depthmap dm) {
   if (!dm) return LSDG_EINVAL;
   lsdg_ctx c = dm->ctx;
   size_t n = (size_t)dm->g.w * dm->g.h;
-  CK(c, hipMemsetAsync(dm->buf[0].valid, 0, n, c->stream));
-  CK(c, hipMemsetAsync(dm->buf[1].valid, 0, n, c->stream));
+  for (int k = 0; k < 3; ++k) CK(c, hipMemsetAsync(dm->buf[k].valid, 0, n, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   dm->kf = nullptr;
   dm->valid = false;
@@ -260,9 +259,8 @@ extern "C" lsdg_status lsdg_depth_update(lsdg_depthmap dm, const lsdg_frame* ref
   ProfMark pm = prof_begin(c, LSDG_KERNEL_DEPTH);
   CK(c, launch_observe(dm, P, dm->h_refs, n, dm->h_refById, (int)byId.size(), offset, 0, n - 1,
                        (float)kf->numFramesTrackedOnThis, (float)kf->numMappedOnThis, c->stream));
-  CK(c, launch_fill_holes(dm, P, c->stream));
-  CK(c, launch_regularize(dm, P, false, P.val_sum_min_for_keep, c->stream));
-  st = set_depth(dm, kf);
+  CK(c, launch_fill_regularize(dm, P, P.val_sum_min_for_keep, kf, c->stream));
+  st = frame_depth_from_level0(kf);
   if (st != LSDG_OK) return st;
   // algorithmic bytes of one update (181 B per pixel: observe 70, fill holes 66,
   // regularize 37, setDepth 8)
@@ -291,8 +289,7 @@ extern "C" lsdg_status lsdg_depth_create_keyframe(lsdg_depthmap dm, lsdg_frame n
   dm->kf = nkf;
   dm->kfReactivated = false;
   CK(c, launch_regularize(dm, P, true, P.val_sum_min_for_keep, c->stream));
-  CK(c, launch_fill_holes(dm, P, c->stream));
-  CK(c, launch_regularize(dm, P, false, P.val_sum_min_for_keep, c->stream));
+  CK(c, launch_fill_regularize(dm, P, P.val_sum_min_for_keep, nullptr, c->stream));
   CK(c, launch_mean_idepth(dm, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   double cnt = dm->h_red[0], sum = dm->h_red[1];
@@ -314,9 +311,8 @@ extern "C" lsdg_status lsdg_depth_finalize_keyframe(lsdg_depthmap dm) {
   if (!dm->valid || !dm->kf) return fail(c, LSDG_ESTATE, "depth map has no active keyframe");
   lsdg_status st0 = frame_require(dm->kf, KF_DATA);
   if (st0 != LSDG_OK) return st0;
-  CK(c, launch_fill_holes(dm, c->P, c->stream));
-  CK(c, launch_regularize(dm, c->P, false, c->P.val_sum_min_for_keep, c->stream));
-  lsdg_status st = set_depth(dm, dm->kf);
+  CK(c, launch_fill_regularize(dm, c->P, c->P.val_sum_min_for_keep, dm->kf, c->stream));
+  lsdg_status st = frame_depth_from_level0(dm->kf);
   if (st != LSDG_OK) return st;
   CK(c, hipStreamSynchronize(c->stream));
   return LSDG_OK;

@@ -5,10 +5,12 @@
 //                  touches its own hypothesis); writes a per-pixel outcome code.
 //                  A gate pass compacts the pixels that search an epipolar
 //                  line into a work list; a stereo pass runs one thread per item.
-//   k_fill_holes   regularizeDepthMapFillHoles; the reference's integral image
-//                  of validity is replaced by the equal integer 5x5 box sum.
-//   k_regularize   regularizeDepthMap(removeOcclusions, validityTH), 5x5 gather.
-//                  Both stage a 16x16 tile plus its 2-pixel halo in LDS.
+//   k_regularize   regularizeDepthMap(removeOcclusions, validityTH), 5x5 gather
+//                  from a 16x16 tile plus its 2-pixel halo in LDS.
+//   k_fill_regularize  regularizeDepthMapFillHoles, regularizeDepthMap(false)
+//                  and optionally Frame::setDepth in one pass over a 4-pixel
+//                  halo; the reference's integral image of validity is
+//                  replaced by the equal integer 5x5 box sum.
 //   k_set_depth    Frame::setDepth (level 0; the pyramid follows).
 //   k_prop_*       propagateDepth as a deterministic scatter: project every
 //                  source, count per destination, scan, bucket, then one
@@ -16,10 +18,14 @@
 //                  which is the order the reference's sequential loop uses.
 //   k_mean_idepth  createKeyFrame's mean inverse depth (FP64, deterministic).
 //
-// Buffers are double-buffered like the reference's current/other maps: a pass
-// reads `in` and writes every pixel of `out` (copy-through for untouched
-// pixels), then the roles swap, so both buffers end up holding exactly what
-// the reference's two buffers hold.  Every per-pixel float expression keeps
+// The hypothesis maps rotate through three buffers: `cur` and `oth` hold what
+// the reference's current and other maps hold, the third is spare.  A pass
+// reads `cur` and writes every pixel of `oth` (copy-through for untouched
+// pixels), then the two swap, as the reference's memcpy-then-update passes
+// leave them.  The fused fill + regularize pass writes the filled map (the
+// reference's other map afterwards: propagateDepth reuses it with only the
+// flags wiped, so its stale fields stay visible) and the regularized map
+// (the new current one) into the two buffers it does not read.  Every per-pixel float expression keeps
 // the CPU restatement's operation order (compiled without FP contraction), so
 // masks, codes and values match it bit for bit.
 #include "lsdg_internal.h"
@@ -826,112 +832,150 @@ __global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P
 }
 
 // ---------------------------------------------------------- fill holes
-// A 16x16 block of pixels and its 2-pixel halo of input hypotheses
-// (validity flag, validity counter, idepth / variance quadruple) in LDS;
-// positions outside the image read as invalid.
-struct HaloTile {
-  static constexpr int N = 20;
+// The tile kernels below issue all of their global loads before their first
+// LDS store (one round trip to memory per block) and read their 5x5
+// neighbourhoods without branches.  They are bound by VALU issue (a wave64
+// instruction takes four cycles on a 16-lane SIMD), so the divisions a 5x5
+// gather repeats per centre pixel are done once per tile element instead:
+// x / var and 1 / var for the fill's inverse-variance mean, and
+// 1 / (var + distFac) for the six distinct squared distances of a 5x5 window
+// for the regularization (each the same IEEE single operation the per-centre
+// loop would do).
+constexpr int TILE_THREADS = 256;
+
+// An N x N tile of hypotheses (validity flag, validity counter, idepth /
+// variance quadruple) in LDS around a 16x16 block of pixels; positions
+// outside the image read as invalid.
+template <int N>
+struct HypTile {
   uint8_t valid[N][N];
   int val[N][N];
   float4 dv[N][N];
 };
 
-__device__ inline void load_halo(HaloTile& t, const HypBuf& in, int w, int h) {
-  const int x0 = blockIdx.x * 16 - 2, y0 = blockIdx.y * 16 - 2;
-  for (int k = threadIdx.x; k < HaloTile::N * HaloTile::N; k += blockDim.x) {
-    const int ty = k / HaloTile::N, tx = k - ty * HaloTile::N;
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= 0 && x < w && y >= 0 && y < h) {
-      const int j = x + y * w;
-      const uint8_t v = in.valid[j];
-      t.valid[ty][tx] = v;
-      t.val[ty][tx] = in.val[j];
-      t.dv[ty][tx] = in.dv[j];
-    } else {
-      t.valid[ty][tx] = 0;
-      t.val[ty][tx] = 0;
-      t.dv[ty][tx] = make_float4(0.f, 1.f, 0.f, 1.f);
-    }
+// squared distances dx^2 + dy^2 of a 5x5 window: 0, 1, 2, 4, 5, 8
+constexpr int DIST_SLOTS = 6;
+__host__ __device__ constexpr int dist_slot(int d2) { return d2 < 3 ? d2 : (d2 == 4 ? 3 : (d2 == 5 ? 4 : 5)); }
+__host__ __device__ constexpr int slot_dist(int k) { return k < 3 ? k : (k == 3 ? 4 : (k == 4 ? 5 : 8)); }
+
+// Per tile element: 1 / (var + d2 * regDistVar) for each squared distance d2.
+template <int N>
+struct IvarTile {
+  float iv[DIST_SLOTS][N][N];
+  __device__ void put(int ty, int tx, float var, float regDistVar) {
+#pragma unroll
+    for (int k = 0; k < DIST_SLOTS; ++k) iv[k][ty][tx] = 1.0f / (var + (float)slot_dist(k) * regDistVar);
   }
-  __syncthreads();
+};
+
+__device__ inline float reg_dist_var(const lsdg_params& P) {
+  return 0.075f * 0.075f * P.depth_smoothing_factor * P.depth_smoothing_factor;
 }
 
-__global__ __launch_bounds__(256) void k_fill_holes(DepthGeom g, HypBuf in, HypBuf out, const float* __restrict__ maxg,
-                                                    lsdg_params P) {
-  __shared__ HaloTile t;
-  const int w = g.w, h = g.h;
-  load_halo(t, in, w, h);
-  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
-  if (x >= w || y >= h) return;
-  const int idx = x + y * w;
-  const int cx = lx + 2, cy = ly + 2;  // this pixel in the tile
-  Hyp d;
-  d.valid = t.valid[cy][cx] != 0;
-  d.bl = in.bl[idx];
-  d.nsid = in.nsid[idx];
-  d.val = t.val[cy][cx];
-  d.dv = t.dv[cy][cx];
-  if (x >= 3 && x < w - 2 && y >= 3 && y < h - 2 && !d.valid && !(maxg[idx] < P.min_abs_grad_decrease)) {
+// Loads the tile whose top-left corner is (x0, y0) in the image: every load
+// first, then the LDS stores.  Positions outside the image load pixel 0 (in
+// bounds) and store the invalid default instead.  With qr set, also stores
+// (idepth / var, 1 / var) per element; with iv set, the reciprocal table.
+template <int N>
+__device__ inline void load_tile(HypTile<N>& t, const HypBuf& in, int w, int h, int x0, int y0,
+                                 float2 (*qr)[N] = nullptr, IvarTile<N>* iv = nullptr, float regDistVar = 0.0f) {
+  constexpr int ITER = (N * N + TILE_THREADS - 1) / TILE_THREADS;
+  uint8_t v[ITER];
+  int c[ITER];
+  float4 q[ITER];
+  bool ok[ITER];
+#pragma unroll
+  for (int i = 0; i < ITER; ++i) {
+    const int k = threadIdx.x + i * TILE_THREADS;
+    const int ty = k / N, tx = k - ty * N;
+    const int x = x0 + tx, y = y0 + ty;
+    ok[i] = k < N * N && x >= 0 && x < w && y >= 0 && y < h;
+    const int j = ok[i] ? x + y * w : 0;
+    v[i] = in.valid[j];
+    c[i] = in.val[j];
+    q[i] = in.dv[j];
+  }
+#pragma unroll
+  for (int i = 0; i < ITER; ++i) {
+    const int k = threadIdx.x + i * TILE_THREADS;
+    if (k < N * N) {
+      const int ty = k / N, tx = k - ty * N;
+      const float4 d = ok[i] ? q[i] : make_float4(0.f, 1.f, 0.f, 1.f);
+      t.valid[ty][tx] = ok[i] ? v[i] : 0;
+      t.val[ty][tx] = ok[i] ? c[i] : 0;
+      t.dv[ty][tx] = d;
+      if (qr) qr[ty][tx] = make_float2(d.x / d.y, 1.0f / d.y);
+      if (iv) iv->put(ty, tx, d.y, regDistVar);
+    }
+  }
+}
+
+// regularizeDepthMapFillHoles for pixel (x, y), whose hypothesis is d (its
+// flag, counter and quadruple taken from tile t at (cx, cy)); the 5x5
+// neighbourhood is read from t and its (idepth / var, 1 / var) from qr.
+template <int N>
+__device__ __forceinline__ Hyp fill_pixel(const HypTile<N>& t, const float2 (*qr)[N], int cx, int cy, int x, int y,
+                                          int w, int h, Hyp d, float mg, const lsdg_params& P) {
+  if (x >= 3 && x < w - 2 && y >= 3 && y < h - 2 && !d.valid && !(mg < P.min_abs_grad_decrease)) {
     // 5x5 box sum of the validity of valid pixels (the reference reads it off
     // an integral image; the integer sum is the same)
     int val = 0;
+#pragma unroll
     for (int yy = -2; yy <= 2; ++yy)
-      for (int xx = -2; xx <= 2; ++xx)
-        if (t.valid[cy + yy][cx + xx]) val += t.val[cy + yy][cx + xx];
+#pragma unroll
+      for (int xx = -2; xx <= 2; ++xx) {
+        const int c = t.val[cy + yy][cx + xx];
+        val += t.valid[cy + yy][cx + xx] ? c : 0;
+      }
     if ((d.bl >= P.min_blacklist && val > P.val_sum_min_for_create) || val > P.val_sum_min_for_unblacklist) {
+      // inverse-variance mean over the valid neighbours, in raster order
       float sumId = 0.0f, sumIv = 0.0f;
+#pragma unroll
       for (int yy = -2; yy <= 2; ++yy)
+#pragma unroll
         for (int xx = -2; xx <= 2; ++xx) {
-          if (!t.valid[cy + yy][cx + xx]) continue;
-          const float4 sv = t.dv[cy + yy][cx + xx];
-          sumId += sv.x / sv.y;
-          sumIv += 1.0f / sv.y;
+          const bool ok = t.valid[cy + yy][cx + xx] != 0;
+          const float2 e = qr[cy + yy][cx + xx];
+          const float a = sumId + e.x, b = sumIv + e.y;
+          sumId = ok ? a : sumId;
+          sumIv = ok ? b : sumIv;
         }
       float obs = sumId / sumIv;
       obs = unzero(obs);
       d = make_hyp(obs, P.var_random_init_initial, 0);
     }
   }
-  store_hyp(out, idx, d);
+  return d;
 }
 
-// ---------------------------------------------------------- regularize
-__global__ __launch_bounds__(256) void k_regularize(DepthGeom g, HypBuf in, HypBuf out, lsdg_params P, int removeOcc,
-                                                    int validityTH) {
-  __shared__ HaloTile t;
-  const int w = g.w, h = g.h;
-  load_halo(t, in, w, h);
-  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
-  if (x >= w || y >= h) return;
-  const int idx = x + y * w;
-  const int cx = lx + 2, cy = ly + 2;
-  Hyp d;
-  d.valid = t.valid[cy][cx] != 0;
-  d.bl = in.bl[idx];
-  d.nsid = in.nsid[idx];
-  d.val = t.val[cy][cx];
-  d.dv = t.dv[cy][cx];
+// regularizeDepthMapRow for pixel (x, y) with hypothesis d; the 5x5
+// neighbourhood is read from tile t around (cx, cy), its reciprocals from
+// iv.  Every neighbour is evaluated and the sums select what the reference's
+// loop would add, in its order (dx outer, dy inner).
+template <int N>
+__device__ __forceinline__ Hyp regularize_pixel(const HypTile<N>& t, const IvarTile<N>& iv, int cx, int cy, int x,
+                                                int y, int w, int h, Hyp d, const lsdg_params& P, int removeOcc,
+                                                int validityTH) {
   if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2 && d.valid) {
-    const float regDistVar = 0.075f * 0.075f * P.depth_smoothing_factor * P.depth_smoothing_factor;
     float sum = 0.0f, val_sum = 0.0f, sumIvar = 0.0f;
     int nOcc = 0, nNot = 0;
+#pragma unroll
     for (int dx = -2; dx <= 2; ++dx)
+#pragma unroll
       for (int dy = -2; dy <= 2; ++dy) {
-        if (!t.valid[cy + dy][cx + dx]) continue;
+        const bool ok = t.valid[cy + dy][cx + dx] != 0;
         const float4 sv = t.dv[cy + dy][cx + dx];
-        float diff = sv.x - d.dv.x;
-        if (P.diff_fac_smoothing * diff * diff > sv.y + d.dv.y) {
-          if (removeOcc && sv.x > d.dv.x) nOcc++;
-          continue;
-        }
-        val_sum += (float)t.val[cy + dy][cx + dx];
-        if (removeOcc) nNot++;
-        float distFac = (float)(dx * dx + dy * dy) * regDistVar;
-        float ivar = 1.0f / (sv.y + distFac);
-        sum += sv.x * ivar;
-        sumIvar += ivar;
+        const int c = t.val[cy + dy][cx + dx];
+        const float diff = sv.x - d.dv.x;
+        const bool rej = P.diff_fac_smoothing * diff * diff > sv.y + d.dv.y;
+        const bool take = ok && !rej;
+        nOcc += (ok && rej && sv.x > d.dv.x) ? 1 : 0;
+        nNot += take ? 1 : 0;
+        const float ivar = iv.iv[dist_slot(dx * dx + dy * dy)][cy + dy][cx + dx];
+        const float vs = val_sum + (float)c, s = sum + sv.x * ivar, si = sumIvar + ivar;
+        val_sum = take ? vs : val_sum;
+        sum = take ? s : sum;
+        sumIvar = take ? si : sumIvar;
       }
     if (val_sum < (float)validityTH) {
       d.valid = false;
@@ -945,21 +989,132 @@ __global__ __launch_bounds__(256) void k_regularize(DepthGeom g, HypBuf in, HypB
       d.dv.w = 1.0f / sumIvar;
     }
   }
-  store_hyp(out, idx, d);
+  return d;
+}
+
+template <int N>
+__device__ inline Hyp tile_hyp(const HypTile<N>& t, int cx, int cy, int bl, float nsid) {
+  Hyp d;
+  d.valid = t.valid[cy][cx] != 0;
+  d.bl = bl;
+  d.nsid = nsid;
+  d.val = t.val[cy][cx];
+  d.dv = t.dv[cy][cx];
+  return d;
+}
+
+// Frame::setDepth for one pixel: the smoothed idepth / variance of a valid
+// hypothesis not behind the camera, else -1 for both.
+__device__ inline void set_depth_pixel(const Hyp& d, int i, float* __restrict__ idepth, float* __restrict__ var) {
+  if (d.valid && (double)d.dv.z >= -0.05) {
+    idepth[i] = unzero(d.dv.z);
+    var[i] = unzero(d.dv.w);
+  } else {
+    idepth[i] = -1.0f;
+    var[i] = -1.0f;
+  }
+}
+
+// ---------------------------------------------------------- regularize
+// regularizeDepthMap(removeOcclusions, validityTH) on a 16x16 block and its
+// 2-pixel halo.
+__global__ __launch_bounds__(TILE_THREADS) void k_regularize(DepthGeom g, HypBuf in, HypBuf out, lsdg_params P,
+                                                             int removeOcc, int validityTH) {
+  __shared__ HypTile<20> t;
+  __shared__ IvarTile<20> iv;
+  const int w = g.w, h = g.h;
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
+  const bool inside = x < w && y < h;
+  const int idx = inside ? x + y * w : 0;
+  const int bl = in.bl[idx];
+  const float nsid = in.nsid[idx];
+  load_tile<20>(t, in, w, h, blockIdx.x * 16 - 2, blockIdx.y * 16 - 2, nullptr, &iv, reg_dist_var(P));
+  __syncthreads();
+  if (!inside) return;
+  const int cx = lx + 2, cy = ly + 2;
+  store_hyp(out, idx,
+            regularize_pixel(t, iv, cx, cy, x, y, w, h, tile_hyp(t, cx, cy, bl, nsid), P, removeOcc, validityTH));
+}
+
+// Fill holes, then regularize (no occlusion removal), in one pass.  A block
+// fills its pixels and their 2-pixel halo from a 4-pixel-halo input tile (a
+// halo pixel's fill depends only on the input, so every block that fills it
+// gets the same result) and regularizes its pixels from the filled tile.  It
+// stores the filled map (`filled`: the reference's other map afterwards) and
+// the regularized one (`out`), and, when kfIdepth is set, Frame::setDepth's
+// level-0 idepth / variance of the regularized map.
+__global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, HypBuf in, HypBuf filled, HypBuf out,
+                                                                  const float* __restrict__ maxg, lsdg_params P,
+                                                                  int validityTH, float* __restrict__ kfIdepth,
+                                                                  float* __restrict__ kfVar) {
+  __shared__ HypTile<24> a;      // input, pixels and a 4-pixel halo
+  __shared__ float2 aqr[24][24];  // its (idepth / var, 1 / var)
+  __shared__ HypTile<20> f;      // filled, pixels and a 2-pixel halo
+  __shared__ IvarTile<20> fiv;   // its reciprocal table
+  __shared__ int fbl[20][20];
+  constexpr int FITER = (20 * 20 + TILE_THREADS - 1) / TILE_THREADS;
+  const int w = g.w, h = g.h;
+  const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int x = bx + lx, y = by + ly;
+  const bool inside = x < w && y < h;
+  const int idx = inside ? x + y * w : 0;
+  const float nsid = in.nsid[idx];
+  // blacklist counter and keyframe gradient of the filled-tile positions
+  int fb[FITER];
+  float fm[FITER];
+#pragma unroll
+  for (int i = 0; i < FITER; ++i) {
+    const int k = threadIdx.x + i * TILE_THREADS;
+    const int ty = k / 20, tx = k - ty * 20;
+    const int px = bx - 2 + tx, py = by - 2 + ty;
+    const bool ok = k < 20 * 20 && px >= 0 && px < w && py >= 0 && py < h;
+    const int j = ok ? px + py * w : 0;
+    fb[i] = in.bl[j];
+    fm[i] = maxg[j];
+  }
+  load_tile(a, in, w, h, bx - 4, by - 4, aqr);
+  __syncthreads();
+  const float rdv = reg_dist_var(P);
+#pragma unroll
+  for (int i = 0; i < FITER; ++i) {
+    const int k = threadIdx.x + i * TILE_THREADS;
+    if (k >= 20 * 20) break;
+    const int ty = k / 20, tx = k - ty * 20;
+    const int px = bx - 2 + tx, py = by - 2 + ty;
+    Hyp d;
+    if (px >= 0 && px < w && py >= 0 && py < h) {
+      d = fill_pixel(a, aqr, tx + 2, ty + 2, px, py, w, h, tile_hyp(a, tx + 2, ty + 2, fb[i], 0.0f), fm[i], P);
+    } else {
+      d.valid = false;
+      d.bl = 0;
+      d.val = 0;
+      d.dv = make_float4(0.f, 1.f, 0.f, 1.f);
+    }
+    f.valid[ty][tx] = d.valid ? 1 : 0;
+    f.val[ty][tx] = d.val;
+    f.dv[ty][tx] = d.dv;
+    fbl[ty][tx] = d.bl;
+    fiv.put(ty, tx, d.dv.y, rdv);
+  }
+  __syncthreads();
+  if (!inside) return;
+  const int cx = lx + 2, cy = ly + 2;
+  // a hypothesis created by the fill has next-stereo id 0
+  const bool created = f.valid[cy][cx] && !a.valid[cy + 2][cx + 2];
+  const Hyp d = tile_hyp(f, cx, cy, fbl[cy][cx], created ? 0.0f : nsid);
+  store_hyp(filled, idx, d);
+  const Hyp r = regularize_pixel(f, fiv, cx, cy, x, y, w, h, d, P, 0, validityTH);
+  store_hyp(out, idx, r);
+  if (kfIdepth) set_depth_pixel(r, idx, kfIdepth, kfVar);
 }
 
 // -------------------------------------------------------- Frame::setDepth
 __global__ void k_set_depth(HypBuf cur, int n, float* __restrict__ idepth, float* __restrict__ var) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float4 dv = cur.dv[i];
-  if (cur.valid[i] && (double)dv.z >= -0.05) {
-    idepth[i] = unzero(dv.z);
-    var[i] = unzero(dv.w);
-  } else {
-    idepth[i] = -1.0f;
-    var[i] = -1.0f;
-  }
+  set_depth_pixel(load_hyp(cur, i), i, idepth, var);
 }
 
 // ------------------------------------------------------------ propagate
@@ -1274,18 +1429,21 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
   return hipGetLastError();
 }
 
-hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s) {
-  int in = dm->cur, out = 1 - dm->cur;
-  k_fill_holes<<<grid16(dm->g.w, dm->g.h), dim3(256), 0, s>>>(dm->g, dm->buf[in], dm->buf[out], dm->kf->d_maxg[0], P);
-  dm->cur = out;
+hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th, hipStream_t s) {
+  k_regularize<<<grid16(dm->g.w, dm->g.h), dim3(TILE_THREADS), 0, s>>>(dm->g, dm->buf[dm->cur], dm->buf[dm->oth], P,
+                                                                       removeOcclusions ? 1 : 0, th);
+  std::swap(dm->cur, dm->oth);
   return hipGetLastError();
 }
 
-hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th, hipStream_t s) {
-  int in = dm->cur, out = 1 - dm->cur;
-  k_regularize<<<grid16(dm->g.w, dm->g.h), dim3(256), 0, s>>>(dm->g, dm->buf[in], dm->buf[out], P,
-                                                               removeOcclusions ? 1 : 0, th);
-  dm->cur = out;
+hipError_t launch_fill_regularize(lsdg_depthmap dm, const lsdg_params& P, int th, lsdg_frame setDepth,
+                                  hipStream_t s) {
+  const int in = dm->cur, fill = dm->oth, reg = 3 - dm->cur - dm->oth;
+  k_fill_regularize<<<grid16(dm->g.w, dm->g.h), dim3(TILE_THREADS), 0, s>>>(
+      dm->g, dm->buf[in], dm->buf[fill], dm->buf[reg], dm->kf->d_maxg[0], P, th,
+      setDepth ? setDepth->d_idepth[0] : nullptr, setDepth ? setDepth->d_ivar[0] : nullptr);
+  dm->oth = fill;
+  dm->cur = reg;
   return hipGetLastError();
 }
 
@@ -1318,9 +1476,9 @@ hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame n
   k_scan_top<<<dim3(1), dim3(1024), 0, s>>>(dm->d_scanblk, nb);
   k_scan_apply<<<dim3(nb), dim3(256), 0, s>>>(dm->d_count, n, dm->d_scanblk);
   k_prop_bucket<<<dim3(nb), dim3(256), 0, s>>>(dm->d_dest, dm->d_count, dm->d_fill, dm->d_bucket, n);
-  int in = dm->cur, out = 1 - dm->cur;
-  k_prop_fold<<<dim3(nb), dim3(256), 0, s>>>(dm->buf[in], dm->buf[out], dm->d_newIV, dm->d_count, dm->d_bucket, n, P);
-  dm->cur = out;
+  k_prop_fold<<<dim3(nb), dim3(256), 0, s>>>(dm->buf[dm->cur], dm->buf[dm->oth], dm->d_newIV, dm->d_count,
+                                             dm->d_bucket, n, P);
+  std::swap(dm->cur, dm->oth);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/lsdg.h"
@@ -217,8 +218,10 @@ struct lsdg_depthmap_s {
   lsdg_ctx ctx = nullptr;
   lsdg::DepthGeom g;
   void* d_pool = nullptr;
-  lsdg::HypBuf buf[2];
-  int cur = 0;
+  // hypothesis maps: buf[cur] and buf[oth] are the reference's current and
+  // other maps; the third is the spare the fused fill + regularize pass writes
+  lsdg::HypBuf buf[3];
+  int cur = 0, oth = 1;
   uint8_t* d_obs = nullptr;
   lsdg_frame kf = nullptr;
   bool kfReactivated = false;
@@ -287,9 +290,13 @@ size_t obs_work_slots(int w, int h);
 hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* h_refs, int nRefs,
                           const int* h_refById, int nRefById, int refByIdOffset, int oldestIdx, int newestIdx,
                           float numTracked, float numMapped, hipStream_t s);
-hipError_t launch_fill_holes(lsdg_depthmap dm, const lsdg_params& P, hipStream_t s);  // swaps buffers
 hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool removeOcclusions, int th,
                              hipStream_t s);  // swaps buffers
+// regularizeDepthMapFillHoles then regularizeDepthMap(false, th) in one pass,
+// leaving the current and other maps as the two passes would; with setDepth
+// non-null, also launch_set_depth(setDepth) (level 0 only)
+hipError_t launch_fill_regularize(lsdg_depthmap dm, const lsdg_params& P, int th, lsdg_frame setDepth,
+                                  hipStream_t s);
 hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s);
 hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame nkf, const Xf& oldToNew,
                             const uint8_t* d_wasGood, hipStream_t s);  // swaps buffers
