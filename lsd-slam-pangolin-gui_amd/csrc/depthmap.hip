@@ -837,125 +837,126 @@ __global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P
 // neighbourhoods without branches.  They are bound by VALU issue (a wave64
 // instruction takes four cycles on a 16-lane SIMD), so the divisions a 5x5
 // gather repeats per centre pixel are done once per tile element instead:
-// x / var and 1 / var for the fill's inverse-variance mean, and
+// idepth / var and 1 / var for the fill's inverse-variance mean, and
 // 1 / (var + distFac) for the six distinct squared distances of a 5x5 window
 // for the regularization (each the same IEEE single operation the per-centre
-// loop would do).
+// loop would do).  The tiles hold only what the neighbour reads use; a
+// thread reads its own pixel's full hypothesis from global memory.
 constexpr int TILE_THREADS = 256;
-
-// An N x N tile of hypotheses (validity flag, validity counter, idepth /
-// variance quadruple) in LDS around a 16x16 block of pixels; positions
-// outside the image read as invalid.
-template <int N>
-struct HypTile {
-  uint8_t valid[N][N];
-  int val[N][N];
-  float4 dv[N][N];
-};
 
 // squared distances dx^2 + dy^2 of a 5x5 window: 0, 1, 2, 4, 5, 8
 constexpr int DIST_SLOTS = 6;
 __host__ __device__ constexpr int dist_slot(int d2) { return d2 < 3 ? d2 : (d2 == 4 ? 3 : (d2 == 5 ? 4 : 5)); }
 __host__ __device__ constexpr int slot_dist(int k) { return k < 3 ? k : (k == 3 ? 4 : (k == 4 ? 5 : 8)); }
 
-// Per tile element: 1 / (var + d2 * regDistVar) for each squared distance d2.
-template <int N>
-struct IvarTile {
-  float iv[DIST_SLOTS][N][N];
-  __device__ void put(int ty, int tx, float var, float regDistVar) {
-#pragma unroll
-    for (int k = 0; k < DIST_SLOTS; ++k) iv[k][ty][tx] = 1.0f / (var + (float)slot_dist(k) * regDistVar);
-  }
-};
-
 __device__ inline float reg_dist_var(const lsdg_params& P) {
   return 0.075f * 0.075f * P.depth_smoothing_factor * P.depth_smoothing_factor;
 }
 
-// Loads the tile whose top-left corner is (x0, y0) in the image: every load
-// first, then the LDS stores.  Positions outside the image load pixel 0 (in
-// bounds) and store the invalid default instead.  With qr set, also stores
-// (idepth / var, 1 / var) per element; with iv set, the reciprocal table.
+// Neighbours of a regularization: validity flag, validity counter (as the
+// float the reference's sum adds), idepth / variance, and per squared
+// distance 1 / (var + d2 * regDistVar).  Positions outside the image are
+// invalid.
 template <int N>
-__device__ inline void load_tile(HypTile<N>& t, const HypBuf& in, int w, int h, int x0, int y0,
-                                 float2 (*qr)[N] = nullptr, IvarTile<N>* iv = nullptr, float regDistVar = 0.0f) {
-  constexpr int ITER = (N * N + TILE_THREADS - 1) / TILE_THREADS;
-  uint8_t v[ITER];
-  int c[ITER];
-  float4 q[ITER];
-  bool ok[ITER];
+struct RegTile {
+  uint8_t valid[N][N];
+  float val[N][N];
+  float2 xy[N][N];
+  float iv[DIST_SLOTS][N][N];
+  __device__ void put(int ty, int tx, bool v, int c, float2 d, float regDistVar) {
+    valid[ty][tx] = v ? 1 : 0;
+    val[ty][tx] = (float)c;
+    xy[ty][tx] = d;
 #pragma unroll
-  for (int i = 0; i < ITER; ++i) {
-    const int k = threadIdx.x + i * TILE_THREADS;
-    const int ty = k / N, tx = k - ty * N;
-    const int x = x0 + tx, y = y0 + ty;
-    ok[i] = k < N * N && x >= 0 && x < w && y >= 0 && y < h;
-    const int j = ok[i] ? x + y * w : 0;
-    v[i] = in.valid[j];
-    c[i] = in.val[j];
-    q[i] = in.dv[j];
+    for (int k = 0; k < DIST_SLOTS; ++k) iv[k][ty][tx] = 1.0f / (d.y + (float)slot_dist(k) * regDistVar);
   }
-#pragma unroll
-  for (int i = 0; i < ITER; ++i) {
-    const int k = threadIdx.x + i * TILE_THREADS;
-    if (k < N * N) {
-      const int ty = k / N, tx = k - ty * N;
-      const float4 d = ok[i] ? q[i] : make_float4(0.f, 1.f, 0.f, 1.f);
-      t.valid[ty][tx] = ok[i] ? v[i] : 0;
-      t.val[ty][tx] = ok[i] ? c[i] : 0;
-      t.dv[ty][tx] = d;
-      if (qr) qr[ty][tx] = make_float2(d.x / d.y, 1.0f / d.y);
-      if (iv) iv->put(ty, tx, d.y, regDistVar);
-    }
-  }
-}
+};
 
-// regularizeDepthMapFillHoles for pixel (x, y), whose hypothesis is d (its
-// flag, counter and quadruple taken from tile t at (cx, cy)); the 5x5
-// neighbourhood is read from t and its (idepth / var, 1 / var) from qr.
+// Neighbours of a hole fill: validity flag, validity counter, idepth /
+// variance, and (idepth / var, 1 / var).
 template <int N>
-__device__ __forceinline__ Hyp fill_pixel(const HypTile<N>& t, const float2 (*qr)[N], int cx, int cy, int x, int y,
-                                          int w, int h, Hyp d, float mg, const lsdg_params& P) {
-  if (x >= 3 && x < w - 2 && y >= 3 && y < h - 2 && !d.valid && !(mg < P.min_abs_grad_decrease)) {
-    // 5x5 box sum of the validity of valid pixels (the reference reads it off
-    // an integral image; the integer sum is the same)
-    int val = 0;
-#pragma unroll
-    for (int yy = -2; yy <= 2; ++yy)
-#pragma unroll
-      for (int xx = -2; xx <= 2; ++xx) {
-        const int c = t.val[cy + yy][cx + xx];
-        val += t.valid[cy + yy][cx + xx] ? c : 0;
-      }
-    if ((d.bl >= P.min_blacklist && val > P.val_sum_min_for_create) || val > P.val_sum_min_for_unblacklist) {
-      // inverse-variance mean over the valid neighbours, in raster order
-      float sumId = 0.0f, sumIv = 0.0f;
-#pragma unroll
-      for (int yy = -2; yy <= 2; ++yy)
-#pragma unroll
-        for (int xx = -2; xx <= 2; ++xx) {
-          const bool ok = t.valid[cy + yy][cx + xx] != 0;
-          const float2 e = qr[cy + yy][cx + xx];
-          const float a = sumId + e.x, b = sumIv + e.y;
-          sumId = ok ? a : sumId;
-          sumIv = ok ? b : sumIv;
-        }
-      float obs = sumId / sumIv;
-      obs = unzero(obs);
-      d = make_hyp(obs, P.var_random_init_initial, 0);
-    }
-  }
+struct FillTile {
+  uint8_t valid[N][N];
+  int val[N][N];
+  float2 xy[N][N];
+  float2 qr[N][N];
+};
+
+// One pixel's hypothesis fields, loaded from global memory.
+struct PixelLoad {
+  uint8_t valid;
+  int bl;
+  float nsid;
+  int val;
+  float4 dv;
+};
+__device__ inline PixelLoad load_pixel(const HypBuf& b, int i) {
+  return PixelLoad{b.valid[i], b.bl[i], b.nsid[i], b.val[i], b.dv[i]};
+}
+__device__ inline Hyp as_hyp(const PixelLoad& p) {
+  Hyp d;
+  d.valid = p.valid != 0;
+  d.bl = p.bl;
+  d.nsid = p.nsid;
+  d.val = p.val;
+  d.dv = p.dv;
   return d;
 }
 
-// regularizeDepthMapRow for pixel (x, y) with hypothesis d; the 5x5
-// neighbourhood is read from tile t around (cx, cy), its reciprocals from
-// iv.  Every neighbour is evaluated and the sums select what the reference's
-// loop would add, in its order (dx outer, dy inner).
+// Tile position k of an N x N tile with top-left corner (x0, y0): whether it
+// lies in the image, and the pixel index to load (pixel 0, in bounds, if not).
 template <int N>
-__device__ __forceinline__ Hyp regularize_pixel(const HypTile<N>& t, const IvarTile<N>& iv, int cx, int cy, int x,
-                                                int y, int w, int h, Hyp d, const lsdg_params& P, int removeOcc,
-                                                int validityTH) {
+__device__ inline bool tile_pos(int k, int x0, int y0, int w, int h, int& ty, int& tx, int& j) {
+  ty = k / N;
+  tx = k - ty * N;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool ok = k < N * N && x >= 0 && x < w && y >= 0 && y < h;
+  j = ok ? x + y * w : 0;
+  return ok;
+}
+
+// regularizeDepthMapFillHoles at pixel (x, y) (tile position (cx, cy)) with
+// validity flag `valid`, blacklist counter bl and keyframe gradient mg:
+// whether a hypothesis is created there, and its idepth.
+template <int N>
+__device__ __forceinline__ bool fill_creates(const FillTile<N>& t, int cx, int cy, int x, int y, int w, int h,
+                                             bool valid, int bl, float mg, const lsdg_params& P, float& obs) {
+  if (!(x >= 3 && x < w - 2 && y >= 3 && y < h - 2 && !valid && !(mg < P.min_abs_grad_decrease))) return false;
+  // 5x5 box sum of the validity of valid pixels (the reference reads it off
+  // an integral image; the integer sum is the same)
+  int val = 0;
+#pragma unroll
+  for (int yy = -2; yy <= 2; ++yy)
+#pragma unroll
+    for (int xx = -2; xx <= 2; ++xx) {
+      const int c = t.val[cy + yy][cx + xx];
+      val += t.valid[cy + yy][cx + xx] ? c : 0;
+    }
+  if (!((bl >= P.min_blacklist && val > P.val_sum_min_for_create) || val > P.val_sum_min_for_unblacklist))
+    return false;
+  // inverse-variance mean over the valid neighbours, in raster order
+  float sumId = 0.0f, sumIv = 0.0f;
+#pragma unroll
+  for (int yy = -2; yy <= 2; ++yy)
+#pragma unroll
+    for (int xx = -2; xx <= 2; ++xx) {
+      const bool ok = t.valid[cy + yy][cx + xx] != 0;
+      const float2 e = t.qr[cy + yy][cx + xx];
+      const float a = sumId + e.x, b = sumIv + e.y;
+      sumId = ok ? a : sumId;
+      sumIv = ok ? b : sumIv;
+    }
+  obs = unzero(sumId / sumIv);
+  return true;
+}
+
+// regularizeDepthMapRow for pixel (x, y) with hypothesis d; the 5x5
+// neighbourhood is read from tile t around (cx, cy).  Every neighbour is
+// evaluated and the sums select what the reference's loop would add, in its
+// order (dx outer, dy inner).
+template <int N>
+__device__ __forceinline__ Hyp regularize_pixel(const RegTile<N>& t, int cx, int cy, int x, int y, int w, int h, Hyp d,
+                                                const lsdg_params& P, int removeOcc, int validityTH) {
   if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2 && d.valid) {
     float sum = 0.0f, val_sum = 0.0f, sumIvar = 0.0f;
     int nOcc = 0, nNot = 0;
@@ -964,15 +965,14 @@ __device__ __forceinline__ Hyp regularize_pixel(const HypTile<N>& t, const IvarT
 #pragma unroll
       for (int dy = -2; dy <= 2; ++dy) {
         const bool ok = t.valid[cy + dy][cx + dx] != 0;
-        const float4 sv = t.dv[cy + dy][cx + dx];
-        const int c = t.val[cy + dy][cx + dx];
+        const float2 sv = t.xy[cy + dy][cx + dx];
         const float diff = sv.x - d.dv.x;
         const bool rej = P.diff_fac_smoothing * diff * diff > sv.y + d.dv.y;
         const bool take = ok && !rej;
         nOcc += (ok && rej && sv.x > d.dv.x) ? 1 : 0;
         nNot += take ? 1 : 0;
-        const float ivar = iv.iv[dist_slot(dx * dx + dy * dy)][cy + dy][cx + dx];
-        const float vs = val_sum + (float)c, s = sum + sv.x * ivar, si = sumIvar + ivar;
+        const float ivar = t.iv[dist_slot(dx * dx + dy * dy)][cy + dy][cx + dx];
+        const float vs = val_sum + t.val[cy + dy][cx + dx], s = sum + sv.x * ivar, si = sumIvar + ivar;
         val_sum = take ? vs : val_sum;
         sum = take ? s : sum;
         sumIvar = take ? si : sumIvar;
@@ -989,17 +989,6 @@ __device__ __forceinline__ Hyp regularize_pixel(const HypTile<N>& t, const IvarT
       d.dv.w = 1.0f / sumIvar;
     }
   }
-  return d;
-}
-
-template <int N>
-__device__ inline Hyp tile_hyp(const HypTile<N>& t, int cx, int cy, int bl, float nsid) {
-  Hyp d;
-  d.valid = t.valid[cy][cx] != 0;
-  d.bl = bl;
-  d.nsid = nsid;
-  d.val = t.val[cy][cx];
-  d.dv = t.dv[cy][cx];
   return d;
 }
 
@@ -1020,21 +1009,38 @@ __device__ inline void set_depth_pixel(const Hyp& d, int i, float* __restrict__ 
 // 2-pixel halo.
 __global__ __launch_bounds__(TILE_THREADS) void k_regularize(DepthGeom g, HypBuf in, HypBuf out, lsdg_params P,
                                                              int removeOcc, int validityTH) {
-  __shared__ HypTile<20> t;
-  __shared__ IvarTile<20> iv;
+  __shared__ RegTile<20> t;
+  constexpr int ITER = (20 * 20 + TILE_THREADS - 1) / TILE_THREADS;
   const int w = g.w, h = g.h;
+  const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
   const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
+  const int x = bx + lx, y = by + ly;
   const bool inside = x < w && y < h;
   const int idx = inside ? x + y * w : 0;
-  const int bl = in.bl[idx];
-  const float nsid = in.nsid[idx];
-  load_tile<20>(t, in, w, h, blockIdx.x * 16 - 2, blockIdx.y * 16 - 2, nullptr, &iv, reg_dist_var(P));
+  const PixelLoad self = load_pixel(in, idx);
+  uint8_t v[ITER];
+  int c[ITER];
+  float4 q[ITER];
+  bool ok[ITER];
+#pragma unroll
+  for (int i = 0; i < ITER; ++i) {
+    int ty, tx, j;
+    ok[i] = tile_pos<20>(threadIdx.x + i * TILE_THREADS, bx - 2, by - 2, w, h, ty, tx, j);
+    v[i] = in.valid[j];
+    c[i] = in.val[j];
+    q[i] = in.dv[j];
+  }
+  const float rdv = reg_dist_var(P);
+#pragma unroll
+  for (int i = 0; i < ITER; ++i) {
+    const int k = threadIdx.x + i * TILE_THREADS;
+    if (k >= 20 * 20) break;
+    const int ty = k / 20, tx = k - ty * 20;
+    t.put(ty, tx, ok[i] && v[i], ok[i] ? c[i] : 0, ok[i] ? make_float2(q[i].x, q[i].y) : make_float2(0.f, 1.f), rdv);
+  }
   __syncthreads();
   if (!inside) return;
-  const int cx = lx + 2, cy = ly + 2;
-  store_hyp(out, idx,
-            regularize_pixel(t, iv, cx, cy, x, y, w, h, tile_hyp(t, cx, cy, bl, nsid), P, removeOcc, validityTH));
+  store_hyp(out, idx, regularize_pixel(t, lx + 2, ly + 2, x, y, w, h, as_hyp(self), P, removeOcc, validityTH));
 }
 
 // Fill holes, then regularize (no occlusion removal), in one pass.  A block
@@ -1048,11 +1054,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, H
                                                                   const float* __restrict__ maxg, lsdg_params P,
                                                                   int validityTH, float* __restrict__ kfIdepth,
                                                                   float* __restrict__ kfVar) {
-  __shared__ HypTile<24> a;      // input, pixels and a 4-pixel halo
-  __shared__ float2 aqr[24][24];  // its (idepth / var, 1 / var)
-  __shared__ HypTile<20> f;      // filled, pixels and a 2-pixel halo
-  __shared__ IvarTile<20> fiv;   // its reciprocal table
-  __shared__ int fbl[20][20];
+  __shared__ FillTile<24> a;  // input, pixels and a 4-pixel halo
+  __shared__ RegTile<20> f;   // filled, pixels and a 2-pixel halo
+  constexpr int AITER = (24 * 24 + TILE_THREADS - 1) / TILE_THREADS;
   constexpr int FITER = (20 * 20 + TILE_THREADS - 1) / TILE_THREADS;
   const int w = g.w, h = g.h;
   const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
@@ -1060,52 +1064,71 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, H
   const int x = bx + lx, y = by + ly;
   const bool inside = x < w && y < h;
   const int idx = inside ? x + y * w : 0;
-  const float nsid = in.nsid[idx];
-  // blacklist counter and keyframe gradient of the filled-tile positions
+  // every global load first: this pixel, the input tile, and the blacklist
+  // counter and keyframe gradient of the filled-tile positions
+  const PixelLoad self = load_pixel(in, idx);
+  uint8_t av[AITER];
+  int ac[AITER];
+  float4 aq[AITER];
+  bool aok[AITER];
+#pragma unroll
+  for (int i = 0; i < AITER; ++i) {
+    int ty, tx, j;
+    aok[i] = tile_pos<24>(threadIdx.x + i * TILE_THREADS, bx - 4, by - 4, w, h, ty, tx, j);
+    av[i] = in.valid[j];
+    ac[i] = in.val[j];
+    aq[i] = in.dv[j];
+  }
   int fb[FITER];
   float fm[FITER];
 #pragma unroll
   for (int i = 0; i < FITER; ++i) {
-    const int k = threadIdx.x + i * TILE_THREADS;
-    const int ty = k / 20, tx = k - ty * 20;
-    const int px = bx - 2 + tx, py = by - 2 + ty;
-    const bool ok = k < 20 * 20 && px >= 0 && px < w && py >= 0 && py < h;
-    const int j = ok ? px + py * w : 0;
+    int ty, tx, j;
+    tile_pos<20>(threadIdx.x + i * TILE_THREADS, bx - 2, by - 2, w, h, ty, tx, j);
     fb[i] = in.bl[j];
     fm[i] = maxg[j];
   }
-  load_tile(a, in, w, h, bx - 4, by - 4, aqr);
+#pragma unroll
+  for (int i = 0; i < AITER; ++i) {
+    const int k = threadIdx.x + i * TILE_THREADS;
+    if (k >= 24 * 24) break;
+    const int ty = k / 24, tx = k - ty * 24;
+    const float2 d = aok[i] ? make_float2(aq[i].x, aq[i].y) : make_float2(0.f, 1.f);
+    a.valid[ty][tx] = aok[i] ? av[i] : 0;
+    a.val[ty][tx] = aok[i] ? ac[i] : 0;
+    a.xy[ty][tx] = d;
+    a.qr[ty][tx] = make_float2(d.x / d.y, 1.0f / d.y);
+  }
   __syncthreads();
   const float rdv = reg_dist_var(P);
 #pragma unroll
   for (int i = 0; i < FITER; ++i) {
     const int k = threadIdx.x + i * TILE_THREADS;
     if (k >= 20 * 20) break;
-    const int ty = k / 20, tx = k - ty * 20;
-    const int px = bx - 2 + tx, py = by - 2 + ty;
-    Hyp d;
-    if (px >= 0 && px < w && py >= 0 && py < h) {
-      d = fill_pixel(a, aqr, tx + 2, ty + 2, px, py, w, h, tile_hyp(a, tx + 2, ty + 2, fb[i], 0.0f), fm[i], P);
-    } else {
-      d.valid = false;
-      d.bl = 0;
-      d.val = 0;
-      d.dv = make_float4(0.f, 1.f, 0.f, 1.f);
+    int ty, tx, j;
+    const bool ok = tile_pos<20>(k, bx - 2, by - 2, w, h, ty, tx, j);
+    bool v = ok && a.valid[ty + 2][tx + 2];
+    int c = ok ? a.val[ty + 2][tx + 2] : 0;
+    float2 e = a.xy[ty + 2][tx + 2];
+    float obs = 0.0f;
+    if (ok && fill_creates(a, tx + 2, ty + 2, bx - 2 + tx, by - 2 + ty, w, h, v, fb[i], fm[i], P, obs)) {
+      v = true;
+      c = 0;
+      e = make_float2(obs, P.var_random_init_initial);
     }
-    f.valid[ty][tx] = d.valid ? 1 : 0;
-    f.val[ty][tx] = d.val;
-    f.dv[ty][tx] = d.dv;
-    fbl[ty][tx] = d.bl;
-    fiv.put(ty, tx, d.dv.y, rdv);
+    f.put(ty, tx, v, c, e, rdv);
   }
   __syncthreads();
   if (!inside) return;
   const int cx = lx + 2, cy = ly + 2;
-  // a hypothesis created by the fill has next-stereo id 0
-  const bool created = f.valid[cy][cx] && !a.valid[cy + 2][cx + 2];
-  const Hyp d = tile_hyp(f, cx, cy, fbl[cy][cx], created ? 0.0f : nsid);
+  // the filled hypothesis: DepthMapPixelHypothesis(obs, VAR_RANDOM_INIT_INITIAL, 0) where created
+  Hyp d = as_hyp(self);
+  if (f.valid[cy][cx] && !d.valid) {
+    const float2 e = f.xy[cy][cx];
+    d = make_hyp(e.x, e.y, 0);
+  }
   store_hyp(filled, idx, d);
-  const Hyp r = regularize_pixel(f, fiv, cx, cy, x, y, w, h, d, P, 0, validityTH);
+  const Hyp r = regularize_pixel(f, cx, cy, x, y, w, h, d, P, 0, validityTH);
   store_hyp(out, idx, r);
   if (kfIdepth) set_depth_pixel(r, idx, kfIdepth, kfVar);
 }
