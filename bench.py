@@ -198,6 +198,51 @@ def export_leg(ctx, steps, warmup, cpu_seconds):
     return out
 
 
+def ingest_leg(ctx, steps, warmup, cpu_seconds):
+    """InputThread's ingest of one 640x480 8-bit camera image (FOV lens,
+    omega 0.92) into a 640x480 pinhole frame: upload, undistortion, float
+    conversion and image pyramid in one launch, into a reused frame and into
+    a new frame per image (as upstream allocates); the oracle's undistort +
+    Frame::buildImage beside it."""
+    in_k, out_k, omega = (330.0, 330.0, 319.5, 239.5), (270.0, 270.0, 319.5, 239.5), 0.92
+    cam = dict(w=640, h=480, fx=out_k[0], fy=out_k[1], cx=out_k[2], cy=out_k[3])
+    src, _ = synth.Scene(21).render(synth.IDENTITY7, synth.camera(640, 480), 2.0, 21)
+    table = lsdg.fov_undistort_table(in_k, omega, (640, 480), out_k, (640, 480))
+    und = ctx.undistorter((640, 480), table)
+    for _ in range(warmup):
+        ctx.frame_undistorted(0, cam, und, src).destroy()
+    ctx.sync()
+    t_new = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        f = ctx.frame_undistorted(0, cam, und, src)
+        t_new += time.perf_counter() - t0
+        f.destroy()
+    # a stream of images into one reused frame (no allocation per image)
+    f = ctx.frame_undistorted(0, cam, und, src)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        f.ingest(k, und, src)
+    t_reuse = time.perf_counter() - t0
+    achieved, ks = kernel_roofline(ctx, lambda: f.ingest(0, und, src), lsdg.KERNEL_INGEST)
+    f.destroy()
+    und.close()
+    out = {"ingest_call_ms": round(1e3 * t_reuse / steps, 4), "ingest_new_frame_call_ms": round(1e3 * t_new / steps, 4),
+           "ingest_kernel_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 3),
+           "ingest_kernel_gbs": round(achieved, 2)}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(0, fast=True)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            of = o.frame(0, cam, o.undistort(src, table).astype(np.float32))
+            of.read(lsdg.WHAT_IMAGE, 4)
+            n += 1
+        out["cpu_ingest_ms"] = round(1e3 * (time.perf_counter() - t0) / n, 4)
+    return out
+
+
 def cpu_tracking_baseline(insts, params, seconds):
     """The oracle's FP32 restatement of the reference CPU path (timing build,
     upstream FP32 sums), one thread, on a bounded sample of the same
@@ -370,6 +415,7 @@ def main():
     depth, dinst = depth_leg(ctx, args.steps, args.warmup)
     ctx.set_params(params)
     export = export_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
+    ingest = ingest_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
 
     out = {
         "metric": "SE3 track frames/s + depth updates/s @640x480, batched vs plain, 1 GPU",
@@ -397,6 +443,7 @@ def main():
         "depth_device_ms_per_update": round(depth["device_ms_per_update"], 4),
         "depth_roofline_gbs": depth["roofline_gbs"],
         "keyframe_export_640x480": export,
+        "ingest_640x480": ingest,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES[dom_name],
                      "kernel": dom_desc,

@@ -343,6 +343,7 @@ lsdg_status lsdg_depth_active_keyframe(lsdg_depthmap dm, int32_t* id);
 #define LSDG_KERNEL_BATCH_SOLVE 4 /* batched LM step (merge, solve, accept) */
 #define LSDG_KERNEL_BATCH_JOB 5   /* batched whole-job LM, one block per job */
 #define LSDG_KERNEL_BATCH_COOP 6  /* batched whole LM, several cooperating blocks per job */
+#define LSDG_KERNEL_INGEST 7      /* undistortion + conversion + image pyramid of one camera image */
 #define LSDG_KERNEL_KINDS 8
 
 typedef struct lsdg_kernel_stats {
@@ -434,6 +435,58 @@ lsdg_status lsdg_points_vbo(lsdg_ctx ctx, const lsdg_point_dense* in, int32_t w,
  * computeVbo with that level's camera. */
 lsdg_status lsdg_keyframe_vbo(lsdg_frame kf, int32_t level, const lsdg_vbo_params* p, lsdg_vertex* out,
                               int32_t* count);
+
+/* ----------------------------------------------------------------- ingest */
+
+/* libvideoio's Undistorter, which tools/LSD.cpp:88 builds from the
+ * calibration file and InputThread applies to every camera image
+ * (lib/App/InputThread.cpp:59-62): a table of one source position (x, y) in
+ * the distorted input per output pixel, x < 0 meaning no source (output 0).
+ * map_xy holds 2 * out_w * out_h floats in raster order; every entry with
+ * x >= 0 must satisfy 0 <= x < in_w - 1 and 0 <= y < in_h - 1 (LSDG_EINVAL
+ * otherwise), which keeps both bilinear taps inside the input.  The table is
+ * copied to device memory; calls that use one undistorter must not overlap. */
+typedef struct lsdg_undistorter_s* lsdg_undistorter;
+lsdg_status lsdg_undistorter_create(lsdg_ctx ctx, int32_t in_w, int32_t in_h, int32_t out_w, int32_t out_h,
+                                    const float* map_xy, lsdg_undistorter* out);
+lsdg_status lsdg_undistorter_destroy(lsdg_undistorter u);
+
+/* The undistortion table of the FOV (ATAN) lens model of the calibration
+ * files LSD-SLAM ships (fx fy cx cy omega, Devernay and Faugeras): an output
+ * pixel (x, y) of the pinhole camera out_k = (fx, fy, cx, cy) at normalised
+ * radius r takes its value from the input camera in_k at radius
+ * atan(2 r tan(omega / 2)) / omega (r itself when omega or r is 0).  Sources
+ * outside 0 < x < in_w - 1, 0 < y < in_h - 1 are marked -1.  Host-only. */
+lsdg_status lsdg_fov_undistort_table(const float in_k[4], float omega, int32_t in_w, int32_t in_h,
+                                     const float out_k[4], int32_t out_w, int32_t out_h, float* map_xy);
+
+/* One camera image through InputThread and Frame's constructor: the
+ * distorted 8-bit image (host memory, in_w x in_h, row stride `stride`) is
+ * undistorted (Undistorter::undistort: bilinear interpolation of the 8-bit
+ * input, truncated to 8 bits), converted to float and reduced to the image
+ * pyramid (Frame::buildImage, levels 0-4) in one launch, into a new frame of
+ * the undistorter's output size (cam->w and cam->h must equal it).  When
+ * `undistorted` is not NULL it receives the 8-bit undistorted image
+ * (out_w * out_h, the live image InputThread.cpp:80 hands the output). */
+lsdg_status lsdg_frame_create_undistorted(lsdg_ctx ctx, int32_t id, const lsdg_camera* cam, lsdg_undistorter u,
+                                          const uint8_t* image, int32_t stride, int32_t precision,
+                                          uint8_t* undistorted, lsdg_frame* out);
+/* The same into an existing frame of the undistorter's output size, for a
+ * stream of camera images without an allocation per image: every field of f
+ * is reset as in a new frame with this id (no depth, identity poses, no
+ * tracking parent, zero counters), its arrays and camera are kept, and point
+ * clouds built from its earlier content are rebuilt when next used.  No
+ * counterpart upstream, which allocates a Frame per image. */
+lsdg_status lsdg_frame_ingest(lsdg_frame f, int32_t id, lsdg_undistorter u, const uint8_t* image, int32_t stride,
+                              uint8_t* undistorted);
+
+/* TextOutputIOWrapper::publishTrackedFrame's pose.txt line
+ * (lib/Pangolin_IOWrapper/TextOutputIOWrapper.cpp:100-120): the frame id, the
+ * camToWorld translation and the thisToParent_raw translation, comma
+ * separated, each number as an ostream prints a double by default (six
+ * significant digits), ending in a newline; NUL-terminated into buf.
+ * LSDG_EINVAL when it does not fit in cap bytes. */
+lsdg_status lsdg_frame_pose_txt(lsdg_frame f, char* buf, size_t cap);
 
 #ifdef __cplusplus
 }

@@ -180,8 +180,8 @@ static void frame_intrinsics(lsdg_frame f, const lsdg_camera* cam) {
   }
 }
 
-static lsdg_status frame_alloc(lsdg_ctx c, int32_t id, const lsdg_camera* cam, int32_t precision,
-                               lsdg_frame* out) {
+lsdg_status lsdg::frame_alloc(lsdg_ctx c, int32_t id, const lsdg_camera* cam, int32_t precision,
+                                lsdg_frame* out) {
   if (!c || !cam || !out) return LSDG_EINVAL;
   if (cam->w < 32 || cam->h < 32 || cam->w > 16384 || cam->h > 16384 || (cam->w >> (PYR - 1)) < 4 ||
       (cam->h >> (PYR - 1)) < 4)
@@ -318,8 +318,16 @@ extern "C" lsdg_status lsdg_frame_create_u8(lsdg_ctx c, int32_t id, const lsdg_c
   lsdg_frame f;
   lsdg_status st = frame_alloc(c, id, cam, precision, &f);
   if (st != LSDG_OK) return st;
-  hipError_t e = hipMemcpy2DAsync(f->d_input_u8, cam->w, image, stride, cam->w, cam->h, hipMemcpyHostToDevice,
-                                  c->stream);
+  // a 2D copy from pageable memory goes row by row: pack padded rows first
+  std::vector<uint8_t> packed;
+  const uint8_t* src = image;
+  if (stride != cam->w) {
+    packed.resize((size_t)cam->w * cam->h);
+    for (int y = 0; y < cam->h; ++y)
+      std::memcpy(packed.data() + (size_t)y * cam->w, image + (size_t)y * stride, (size_t)cam->w);
+    src = packed.data();
+  }
+  hipError_t e = hipMemcpyAsync(f->d_input_u8, src, (size_t)cam->w * cam->h, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     // the image stage converts level 0 and builds levels 1-4; the rest is built on first use
     ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);

@@ -255,6 +255,11 @@ namespace lsdg {
 
 // pyramid.hip
 hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s);
+// undistorts the 8-bit image d_src (row length inW) through the table d_map
+// (one source position per level-0 pixel) into f's level 0, builds levels
+// 1-4, and writes the 8-bit undistorted image to d_undist unless it is null
+hipError_t launch_ingest(lsdg_frame f, const uint8_t* d_src, int inW, const float2* d_map, uint8_t* d_undist,
+                         hipStream_t s);
 // Gradients / packed copies / max-gradients of the levels named in `need`
 // (FD_GRAD, FD_GRADH, FD_MAXG bits); other levels launch no blocks.
 hipError_t launch_pyr_grads(lsdg_frame f, unsigned need, hipStream_t s);

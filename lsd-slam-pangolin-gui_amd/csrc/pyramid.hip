@@ -9,6 +9,8 @@
 //                (recomputed from the image; identical to the reference's
 //                two-pass vertical-then-horizontal max on its linear index).
 //  k_pyr_depth   Frame::buildIDepthAndIDepthVar for levels 1-4 in one launch.
+//  k_ingest      undistortion of an 8-bit camera image, its conversion to
+//                float and buildImage's levels 1-4, in one launch.
 //  k_pc_*        TrackingReference::makePointCloud as a stream compaction
 //                (count, scan, write) in row-major pixel order.
 #include "lsdg_internal.h"
@@ -20,38 +22,11 @@ struct PyrImgArgs {
   int w[PYR], h[PYR];
 };
 
-__device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __restrict__ u8, int bx, int by) {
-  __shared__ float L1[16][17];
+// Levels 2-4 from a block's 16x16 level-1 tile L1 (level 1 already stored).
+__device__ inline void pyr_levels_2to4(const PyrImgArgs& p, const float (*L1)[17], int bx, int by) {
   __shared__ float L2[8][9];
   __shared__ float L3[4][5];
   const int t = threadIdx.x;
-  const int w0 = p.w[0], h0 = p.h[0];
-  if (u8) {
-    for (int k = t; k < 1024; k += 256) {
-      int x = bx * 32 + (k & 31), y = by * 32 + (k >> 5);
-      if (x < w0 && y < h0) p.img[0][x + y * w0] = (float)u8[x + y * w0];
-    }
-  }
-  // level 1
-  {
-    int lx = t & 15, ly = t >> 4;
-    int x = bx * 16 + lx, y = by * 16 + ly;
-    float v = 0.f;
-    if (x < p.w[1] && y < p.h[1]) {
-      int s = 2 * x + 2 * y * w0;
-      float a, b, c, d;
-      if (u8) {
-        a = (float)u8[s]; b = (float)u8[s + 1]; c = (float)u8[s + w0]; d = (float)u8[s + w0 + 1];
-      } else {
-        const float* I = p.img[0];
-        a = I[s]; b = I[s + 1]; c = I[s + w0]; d = I[s + w0 + 1];
-      }
-      v = (a + b + c + d) * 0.25f;
-      p.img[1][x + y * p.w[1]] = v;
-    }
-    L1[ly][lx] = v;
-  }
-  __syncthreads();
   if (t < 64) {
     int lx = t & 7, ly = t >> 3;
     int x = bx * 8 + lx, y = by * 8 + ly;
@@ -84,8 +59,97 @@ __device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __res
   }
 }
 
+__device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __restrict__ u8, int bx, int by) {
+  __shared__ float L1[16][17];
+  const int t = threadIdx.x;
+  const int w0 = p.w[0], h0 = p.h[0];
+  if (u8) {
+    for (int k = t; k < 1024; k += 256) {
+      int x = bx * 32 + (k & 31), y = by * 32 + (k >> 5);
+      if (x < w0 && y < h0) p.img[0][x + y * w0] = (float)u8[x + y * w0];
+    }
+  }
+  // level 1
+  {
+    int lx = t & 15, ly = t >> 4;
+    int x = bx * 16 + lx, y = by * 16 + ly;
+    float v = 0.f;
+    if (x < p.w[1] && y < p.h[1]) {
+      int s = 2 * x + 2 * y * w0;
+      float a, b, c, d;
+      if (u8) {
+        a = (float)u8[s]; b = (float)u8[s + 1]; c = (float)u8[s + w0]; d = (float)u8[s + w0 + 1];
+      } else {
+        const float* I = p.img[0];
+        a = I[s]; b = I[s + 1]; c = I[s + w0]; d = I[s + w0 + 1];
+      }
+      v = (a + b + c + d) * 0.25f;
+      p.img[1][x + y * p.w[1]] = v;
+    }
+    L1[ly][lx] = v;
+  }
+  __syncthreads();
+  pyr_levels_2to4(p, L1, bx, by);
+}
+
 __global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, const uint8_t* __restrict__ u8) {
   pyr_images_tile(p, u8, blockIdx.x, blockIdx.y);
+}
+
+// Undistorter::undistort for one output pixel: the source position m (from
+// the undistortion table) is bilinearly interpolated in the 8-bit input and
+// truncated back to 8 bits; m.x < 0 has no source and gives 0.  Table entries
+// are checked on the host (0 <= m < size - 1), so both taps are in bounds.
+__device__ inline uint8_t undistort_pixel(const uint8_t* __restrict__ src, int inW, float2 m) {
+  if (m.x < 0.0f) return 0;
+  const int xi = (int)m.x, yi = (int)m.y;
+  const float xx = m.x - (float)xi, yy = m.y - (float)yi;
+  const float xxyy = xx * yy;
+  const uint8_t* b = src + xi + yi * inW;
+  const float v = xxyy * (float)b[1 + inW] + (yy - xxyy) * (float)b[inW] + (xx - xxyy) * (float)b[1] +
+                  (1.0f - xx - yy + xxyy) * (float)b[0];
+  return (uint8_t)v;
+}
+
+// InputThread's ingest of one camera image (lib/App/InputThread.cpp:59-71):
+// undistortion, the 8-bit to float conversion of Frame's constructor and
+// Frame::buildImage for levels 1-4, in one launch.  A block owns a 32x32
+// level-0 tile: it remaps the tile into LDS (writing level 0 and, when
+// `undist` is set, the 8-bit undistorted image) and reduces it through the
+// levels as k_pyr_images does.
+__global__ __launch_bounds__(256) void k_ingest(PyrImgArgs p, const uint8_t* __restrict__ src, int inW,
+                                                const float2* __restrict__ map, uint8_t* __restrict__ undist) {
+  __shared__ float L0[32][33];
+  __shared__ float L1[16][17];
+  const int t = threadIdx.x;
+  const int bx = blockIdx.x, by = blockIdx.y;
+  const int w0 = p.w[0], h0 = p.h[0];
+  for (int k = t; k < 1024; k += 256) {
+    const int lx = k & 31, ly = k >> 5;
+    const int x = bx * 32 + lx, y = by * 32 + ly;
+    float v = 0.0f;
+    if (x < w0 && y < h0) {
+      const int i = x + y * w0;
+      const uint8_t u = undistort_pixel(src, inW, map[i]);
+      v = (float)u;
+      p.img[0][i] = v;
+      if (undist) undist[i] = u;
+    }
+    L0[ly][lx] = v;
+  }
+  __syncthreads();
+  {
+    const int lx = t & 15, ly = t >> 4;
+    const int x = bx * 16 + lx, y = by * 16 + ly;
+    float v = 0.f;
+    if (x < p.w[1] && y < p.h[1]) {
+      v = (L0[2 * ly][2 * lx] + L0[2 * ly][2 * lx + 1] + L0[2 * ly + 1][2 * lx] + L0[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[1][x + y * p.w[1]] = v;
+    }
+    L1[ly][lx] = v;
+  }
+  __syncthreads();
+  pyr_levels_2to4(p, L1, bx, by);
 }
 
 struct PyrGradArgs {
@@ -380,6 +444,19 @@ hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s) {
   }
   dim3 grid((f->w[0] + 31) / 32, (f->h[0] + 31) / 32);
   hipLaunchKernelGGL(k_pyr_images, grid, dim3(256), 0, s, p, d_u8);
+  return hipGetLastError();
+}
+
+hipError_t launch_ingest(lsdg_frame f, const uint8_t* d_src, int inW, const float2* d_map, uint8_t* d_undist,
+                         hipStream_t s) {
+  PyrImgArgs p;
+  for (int l = 0; l < PYR; ++l) {
+    p.img[l] = f->d_image[l];
+    p.w[l] = f->w[l];
+    p.h[l] = f->h[l];
+  }
+  dim3 grid((f->w[0] + 31) / 32, (f->h[0] + 31) / 32);
+  k_ingest<<<grid, dim3(256), 0, s>>>(p, d_src, inW, d_map, d_undist);
   return hipGetLastError();
 }
 

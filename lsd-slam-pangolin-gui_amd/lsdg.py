@@ -120,7 +120,8 @@ def hyp_struct(arrs):
     return Hypotheses(*vals)
 
 
-KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE, KERNEL_BATCH_JOB, KERNEL_BATCH_COOP = range(7)
+KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE, KERNEL_BATCH_JOB, KERNEL_BATCH_COOP, \
+    KERNEL_INGEST = range(8)
 
 
 class KernelStats(C.Structure):
@@ -251,6 +252,12 @@ def load():
         "lsdg_keyframe_points": (i32, [vp, i32, fp]),
         "lsdg_points_vbo": (i32, [vp, fp, i32, i32, f32, f32, f32, f32, C.POINTER(VboParams), fp, C.POINTER(i32)]),
         "lsdg_keyframe_vbo": (i32, [vp, i32, C.POINTER(VboParams), fp, C.POINTER(i32)]),
+        "lsdg_undistorter_create": (i32, [vp, i32, i32, i32, i32, fp, C.POINTER(vp)]),
+        "lsdg_undistorter_destroy": (i32, [vp]),
+        "lsdg_fov_undistort_table": (i32, [fp, f32, i32, i32, fp, i32, i32, fp]),
+        "lsdg_frame_create_undistorted": (i32, [vp, i32, C.POINTER(Camera), vp, fp, i32, i32, fp, C.POINTER(vp)]),
+        "lsdg_frame_pose_txt": (i32, [vp, C.c_char_p, C.c_size_t]),
+        "lsdg_frame_ingest": (i32, [vp, i32, vp, fp, i32, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -378,6 +385,30 @@ class Context:
                                           C.byref(n)), "lsdg_points_vbo")
         return out[:n.value]
 
+    def undistorter(self, in_size, table):
+        """libvideoio's Undistorter from a (out_h, out_w, 2) table of source
+        positions into the in_size = (w, h) distorted image (x < 0: no source)."""
+        tab = np.ascontiguousarray(table, np.float32)
+        h = C.c_void_p()
+        self.check(self.L.lsdg_undistorter_create(self.h, in_size[0], in_size[1], tab.shape[1], tab.shape[0],
+                                                  tab.ctypes.data, C.byref(h)), "lsdg_undistorter_create")
+        return Undistorter(self, h, in_size, (tab.shape[1], tab.shape[0]))
+
+    def frame_undistorted(self, fid, cam, und, image, precision=FP32, want_undistorted=False):
+        """InputThread's ingest: undistort the 8-bit image, convert, build the
+        image pyramid; returns the frame (and the 8-bit undistorted image)."""
+        img = np.ascontiguousarray(image, np.uint8)
+        assert img.shape == (und.in_size[1], und.in_size[0])
+        cc = Camera.of(cam)
+        out = np.zeros((cam["h"], cam["w"]), np.uint8) if want_undistorted else None
+        h = C.c_void_p()
+        self.check(self.L.lsdg_frame_create_undistorted(self.h, fid, C.byref(cc), und.h, img.ctypes.data,
+                                                        img.shape[1], precision,
+                                                        None if out is None else out.ctypes.data, C.byref(h)),
+                   "lsdg_frame_create_undistorted")
+        f = Frame(self, h, fid, cam)
+        return (f, out) if want_undistorted else f
+
     def depthmap(self, cam):
         h = C.c_void_p()
         cc = Camera.of(cam)
@@ -389,6 +420,29 @@ class Context:
         h = C.c_void_p()
         self.check(self.L.lsdg_batch_create(self.h, arr, len(jobs), C.byref(h)), "lsdg_batch_create")
         return Batch(self, h, len(jobs))
+
+
+def fov_undistort_table(in_k, omega, in_size, out_k, out_size):
+    """The FOV lens model's undistortion table, (out_h, out_w, 2) float32."""
+    L = load()
+    ik = (C.c_float * 4)(*in_k)
+    ok = (C.c_float * 4)(*out_k)
+    out = np.zeros((out_size[1], out_size[0], 2), np.float32)
+    st = L.lsdg_fov_undistort_table(ik, float(omega), in_size[0], in_size[1], ok, out_size[0], out_size[1],
+                                    out.ctypes.data)
+    if st < 0:
+        raise LsdgError(f"lsdg_fov_undistort_table failed with status {st}")
+    return out
+
+
+class Undistorter:
+    def __init__(self, ctx, h, in_size, out_size):
+        self.ctx, self.h, self.in_size, self.out_size = ctx, h, in_size, out_size
+
+    def close(self):
+        if self.h:
+            self.ctx.L.lsdg_undistorter_destroy(self.h)
+            self.h = None
 
 
 class Frame:
@@ -430,6 +484,22 @@ class Frame:
         self.ctx.check(self.ctx.L.lsdg_keyframe_vbo(self.h, level, C.byref(p), out.ctypes.data, C.byref(n)),
                        "lsdg_keyframe_vbo")
         return out[:n.value]
+
+    def ingest(self, fid, und, image, want_undistorted=False):
+        """Reuse this frame for a new camera image (lsdg_frame_ingest)."""
+        img = np.ascontiguousarray(image, np.uint8)
+        assert img.shape == (und.in_size[1], und.in_size[0])
+        out = np.zeros((self.cam["h"], self.cam["w"]), np.uint8) if want_undistorted else None
+        self.ctx.check(self.ctx.L.lsdg_frame_ingest(self.h, fid, und.h, img.ctypes.data, img.shape[1],
+                                                    None if out is None else out.ctypes.data), "lsdg_frame_ingest")
+        self.id = fid
+        return out
+
+    def pose_txt(self):
+        """TextOutputIOWrapper::publishTrackedFrame's pose.txt line."""
+        buf = C.create_string_buffer(512)
+        self.ctx.check(self.ctx.L.lsdg_frame_pose_txt(self.h, buf, 512), "lsdg_frame_pose_txt")
+        return buf.value.decode()
 
     def invalidate(self):
         """Drop the derived pyramid levels; the next reader rebuilds what it needs."""

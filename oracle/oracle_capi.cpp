@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 
+#include "ingest.hpp"
 #include "keyframe_export.hpp"
 #include "lsd_oracle.hpp"
 
@@ -631,4 +632,19 @@ extern "C" void orc_frame_camera(void* f, int level, float* k4) {
   } else {
     k4[0] = F->f32.fx[level]; k4[1] = F->f32.fy[level]; k4[2] = F->f32.cx[level]; k4[3] = F->f32.cy[level];
   }
+}
+
+// Ingest (Undistorter::undistort), the FOV model's table, the pose.txt line.
+extern "C" void orc_undistort(const uint8_t* src, int inW, const float* map, int outW, int outH, uint8_t* out) {
+  undistort(src, inW, map, outW, outH, out);
+}
+extern "C" void orc_fov_table(const float* ik, float omega, int inW, int inH, const float* ok, int outW, int outH,
+                              float* map) {
+  fov_table(ik, omega, inW, inH, ok, outW, outH, map);
+}
+extern "C" int orc_pose_txt_line(int id, const double* c2wT, const double* t2pT, char* buf, int cap) {
+  const std::string s = pose_txt_line(id, c2wT, t2pT);
+  if ((int)s.size() >= cap) return -1;
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return (int)s.size();
 }

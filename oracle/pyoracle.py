@@ -72,6 +72,9 @@ def load(fast=False):
         "orc_keyframe_points": (i32, [vp, i32, fp]),
         "orc_compute_vbo": (i32, [fp, i32, i32, f32, f32, f32, f32, C.POINTER(lsdg.VboParams), fp]),
         "orc_frame_camera": (None, [vp, i32, fp]),
+        "orc_undistort": (None, [vp, i32, vp, i32, i32, vp]),
+        "orc_fov_table": (None, [fp, f32, i32, i32, fp, i32, i32, vp]),
+        "orc_pose_txt_line": (i32, [i32, dp, dp, C.c_char_p, i32]),
     }
     for n, (r, a) in sig.items():
         fn = getattr(L, n)
@@ -97,6 +100,30 @@ class Oracle:
         self.L = load(fast)
         self.prec = prec
         self.params = params if params is not None else default_params()
+
+    def undistort(self, src, table):
+        """Undistorter::undistort of an 8-bit image through a (h, w, 2) table."""
+        src = np.ascontiguousarray(src, np.uint8)
+        tab = np.ascontiguousarray(table, np.float32)
+        out = np.zeros(tab.shape[:2], np.uint8)
+        self.L.orc_undistort(src.ctypes.data, src.shape[1], tab.ctypes.data, tab.shape[1], tab.shape[0],
+                             out.ctypes.data)
+        return out
+
+    def fov_table(self, in_k, omega, in_size, out_k, out_size):
+        ik = (C.c_float * 4)(*in_k)
+        ok = (C.c_float * 4)(*out_k)
+        out = np.zeros((out_size[1], out_size[0], 2), np.float32)
+        self.L.orc_fov_table(ik, float(omega), in_size[0], in_size[1], ok, out_size[0], out_size[1], out.ctypes.data)
+        return out
+
+    def pose_txt_line(self, fid, cam_to_world_t, this_to_parent_t):
+        a, pa = _dp(cam_to_world_t)
+        b, pb = _dp(this_to_parent_t)
+        buf = C.create_string_buffer(512)
+        n = self.L.orc_pose_txt_line(int(fid), pa, pb, buf, 512)
+        assert n >= 0
+        return buf.value.decode()
 
     def compute_vbo(self, records, w, h, cam4, params):
         rec = np.ascontiguousarray(records, lsdg.POINT_DENSE_DTYPE)
