@@ -243,6 +243,53 @@ def ingest_leg(ctx, steps, warmup, cpu_seconds):
     return out
 
 
+def constraint_leg(ctx, steps, warmup, cpu_seconds, pairs=16):
+    """SlamSystem::testConstraint for 16 keyframe pairs of 640x480 (a third of
+    them of unrelated scenes), strictness 1.5: three stages of batched Sim3
+    alignments in both directions plus the consistency test; the oracle's
+    sequential restatement (FP32, timing build, one thread) beside it."""
+    init = [0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
+    cam = synth.camera(640, 480)
+    data = []
+    for k in range(pairs):
+        d = synth.sim3_instance(2000 + k, cam=cam, max_trans=0.04, max_rot_deg=2.0, scale_range=(0.95, 1.05),
+                                fraction=0.3)
+        if k % 3 == 2:
+            e = synth.sim3_instance(2500 + k, cam=cam, fraction=0.3)
+            d = dict(d, b_img=e["b_img"], b_idepth=e["b_idepth"], b_var=e["b_var"])
+        data.append(d)
+    items = []
+    for k, d in enumerate(data):
+        a = ctx.frame(2 * k, cam, d["a_img"], lsdg.FP16)
+        a.set_depth(d["a_idepth"], d["a_var"])
+        b = ctx.frame(2 * k + 1, cam, d["b_img"], lsdg.FP16)
+        b.set_depth(d["b_idepth"], d["b_var"])
+        items.append((a, b, ctx.ref(a), ctx.ref(b)))
+    cands = [(ra, rb, init) for a, b, ra, rb in items]
+    t = timed(ctx, lambda: ctx.constraint_search(cands, 1.5), steps, warmup)
+    res = ctx.constraint_search(cands, 1.5)
+    out = {"pairs_per_s": round(pairs * steps / t, 1), "pairs": pairs,
+           "accepted": int(sum(r.accepted for r in res))}
+    for a, b, ra, rb in items:
+        for x in (ra, rb, a, b):
+            x.destroy()
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(0, fast=True)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            d = data[n % pairs]
+            a = o.frame(0, cam, d["a_img"].astype(np.float32))
+            a.set_depth(d["a_idepth"], d["a_var"])
+            b = o.frame(1, cam, d["b_img"].astype(np.float32))
+            b.set_depth(d["b_idepth"], d["b_var"])
+            o.constraint_test(o.ref(a), o.ref(b), init, 1.5)
+            n += 1
+        out["cpu_pairs_per_s"] = round(n / (time.perf_counter() - t0), 2)
+    return out
+
+
 def cpu_tracking_baseline(insts, params, seconds):
     """The oracle's FP32 restatement of the reference CPU path (timing build,
     upstream FP32 sums), one thread, on a bounded sample of the same
@@ -416,6 +463,7 @@ def main():
     ctx.set_params(params)
     export = export_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
     ingest = ingest_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
+    constraint = constraint_leg(ctx, 3, 1, 0.0 if args.no_cpu else 4.0)
 
     out = {
         "metric": "SE3 track frames/s + depth updates/s @640x480, batched vs plain, 1 GPU",
@@ -444,6 +492,7 @@ def main():
         "depth_roofline_gbs": depth["roofline_gbs"],
         "keyframe_export_640x480": export,
         "ingest_640x480": ingest,
+        "constraint_search_640x480": constraint,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES[dom_name],
                      "kernel": dom_desc,

@@ -436,6 +436,50 @@ lsdg_status lsdg_points_vbo(lsdg_ctx ctx, const lsdg_point_dense* in, int32_t w,
 lsdg_status lsdg_keyframe_vbo(lsdg_frame kf, int32_t level, const lsdg_vbo_params* p, lsdg_vertex* out,
                               int32_t* count);
 
+/* ----------------------------------------------------- constraint search */
+
+/* One keyframe pair for lsd-slam's SlamSystem::testConstraint, the consumer
+ * of keyframe-to-keyframe Sim3 alignment that feeds the pose graph whose
+ * poses PangolinOutputIOWrapper::publishKeyframeGraph shows
+ * (lib/Pangolin_IOWrapper/PangolinOutputIOWrapper.cpp:148-169): the new
+ * keyframe's tracking reference (A), a candidate keyframe's (B), and the
+ * initial candidate-to-keyframe Sim3 (qx, qy, qz, qw, tx, ty, tz, s).  Both
+ * keyframes must carry depth. */
+typedef struct lsdg_constraint_candidate {
+  lsdg_ref frame_ref;
+  lsdg_ref candidate_ref;
+  double candidate_to_frame_init[8];
+} lsdg_constraint_candidate;
+
+/* KFConstraintStruct: one directed edge of an accepted pair. */
+typedef struct lsdg_constraint_edge {
+  int32_t first_id, second_id;  /* keyframe ids */
+  double second_to_first[8];    /* the Sim3 alignment of `second` on `first` */
+  double information[49];       /* its lastSim3Hessian (row-major 7x7) */
+  float mean_residual, mean_residual_d, mean_residual_p, usage;
+} lsdg_constraint_edge;
+
+typedef struct lsdg_constraint_result {
+  int32_t accepted;
+  int32_t rejected_at;          /* 0 if accepted, else the stage (1: levels 4-3, 2: level 2, 3: level 1) */
+  float error[3];               /* reciprocal consistency of each stage run; 1e20 when an alignment failed */
+  double candidate_to_frame[8]; /* the last alignments of the two directions */
+  double frame_to_candidate[8];
+  lsdg_constraint_edge candidate_on_frame;  /* first = A, second = B (when accepted) */
+  lsdg_constraint_edge frame_on_candidate;  /* first = B, second = A */
+} lsdg_constraint_result;
+
+/* testConstraint for n pairs: per stage (levels 4..3, 2, 1) both directions
+ * of every pair still in the running are tracked in one batch
+ * (trackFrameSim3 of B's keyframe on A and of A's keyframe on B, each from
+ * its previous result), a direction that diverges or whose scale or
+ * Hessian degenerates fails the pair's stage, and otherwise the reciprocal
+ * consistency diff' (Sigma_AB + Adj_AB Sigma_BA Adj_AB')^-1 diff with
+ * diff = log(AtoB * BtoA) is compared against 3000, 4000 and 6000 times
+ * `strictness`.  The 7x7 algebra is double precision here (float upstream). */
+lsdg_status lsdg_constraint_search(lsdg_ctx ctx, const lsdg_constraint_candidate* candidates, int32_t n,
+                                   float strictness, lsdg_constraint_result* results);
+
 /* ----------------------------------------------------------------- ingest */
 
 /* libvideoio's Undistorter, which tools/LSD.cpp:88 builds from the

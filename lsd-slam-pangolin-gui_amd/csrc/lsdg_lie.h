@@ -140,13 +140,9 @@ LSDG_HD Pose se3_exp(const double* xi) {
   return r;
 }
 
-// exp of a sim3 twist (v, w, sigma): s = e^sigma, R = exp(w),
-// t = (A I + B W + C W^2) v with V = int_0^1 e^{u sigma} exp(u W) du.
-LSDG_HD Pose sim3_exp(const double* xi) {
-  Pose r;
-  const double* w = xi + 3;
-  const double sg = xi[6];
-  so3_exp_q(w, r.q);
+// Coefficients of V = A I + B W + C W^2 = int_0^1 e^{u sigma} exp(u W) du
+// for the rotation vector w (W = hat(w)) and log-scale sigma.
+LSDG_HD void sim3_v_coeffs(double sg, const double* w, double* Ao, double* Bo, double* Co) {
   double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   double th = sqrt(th2);
   double es = exp(sg);
@@ -173,8 +169,20 @@ LSDG_HD Pose sim3_exp(const double* xi) {
       C = (A - (es * (sg * cs + th * sn) - sg) / den) / th2;
     }
   }
+  *Ao = A;
+  *Bo = B;
+  *Co = C;
+}
+
+// exp of a sim3 twist (v, w, sigma): s = e^sigma, R = exp(w), t = V v.
+LSDG_HD Pose sim3_exp(const double* xi) {
+  Pose r;
+  const double* w = xi + 3;
+  so3_exp_q(w, r.q);
+  double A, B, C;
+  sim3_v_coeffs(xi[6], w, &A, &B, &C);
   apply_abc(A, B, C, w, xi, r.t);
-  r.s = es;
+  r.s = exp(xi[6]);
   return r;
 }
 

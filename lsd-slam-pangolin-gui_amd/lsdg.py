@@ -138,6 +138,24 @@ class JobResult(C.Structure):
     _fields_ = [("frame_to_ref", C.c_double * 8), ("stats", TrackStats), ("status", C.c_int32)]
 
 
+class ConstraintCandidate(C.Structure):
+    _fields_ = [("frame_ref", C.c_void_p), ("candidate_ref", C.c_void_p),
+                ("candidate_to_frame_init", C.c_double * 8)]
+
+
+class ConstraintEdge(C.Structure):
+    """KFConstraintStruct: second_to_first, information and residuals."""
+    _fields_ = [("first_id", C.c_int32), ("second_id", C.c_int32), ("second_to_first", C.c_double * 8),
+                ("information", C.c_double * 49), ("mean_residual", C.c_float), ("mean_residual_d", C.c_float),
+                ("mean_residual_p", C.c_float), ("usage", C.c_float)]
+
+
+class ConstraintResult(C.Structure):
+    _fields_ = [("accepted", C.c_int32), ("rejected_at", C.c_int32), ("error", C.c_float * 3),
+                ("candidate_to_frame", C.c_double * 8), ("frame_to_candidate", C.c_double * 8),
+                ("candidate_on_frame", ConstraintEdge), ("frame_on_candidate", ConstraintEdge)]
+
+
 class PointDense(C.Structure):
     """InputPointDense (lib/Pangolin_IOWrapper/Keyframe.h:16-21)."""
     _fields_ = [("idepth", C.c_float), ("idepth_var", C.c_float), ("color", C.c_uint8 * 4)]
@@ -258,6 +276,8 @@ def load():
         "lsdg_frame_create_undistorted": (i32, [vp, i32, C.POINTER(Camera), vp, fp, i32, i32, fp, C.POINTER(vp)]),
         "lsdg_frame_pose_txt": (i32, [vp, C.c_char_p, C.c_size_t]),
         "lsdg_frame_ingest": (i32, [vp, i32, vp, fp, i32, fp]),
+        "lsdg_constraint_search": (i32, [vp, C.POINTER(ConstraintCandidate), i32, f32,
+                                         C.POINTER(ConstraintResult)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -384,6 +404,18 @@ class Context:
         self.check(self.L.lsdg_points_vbo(self.h, rec.ctypes.data, w, h, fx, fy, cx, cy, C.byref(p), out.ctypes.data,
                                           C.byref(n)), "lsdg_points_vbo")
         return out[:n.value]
+
+    def constraint_search(self, candidates, strictness=1.5):
+        """SlamSystem::testConstraint for [(frame_ref, candidate_ref, candidate_to_frame_init8), ...]."""
+        n = len(candidates)
+        arr = (ConstraintCandidate * max(n, 1))()
+        for i, (a, b, init) in enumerate(candidates):
+            arr[i].frame_ref = a.h
+            arr[i].candidate_ref = b.h
+            arr[i].candidate_to_frame_init[:] = [float(v) for v in init]
+        out = (ConstraintResult * max(n, 1))()
+        self.check(self.L.lsdg_constraint_search(self.h, arr, n, float(strictness), out), "lsdg_constraint_search")
+        return list(out[:n])
 
     def undistorter(self, in_size, table):
         """libvideoio's Undistorter from a (out_h, out_w, 2) table of source

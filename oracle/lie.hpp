@@ -130,17 +130,11 @@ inline Sim3 se3_exp(const double* xi) {
   return r;
 }
 
-// Sim3 exp of (v1,v2,v3,w1,w2,w3,sigma): s = e^sigma, R = exp(w),
-// t = V u with V = int_0^1 e^{s sigma} exp(s W) ds = A I + B W + C W^2.
-inline Sim3 sim3_exp(const double* xi) {
-  Sim3 r;
-  const double* w = xi + 3;
-  const double sg = xi[6];
-  r.q = so3_exp(w);
+// Coefficients of V = A I + B W + C W^2 = int_0^1 e^{s sigma} exp(s W) ds.
+inline void sim3_abc(double sg, const double* w, double& A, double& B, double& C) {
   double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   double th = std::sqrt(th2);
   double es = std::exp(sg);
-  double A, B, C;
   if (std::fabs(sg) < 1e-6) {
     A = 1.0 + 0.5 * sg + sg * sg / 6.0;
     if (th < 1e-5) {
@@ -164,8 +158,17 @@ inline Sim3 sim3_exp(const double* xi) {
       C = (A - (es * (sg * c + th * s) - sg) / den) / th2;
     }
   }
+}
+
+// Sim3 exp of (v1,v2,v3,w1,w2,w3,sigma): s = e^sigma, R = exp(w), t = V u.
+inline Sim3 sim3_exp(const double* xi) {
+  Sim3 r;
+  const double* w = xi + 3;
+  r.q = so3_exp(w);
+  double A, B, C;
+  sim3_abc(xi[6], w, A, B, C);
   apply_V(A, B, C, w, xi, r.t);
-  r.s = es;
+  r.s = std::exp(xi[6]);
   return r;
 }
 

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 
+#include "constraint.hpp"
 #include "ingest.hpp"
 #include "keyframe_export.hpp"
 #include "lsd_oracle.hpp"
@@ -647,4 +648,66 @@ extern "C" int orc_pose_txt_line(int id, const double* c2wT, const double* t2pT,
   if ((int)s.size() >= cap) return -1;
   std::memcpy(buf, s.c_str(), s.size() + 1);
   return (int)s.size();
+}
+
+// testConstraint for one pair (tracking references of the new keyframe A and
+// a candidate B, keyframe ids idA / idB), sequential as upstream.
+static bool usable(const double* pose8, const lsdg_track_stats& st) {
+  return !st.diverged && pose8[7] <= 1.0 / 1e-10 && pose8[7] >= 1e-10 && st.hessian[0] != 0.0 && st.hessian[48] != 0.0;
+}
+static void edge_of(const double* pose8, const lsdg_track_stats& st, int first, int second, lsdg_constraint_edge& e) {
+  e.first_id = first;
+  e.second_id = second;
+  std::memcpy(e.second_to_first, pose8, sizeof(e.second_to_first));
+  std::memcpy(e.information, st.hessian, sizeof(e.information));
+  e.mean_residual = st.last_residual;
+  e.mean_residual_d = st.last_depth_residual;
+  e.mean_residual_p = st.last_photometric_residual;
+  e.usage = st.point_usage;
+}
+extern "C" void orc_constraint_test(void* refA, void* refB, int idA, int idB, const double* init8,
+                                    const lsdg_params* P, float strictness, lsdg_constraint_result* out) {
+  ORef* A = (ORef*)refA;
+  ORef* B = (ORef*)refB;
+  std::memset(out, 0, sizeof(*out));
+  static const int lv[3][2] = {{4, 3}, {2, 2}, {1, 1}};
+  static const float th[3] = {3000.0f, 4000.0f, 6000.0f};
+  double ctof[8], ftoc[8];
+  std::memcpy(ctof, init8, sizeof(ctof));
+  Sim3::from8(init8).inverse().to8(ftoc);
+  lsdg_track_stats sb, sa;
+  for (int s = 0; s < 3; ++s) {
+    double o[8];
+    orc_sim3_track(A, B->kf, P, ctof, lv[s][0], lv[s][1], o, &sb);
+    std::memcpy(ctof, o, sizeof(o));
+    float err;
+    if (!usable(ctof, sb)) {
+      err = 1e20f;
+    } else {
+      orc_sim3_track(B, A->kf, P, ftoc, lv[s][0], lv[s][1], o, &sa);
+      std::memcpy(ftoc, o, sizeof(o));
+      err = usable(ftoc, sa) ? (float)reciprocal_consistency(Sim3::from8(ftoc), sa.hessian, Sim3::from8(ctof),
+                                                              sb.hessian)
+                             : 1e20f;
+    }
+    out->error[s] = err;
+    if (err > th[s] * strictness) {
+      out->rejected_at = s + 1;
+      break;
+    }
+    if (s == 2) {
+      out->accepted = 1;
+      edge_of(ctof, sb, idA, idB, out->candidate_on_frame);
+      edge_of(ftoc, sa, idB, idA, out->frame_on_candidate);
+    }
+  }
+  std::memcpy(out->candidate_to_frame, ctof, sizeof(ctof));
+  std::memcpy(out->frame_to_candidate, ftoc, sizeof(ftoc));
+}
+// The Sim3 log, adjoint and reciprocal consistency, for their pinning tests.
+extern "C" void orc_sim3_log(const double* p8, double* xi) { sim3_log(Sim3::from8(p8), xi); }
+extern "C" void orc_sim3_adj(const double* p8, double* M) { sim3_adj(Sim3::from8(p8), M); }
+extern "C" double orc_reciprocal_consistency(const double* atob8, const double* atobInfo, const double* btoa8,
+                                             const double* btoaInfo) {
+  return reciprocal_consistency(Sim3::from8(atob8), atobInfo, Sim3::from8(btoa8), btoaInfo);
 }

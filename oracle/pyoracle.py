@@ -75,6 +75,10 @@ def load(fast=False):
         "orc_undistort": (None, [vp, i32, vp, i32, i32, vp]),
         "orc_fov_table": (None, [fp, f32, i32, i32, fp, i32, i32, vp]),
         "orc_pose_txt_line": (i32, [i32, dp, dp, C.c_char_p, i32]),
+        "orc_constraint_test": (None, [vp, vp, i32, i32, dp, P, f32, C.POINTER(lsdg.ConstraintResult)]),
+        "orc_sim3_log": (None, [dp, dp]),
+        "orc_sim3_adj": (None, [dp, dp]),
+        "orc_reciprocal_consistency": (C.c_double, [dp, dp, dp, dp]),
     }
     for n, (r, a) in sig.items():
         fn = getattr(L, n)
@@ -100,6 +104,33 @@ class Oracle:
         self.L = load(fast)
         self.prec = prec
         self.params = params if params is not None else default_params()
+
+    def constraint_test(self, frame_ref, candidate_ref, candidate_to_frame_init8, strictness=1.5):
+        """SlamSystem::testConstraint for one pair, sequential as upstream."""
+        init, pi = _dp(candidate_to_frame_init8)
+        out = lsdg.ConstraintResult()
+        self.L.orc_constraint_test(frame_ref.h, candidate_ref.h, frame_ref.kf.id, candidate_ref.kf.id, pi,
+                                   C.byref(self.params), float(strictness), C.byref(out))
+        return out
+
+    def sim3_log(self, pose8):
+        p, pp = _dp(pose8)
+        out = np.zeros(7)
+        self.L.orc_sim3_log(pp, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def sim3_adj(self, pose8):
+        p, pp = _dp(pose8)
+        out = np.zeros((7, 7))
+        self.L.orc_sim3_adj(pp, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def reciprocal_consistency(self, atob8, atob_info, btoa8, btoa_info):
+        a, pa = _dp(atob8)
+        ai, pai = _dp(atob_info)
+        b, pb = _dp(btoa8)
+        bi, pbi = _dp(btoa_info)
+        return self.L.orc_reciprocal_consistency(pa, pai, pb, pbi)
 
     def undistort(self, src, table):
         """Undistorter::undistort of an 8-bit image through a (h, w, 2) table."""
