@@ -290,6 +290,37 @@ def constraint_leg(ctx, steps, warmup, cpu_seconds, pairs=16):
     return out
 
 
+def system_leg(ctx, cpu_seconds, n=60):
+    """lsd-slam's SlamSystem front end (trackFrame + the mapping it waits
+    for) over a 60-frame 640x480 synthetic sequence (1.3 cm and 0.15 deg per
+    frame), FP16 pyramids, GT-depth initialisation untimed; the oracle's
+    restatement (FP32, timing build, one thread) on the first frames beside it."""
+    d = synth.vo_sequence(7, n, cam=synth.camera(640, 480))
+    s = ctx.system(d["cam"], lsdg.FP16)
+    s.init_gt(0, d["images"][0], d["idepth"], d["var"])
+    ctx.sync()
+    t0 = time.perf_counter()
+    infos = [s.track(k, d["images"][k]) for k in range(1, n)]
+    t = time.perf_counter() - t0
+    err = max(float(np.linalg.norm(np.array(i.cam_to_world[4:7]) - d["poses"][k + 1][4:7]))
+              for k, i in enumerate(infos))
+    ids, _ = s.keyframes()
+    s.destroy()
+    out = {"frames_per_s": round((n - 1) / t, 1), "frames": n - 1, "keyframes": len(ids),
+           "max_translation_error_m": round(err, 5)}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.OSystem(pyoracle.Oracle(0, fast=True), d["cam"])
+        o.init_gt(0, d["images"][0], d["idepth"], d["var"])
+        k, t0 = 1, time.perf_counter()
+        while k < n and time.perf_counter() - t0 < cpu_seconds:
+            o.track(k, d["images"][k])
+            k += 1
+        out["cpu_frames_per_s"] = round((k - 1) / (time.perf_counter() - t0), 2)
+    return out
+
+
 def cpu_tracking_baseline(insts, params, seconds):
     """The oracle's FP32 restatement of the reference CPU path (timing build,
     upstream FP32 sums), one thread, on a bounded sample of the same
@@ -464,6 +495,7 @@ def main():
     export = export_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
     ingest = ingest_leg(ctx, 20, 3, 0.0 if args.no_cpu else 1.0)
     constraint = constraint_leg(ctx, 3, 1, 0.0 if args.no_cpu else 4.0)
+    system = system_leg(ctx, 0.0 if args.no_cpu else 6.0)
 
     out = {
         "metric": "SE3 track frames/s + depth updates/s @640x480, batched vs plain, 1 GPU",
@@ -493,6 +525,7 @@ def main():
         "keyframe_export_640x480": export,
         "ingest_640x480": ingest,
         "constraint_search_640x480": constraint,
+        "slam_system_640x480": system,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES[dom_name],
                      "kernel": dom_desc,

@@ -480,6 +480,61 @@ typedef struct lsdg_constraint_result {
 lsdg_status lsdg_constraint_search(lsdg_ctx ctx, const lsdg_constraint_candidate* candidates, int32_t n,
                                    float strictness, lsdg_constraint_result* results);
 
+/* ----------------------------------------------------------- the system */
+
+/* lsd-slam's SlamSystem front end, the direct caller of the path
+ * (lib/App/InputThread.cpp:71 hands it every image; tools/LSD.cpp:102,139
+ * builds it): per image trackFrame (SE3 tracking on the current keyframe,
+ * loss detection, keyframe selection) followed, as with blockUntilMapped, by
+ * the mapping iterations it waits for (updateKeyframe with the queued
+ * frames, or finalizeKeyFrame + createKeyFrame on the latest tracked frame).
+ * One thread, deterministic.  The pose graph, loop closures and
+ * relocalisation are not part of it: once tracking is lost, later images
+ * are reported as not tracked. */
+typedef struct lsdg_system_s* lsdg_system;
+
+/* SlamSystem's keyframe-selection constants (lsd-slam settings). */
+typedef struct lsdg_system_params {
+  float kf_dist_weight;                /* KFDistWeight = 4 */
+  float kf_usage_weight;               /* KFUsageWeight = 3 */
+  int32_t min_num_mapped;              /* MIN_NUM_MAPPED = 5 */
+  int32_t initialization_phase_count;  /* INITIALIZATION_PHASE_COUNT = 5 */
+} lsdg_system_params;
+void lsdg_system_params_default(lsdg_system_params* p);
+
+#define LSDG_SYSTEM_TRACKED 0
+#define LSDG_SYSTEM_LOST 1        /* tracking lost on this image */
+#define LSDG_SYSTEM_NOT_TRACKED 2 /* tracking was lost before */
+
+typedef struct lsdg_system_frame {
+  int32_t id;
+  int32_t status;             /* LSDG_SYSTEM_* */
+  int32_t keyframe_id;        /* the keyframe it was tracked on */
+  int32_t new_keyframe;       /* 1 when the mapping made it the new keyframe */
+  float closeness_score;      /* lastTrackingClosenessScore; -1 when not evaluated */
+  float keyframe_mean_idepth; /* the keyframe's meanIdepth the score used */
+  double frame_to_keyframe[7];
+  double cam_to_world[8];
+  lsdg_track_stats stats;
+} lsdg_system_frame;
+
+lsdg_status lsdg_system_create(lsdg_ctx ctx, const lsdg_camera* cam, int32_t precision,
+                               const lsdg_system_params* params, lsdg_system* out);
+lsdg_status lsdg_system_destroy(lsdg_system s);
+/* SlamSystem::gtDepthInit: the first image (8-bit, row stride `stride`) with
+ * its inverse depth and variance (idepth <= 0: none) becomes the first
+ * keyframe, its depth map initialised from them (initializeFromGTDepth). */
+lsdg_status lsdg_system_init_gt(lsdg_system s, int32_t id, const uint8_t* image, int32_t stride,
+                                const float* idepth, const float* idepth_var);
+/* SlamSystem::trackFrame(image, id, blockUntilMapped = true). */
+lsdg_status lsdg_system_track(lsdg_system s, int32_t id, const uint8_t* image, int32_t stride,
+                              lsdg_system_frame* out);
+/* The keyframes so far, oldest first: ids and camToWorld (8 doubles each);
+ * *count receives their number (at most cap are written). */
+lsdg_status lsdg_system_keyframes(lsdg_system s, int32_t* ids, double* cam_to_world, int32_t cap, int32_t* count);
+/* The current keyframe's depth map hypotheses (lsdg_depth_read). */
+lsdg_status lsdg_system_depth_read(lsdg_system s, lsdg_hypotheses* out);
+
 /* ----------------------------------------------------------------- ingest */
 
 /* libvideoio's Undistorter, which tools/LSD.cpp:88 builds from the

@@ -156,6 +156,20 @@ class ConstraintResult(C.Structure):
                 ("candidate_on_frame", ConstraintEdge), ("frame_on_candidate", ConstraintEdge)]
 
 
+class SystemParams(C.Structure):
+    _fields_ = [("kf_dist_weight", C.c_float), ("kf_usage_weight", C.c_float), ("min_num_mapped", C.c_int32),
+                ("initialization_phase_count", C.c_int32)]
+
+
+class SystemFrame(C.Structure):
+    _fields_ = [("id", C.c_int32), ("status", C.c_int32), ("keyframe_id", C.c_int32), ("new_keyframe", C.c_int32),
+                ("closeness_score", C.c_float), ("keyframe_mean_idepth", C.c_float),
+                ("frame_to_keyframe", C.c_double * 7), ("cam_to_world", C.c_double * 8), ("stats", TrackStats)]
+
+
+SYSTEM_TRACKED, SYSTEM_LOST, SYSTEM_NOT_TRACKED = 0, 1, 2
+
+
 class PointDense(C.Structure):
     """InputPointDense (lib/Pangolin_IOWrapper/Keyframe.h:16-21)."""
     _fields_ = [("idepth", C.c_float), ("idepth_var", C.c_float), ("color", C.c_uint8 * 4)]
@@ -276,6 +290,13 @@ def load():
         "lsdg_frame_create_undistorted": (i32, [vp, i32, C.POINTER(Camera), vp, fp, i32, i32, fp, C.POINTER(vp)]),
         "lsdg_frame_pose_txt": (i32, [vp, C.c_char_p, C.c_size_t]),
         "lsdg_frame_ingest": (i32, [vp, i32, vp, fp, i32, fp]),
+        "lsdg_system_params_default": (None, [C.POINTER(SystemParams)]),
+        "lsdg_system_create": (i32, [vp, C.POINTER(Camera), i32, C.POINTER(SystemParams), C.POINTER(vp)]),
+        "lsdg_system_destroy": (i32, [vp]),
+        "lsdg_system_init_gt": (i32, [vp, i32, fp, i32, fp, fp]),
+        "lsdg_system_track": (i32, [vp, i32, fp, i32, C.POINTER(SystemFrame)]),
+        "lsdg_system_keyframes": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_double), i32, C.POINTER(i32)]),
+        "lsdg_system_depth_read": (i32, [vp, C.POINTER(Hypotheses)]),
         "lsdg_constraint_search": (i32, [vp, C.POINTER(ConstraintCandidate), i32, f32,
                                          C.POINTER(ConstraintResult)]),
     }
@@ -405,6 +426,14 @@ class Context:
                                           C.byref(n)), "lsdg_points_vbo")
         return out[:n.value]
 
+    def system(self, cam, precision=FP32, params=None):
+        """lsd-slam's SlamSystem front end (tracking + mapping) on this context."""
+        h = C.c_void_p()
+        cc = Camera.of(cam)
+        self.check(self.L.lsdg_system_create(self.h, C.byref(cc), precision, None if params is None else C.byref(params),
+                                             C.byref(h)), "lsdg_system_create")
+        return System(self, h, cam)
+
     def constraint_search(self, candidates, strictness=1.5):
         """SlamSystem::testConstraint for [(frame_ref, candidate_ref, candidate_to_frame_init8), ...]."""
         n = len(candidates)
@@ -452,6 +481,44 @@ class Context:
         h = C.c_void_p()
         self.check(self.L.lsdg_batch_create(self.h, arr, len(jobs), C.byref(h)), "lsdg_batch_create")
         return Batch(self, h, len(jobs))
+
+
+class System:
+    def __init__(self, ctx, h, cam):
+        self.ctx, self.h, self.cam = ctx, h, cam
+
+    def init_gt(self, fid, image, idepth, var):
+        img = np.ascontiguousarray(image, np.uint8)
+        a, pa = _f32(idepth)
+        b, pb = _f32(var)
+        self.ctx.check(self.ctx.L.lsdg_system_init_gt(self.h, fid, img.ctypes.data, img.shape[1], pa, pb),
+                       "lsdg_system_init_gt")
+
+    def track(self, fid, image):
+        img = np.ascontiguousarray(image, np.uint8)
+        out = SystemFrame()
+        self.ctx.check(self.ctx.L.lsdg_system_track(self.h, fid, img.ctypes.data, img.shape[1], C.byref(out)),
+                       "lsdg_system_track")
+        return out
+
+    def keyframes(self):
+        n = C.c_int32()
+        self.ctx.check(self.ctx.L.lsdg_system_keyframes(self.h, None, None, 0, C.byref(n)), "lsdg_system_keyframes")
+        ids = (C.c_int32 * max(n.value, 1))()
+        c2w = (C.c_double * (8 * max(n.value, 1)))()
+        self.ctx.check(self.ctx.L.lsdg_system_keyframes(self.h, ids, c2w, n.value, C.byref(n)), "lsdg_system_keyframes")
+        return list(ids[:n.value]), np.array(c2w[:8 * n.value]).reshape(-1, 8)
+
+    def depth_read(self):
+        n = self.cam["w"] * self.cam["h"]
+        h, arrs = hyp_arrays(n)
+        self.ctx.check(self.ctx.L.lsdg_system_depth_read(self.h, C.byref(h)), "lsdg_system_depth_read")
+        return arrs
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.lsdg_system_destroy(self.h)
+            self.h = None
 
 
 def fov_undistort_table(in_k, omega, in_size, out_k, out_size):

@@ -205,3 +205,22 @@ def batch_instances(n=64, seed=1003, max_trans=0.05, max_rot_deg=3.0, resolution
                                  uniform=True, threshold=None, fraction=float(fr[i]))
         out.append(inst)
     return out
+
+
+def vo_sequence(seed, n, cam=None, step=(0.004, 0.001, 0.012), yaw_deg=0.15, noise=1.0, fraction=0.3):
+    """A camera moving steadily through one scene (the SlamSystem workload):
+    frame k at camToWorld (0, sin(k a / 2), 0, cos(k a / 2), k * step) with
+    a = yaw_deg per frame; 8-bit images with sensor noise, the first frame's
+    semi-dense inverse depth (the gtDepthInit input) and every camToWorld."""
+    cam = cam or camera(320, 240)
+    sc = Scene(seed)
+    imgs, poses = [], []
+    for k in range(n):
+        a = np.deg2rad(yaw_deg) * k
+        pose = np.array([0.0, np.sin(a / 2), 0.0, np.cos(a / 2), k * step[0], k * step[1], k * step[2]])
+        img, dep = sc.render(pose, cam, noise, seed * 7919 + k)
+        if k == 0:
+            idepth, var, _, _ = semi_dense_depth(img, dep, seed * 7919 + 999, fraction=fraction)
+        imgs.append(img)
+        poses.append(pose)
+    return dict(cam=cam, images=imgs, poses=np.array(poses), idepth=idepth, var=var)

@@ -399,3 +399,105 @@ def ldlt(A, b):
     ba, pb = _dp(b)
     (load().orc_ldlt6 if n == 6 else load().orc_ldlt7)(pA, pb, x)
     return np.array(x[:])
+
+
+class OSystem:
+    """lsd-slam's SlamSystem front end restated on the oracle (trackFrame's
+    tracking, loss test and keyframe selection, then the mapping iterations
+    trackFrame(blockUntilMapped) waits for), single-threaded, with SLAM
+    enabled (keyframesAll counts the finished keyframes).  Poses are Sim3
+    8-vectors (qx, qy, qz, qw, tx, ty, tz, s)."""
+
+    KF_DIST_WEIGHT, KF_USAGE_WEIGHT, MIN_NUM_MAPPED, INIT_PHASE = 4.0, 3.0, 5, 5
+
+    def __init__(self, o, cam):
+        self.o, self.cam = o, cam
+        self.dm = o.depthmap(cam)
+        self.kf = None
+        self.kf_c2w = None
+        self.keyframes = []   # (frame, camToWorld)
+        self.unmapped = []    # frames, oldest first
+        self.latest = None
+        self.last_c2w = None
+        self.good = True
+        self.create = False
+
+    def init_gt(self, fid, image, idepth, var):
+        f = self.o.frame(fid, self.cam, np.asarray(image, np.float32))
+        f.set_depth(idepth, var)
+        self.dm.init_gt(f)
+        c2w = np.r_[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
+        f.set_cam_to_world(c2w)
+        self.kf, self.kf_c2w = f, c2w
+        self.keyframes.append((f, c2w))
+        self.last_c2w = c2w
+
+    def _mean_idepth(self):
+        d = self.kf.read(lsdg.WHAT_IDEPTH, 0).ravel()
+        v = d[d != np.float32(-1.0)]
+        return np.cumsum(v, dtype=np.float32)[-1] / np.float32(len(v))
+
+    def _map_until_idle(self, info):
+        while True:
+            if self.create:
+                self.dm.finalize()
+                nk = self.latest
+                self.dm.create_keyframe(nk)
+                pid, tp, _ = nk.get_tracking()
+                c2w = sim3_mul(self.kf_c2w, tp)
+                nk.set_cam_to_world(c2w)
+                self.last_c2w = c2w
+                self.kf, self.kf_c2w = nk, c2w
+                self.keyframes.append((nk, c2w))
+                self.create = False
+                if info["id"] == nk.id:
+                    info["new_keyframe"] = 1
+                    info["cam_to_world"] = c2w
+                continue
+            while self.unmapped and self.unmapped[0].get_tracking()[0] != self.kf.id:
+                self.unmapped.pop(0)
+            if not self.unmapped:
+                return
+            self.dm.update(list(self.unmapped))
+            self.unmapped.pop(0)
+
+    def track(self, fid, image):
+        info = dict(id=fid, status=0, keyframe_id=-1, new_keyframe=0, closeness_score=-1.0)
+        if not self.good:
+            info["status"] = 2
+            return info
+        f = self.o.frame(fid, self.cam, np.asarray(image, np.float32))
+        my_create = self.create
+        ref = self.o.ref(self.kf)
+        rel = sim3_mul(sim3_inv(self.kf_c2w), self.last_c2w)
+        init = list(rel[:7])
+        pose, st, status = self.o.se3_track(ref, f, init)
+        info.update(keyframe_id=self.kf.id, frame_to_keyframe=pose, stats=st)
+        n_kf = len(self.keyframes) - 1
+        if st.diverged or (n_kf > self.INIT_PHASE and not st.tracking_was_good):
+            self.good = False
+            info["status"] = 1
+            return info
+        c2w = sim3_mul(self.kf_c2w, np.r_[pose, 1.0])
+        f.set_cam_to_world(c2w)
+        self.last_c2w = c2w
+        self.latest = f
+        info["cam_to_world"] = c2w
+        if not my_create and self.kf.counters()[1] > self.MIN_NUM_MAPPED:
+            mean = self._mean_idepth()
+            dist = np.asarray(pose[4:7], np.float64) * np.float64(mean)
+            f32 = np.float32
+            min_val = min(f32(0.2) + f32(n_kf) * f32(0.8) / f32(self.INIT_PHASE), f32(1.0))
+            if n_kf < self.INIT_PHASE:
+                min_val = f32(min_val * f32(0.7))
+            u = f32(st.point_usage)
+            d2 = f32(float(np.dot(dist, dist)))
+            w, wu = f32(self.KF_DIST_WEIGHT), f32(self.KF_USAGE_WEIGHT)
+            score = f32(f32(f32(d2 * w) * w) + f32(f32(f32(f32(f32(1) - u) * f32(f32(1) - u)) * wu) * wu))
+            info["closeness_score"] = float(score)
+            info["keyframe_mean_idepth"] = float(mean)
+            if score > min_val:
+                self.create = True
+        self.unmapped.append(f)
+        self._map_until_idle(info)
+        return info

@@ -36,13 +36,14 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lsdg.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
                    'sizeof(lsdg_params), '
                    'sizeof(lsdg_track_stats), sizeof(lsdg_eval), sizeof(lsdg_hypotheses), sizeof(lsdg_camera), '
                    'sizeof(lsdg_job), sizeof(lsdg_job_result), offsetof(lsdg_params, stereo_max_steps), '
                    'sizeof(lsdg_point_dense), sizeof(lsdg_vertex), sizeof(lsdg_vbo_params), '
                    'sizeof(lsdg_constraint_candidate), sizeof(lsdg_constraint_edge), sizeof(lsdg_constraint_result), '
-                   'offsetof(lsdg_constraint_result, frame_on_candidate));return 0;}\n')
+                   'offsetof(lsdg_constraint_result, frame_on_candidate), sizeof(lsdg_system_params), '
+                   'sizeof(lsdg_system_frame), offsetof(lsdg_system_frame, stats));return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     got = [int(v) for v in subprocess.check_output([str(exe)]).split()]
@@ -50,7 +51,8 @@ def test_struct_layouts_match_c(tmp_path):
             C.sizeof(lsdg.Camera), C.sizeof(lsdg.Job), C.sizeof(lsdg.JobResult), lsdg.Params.stereo_max_steps.offset,
             C.sizeof(lsdg.PointDense), C.sizeof(lsdg.Vertex), C.sizeof(lsdg.VboParams),
             C.sizeof(lsdg.ConstraintCandidate), C.sizeof(lsdg.ConstraintEdge), C.sizeof(lsdg.ConstraintResult),
-            lsdg.ConstraintResult.frame_on_candidate.offset]
+            lsdg.ConstraintResult.frame_on_candidate.offset, C.sizeof(lsdg.SystemParams), C.sizeof(lsdg.SystemFrame),
+            lsdg.SystemFrame.stats.offset]
     assert got == want
 
 
