@@ -224,7 +224,7 @@ def ingest_leg(ctx, steps, warmup, cpu_seconds):
     for k in range(steps):
         f.ingest(k, und, src)
     t_reuse = time.perf_counter() - t0
-    achieved, ks = kernel_roofline(ctx, lambda: f.ingest(0, und, src), lsdg.KERNEL_INGEST)
+    achieved, ks = kernel_roofline(ctx, lambda: [f.ingest(k, und, src) for k in range(20)], lsdg.KERNEL_INGEST)
     f.destroy()
     und.close()
     out = {"ingest_call_ms": round(1e3 * t_reuse / steps, 4), "ingest_new_frame_call_ms": round(1e3 * t_new / steps, 4),
@@ -398,6 +398,8 @@ def main():
                     help="run only the batched FP16 leg (for kernel profiling)")
     ap.add_argument("--depth-only", action="store_true",
                     help="run only the depth-map update leg (for kernel profiling)")
+    ap.add_argument("--ingest-only", action="store_true",
+                    help="run only the ingest leg (for kernel profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -422,6 +424,9 @@ def main():
                           "depth_update_ms": round(depth["ms_per_update"], 4),
                           "depth_device_ms_per_update": round(depth["device_ms_per_update"], 4),
                           "depth_roofline_gbs": depth["roofline_gbs"]}))
+        return
+    if args.ingest_only:
+        print(json.dumps(ingest_leg(ctx, args.steps, args.warmup, 0.0)))
         return
     insts = [synth.tracking_instance(1001 + 17 * i) for i in range(args.batch)]
     B = args.batch
