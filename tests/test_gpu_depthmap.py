@@ -106,6 +106,27 @@ def test_update_and_create_keyframe_bit_exact(ctx, depth_inst, crop):
     assert abs(g[4][1][7] - o0[4][1][7]) <= 1e-4 * abs(o0[4][1][7])
 
 
+@pytest.mark.parametrize("size", [(200, 136), (328, 250)])
+def test_ragged_sizes_bit_exact(ctx, size):
+    """Image sizes that are not multiples of the 16x16 tiles (partial tiles
+    at the right and bottom edges of every per-pixel and tile pass)."""
+    inst = synth.depth_instance(1017, cam=synth.camera(*size))
+    p = _params(100)
+    ctx.set_params(p)
+    try:
+        g = _scenario(ctx, inst, n_refs=2, updates=2)
+    finally:
+        ctx.set_params(lsdg.default_params())
+    o2 = _scenario(pyoracle.Oracle(2, params=p), inst, n_refs=2, updates=2)
+    o0 = _scenario(pyoracle.Oracle(0, params=p), inst, n_refs=2, updates=2)
+    assert not _diff(g[0], o0[0]), _diff(g[0], o0[0])
+    assert not _diff(g[1], o2[1]), _diff(g[1], o2[1])
+    for a, b in zip(g[3], o2[3]):
+        assert np.array_equal(_bits(a), _bits(b))
+    codes = np.bincount(g[0]["observe_code"], minlength=15)
+    assert codes[lsdg.OBS["UPDATED"]] > 50 and codes[lsdg.OBS["CREATED"]] > 50
+
+
 @pytest.mark.parametrize("ids", [(1, 3, 4), (1, 40, 41)], ids=["inline", "byid-in-device-memory"])
 def test_update_with_tracking_masks_and_several_refs(ctx, depth_inst, ids):
     """refPixelWasGood gates (from a seeded mask), three reference frames with

@@ -31,6 +31,8 @@ CASES = {
     "vga-fov": ((640, 480), (330.0, 330.0, 319.5, 239.5), 0.92, (270.0, 270.0, 319.5, 239.5), (640, 480)),
     "wvga-to-vga": ((752, 480), (360.0, 358.0, 375.5, 241.0), 0.85, (300.0, 300.0, 319.5, 239.5), (640, 480)),
     "pinhole-crop": ((400, 300), (300.0, 300.0, 199.5, 149.5), 0.0, (300.0, 300.0, 159.5, 119.5), (320, 240)),
+    # output tiles cut at the right and bottom edges
+    "ragged": ((230, 171), (150.0, 150.0, 114.5, 85.0), 0.7, (130.0, 130.0, 104.5, 74.5), (210, 150)),
 }
 
 
