@@ -141,7 +141,7 @@ extern "C" lsdg_status lsdg_frame_ingest(lsdg_frame f, int32_t id, lsdg_undistor
   // depth version moves on so point clouds built from the old content are
   // rebuilt
   CK(c, hipStreamSynchronize(c->stream));
-  lsdg_frame_s fresh;
+  lsdg_frame_s fresh{};
   fresh.ctx = c;
   fresh.id = id;
   std::memcpy(fresh.w, f->w, sizeof(f->w));
