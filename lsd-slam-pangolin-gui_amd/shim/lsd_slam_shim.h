@@ -1,4 +1,4 @@
-// lsd_slam::SE3Tracker / Sim3Tracker / DepthMap on top of liblsdg.so.
+// lsd_slam::SE3Tracker / Sim3Tracker / DepthMap / SlamSystem on top of liblsdg.so.
 //
 // The upstream classes (lsd-slam's Tracking/SE3Tracker.h, Tracking/
 // Sim3Tracker.h, DepthEstimation/DepthMap.h, linked by the reference as the
@@ -268,6 +268,53 @@ class DepthMap {
 
  private:
   lsdg_depthmap dm_ = nullptr;
+};
+
+// SlamSystem's front end (trackFrame with blockUntilMapped, gtDepthInit) on
+// one thread; no pose graph, loop closure or relocalisation (see
+// lsdg_system_track).  Images are 8-bit, row stride `stride` bytes.
+class SlamSystem {
+ public:
+  SlamSystem(Context& ctx, int w, int h, float fx, float fy, float cx, float cy, int32_t precision = LSDG_FP32)
+      : w_(w), h_(h) {
+    lsdg_camera cam{w, h, fx, fy, cx, cy};
+    if (lsdg_system_create(ctx.get(), &cam, precision, nullptr, &s_) != LSDG_OK) s_ = nullptr;
+  }
+  ~SlamSystem() {
+    if (s_) lsdg_system_destroy(s_);
+  }
+  SlamSystem(const SlamSystem&) = delete;
+  SlamSystem& operator=(const SlamSystem&) = delete;
+
+  // gtDepthInit(image, depth, timeStamp, id): depth in metres (<= 0: none);
+  // the frame's variance is VAR_GT_INIT_INITIAL, as setDepthFromGroundTruth sets it.
+  bool gtDepthInit(const uint8_t* image, int stride, const float* depth, int id) {
+    lsdg_params p;
+    lsdg_params_default(&p);
+    std::vector<float> idepth((size_t)w_ * h_), var((size_t)w_ * h_);
+    for (size_t i = 0; i < idepth.size(); ++i) {
+      const bool ok = depth[i] > 0.0f && depth[i] == depth[i];
+      idepth[i] = ok ? 1.0f / depth[i] : -1.0f;
+      var[i] = ok ? p.var_gt_init_initial : -1.0f;
+    }
+    return lsdg_system_init_gt(s_, id, image, stride, idepth.data(), var.data()) == LSDG_OK;
+  }
+  // trackFrame(image, frameID, blockUntilMapped = true, timestamp); the
+  // per-frame outcome (pose, keyframe decision, tracker statistics) in `out`.
+  bool trackFrame(const uint8_t* image, int stride, int frameID, lsdg_system_frame* out = nullptr) {
+    lsdg_system_frame f;
+    const bool ok = lsdg_system_track(s_, frameID, image, stride, &f) == LSDG_OK;
+    if (ok) trackingIsGood_ = f.status == LSDG_SYSTEM_TRACKED;
+    if (out) *out = f;
+    return ok;
+  }
+  bool trackingIsGood() const { return trackingIsGood_; }
+  lsdg_system handle() const { return s_; }
+
+ private:
+  int w_, h_;
+  lsdg_system s_ = nullptr;
+  bool trackingIsGood_ = true;
 };
 
 }  // namespace lsd_slam
