@@ -276,7 +276,7 @@ lsdg_status lsdg::frame_require(lsdg_frame f, unsigned need) {
   bool buildImages = (f->valid & images) != images && (miss & ~level0) != 0;
   unsigned grads = miss & derived;
   ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);
-  if (buildImages) CK(c, launch_pyr_images(f, nullptr, c->stream));
+  if (buildImages) CK(c, launch_pyr_images(f, false, c->stream));
   if (grads) CK(c, launch_pyr_grads(f, grads, c->stream));
   prof_end(c, pm, pyramid_bytes(f, buildImages, grads));
   f->valid |= (buildImages ? images : 0u) | grads;
@@ -331,9 +331,12 @@ extern "C" lsdg_status lsdg_frame_create_u8(lsdg_ctx c, int32_t id, const lsdg_c
   if (e == hipSuccess) {
     // the image stage converts level 0 and builds levels 1-4; the rest is built on first use
     ProfMark pm = prof_begin(c, LSDG_KERNEL_PYRAMID);
-    e = launch_pyr_images(f, f->d_input_u8, c->stream);
+    e = launch_pyr_images(f, true, c->stream);
     prof_end(c, pm, pyramid_bytes(f, true, 0));
-    if (e == hipSuccess) f->valid = fd_levels(FD_IMAGE, 0, PYR - 1);
+    if (e == hipSuccess) {
+      f->valid = fd_levels(FD_IMAGE, 0, PYR - 1);
+      f->u8Level0 = true;
+    }
     else st = fail(c, LSDG_EDEVICE, "frame pyramid", e);
   } else {
     st = fail(c, LSDG_EDEVICE, "upload image", e);

@@ -94,15 +94,17 @@ static lsdg_status ingest_into(lsdg_frame f, lsdg_undistorter u, const uint8_t* 
   hipError_t e = hipMemcpyAsync(u->d_src, u->h_src, (size_t)u->inW * u->inH, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     ProfMark pm = prof_begin(c, LSDG_KERNEL_INGEST);
-    e = launch_ingest(f, u->d_src, u->inW, u->d_map, undistorted ? f->d_input_u8 : nullptr, c->stream);
+    // the 8-bit copy is always written: later pyramid rebuilds read it
+    e = launch_ingest(f, u->d_src, u->inW, u->d_map, f->d_input_u8, c->stream);
     // table 8 B, one input byte, level 0 (4 B) and the 8-bit copy per output pixel, plus levels 1-4
-    prof_end(c, pm, pyramid_bytes(f, true, 0) + (double)w * h * (8.0 + 1.0 + (undistorted ? 1.0 : 0.0)));
+    prof_end(c, pm, pyramid_bytes(f, true, 0) + (double)w * h * (8.0 + 1.0 + 1.0));
   }
   if (e == hipSuccess && undistorted)
     e = hipMemcpyAsync(undistorted, f->d_input_u8, (size_t)w * h, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, LSDG_EDEVICE, "ingest", e);
   f->valid = fd_levels(FD_IMAGE, 0, PYR - 1);  // gradients are built on first use
+  f->u8Level0 = true;
   return LSDG_OK;
 }
 

@@ -185,6 +185,7 @@ struct lsdg_frame_s {
   float* d_idepth[lsdg::PYR];
   float* d_ivar[lsdg::PYR];
   uint8_t* d_input_u8 = nullptr;  // staging for u8 uploads (w*h)
+  bool u8Level0 = false;          // d_input_u8 holds level 0 (every level-0 writer keeps it so)
   uint8_t* d_wasGood = nullptr;   // refPixelWasGood (level 1)
   unsigned valid = 0;             // built pyramid data (fd_bit flags)
   bool wasGoodAllocated = false;
@@ -254,7 +255,10 @@ struct lsdg_depthmap_s {
 namespace lsdg {
 
 // pyramid.hip
-hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s);
+// image pyramid of f: with fromU8, level 0 converted from the 8-bit staging
+// and levels 1-4; otherwise levels 1-4 from level 0 (read from the 8-bit
+// copy when it still holds level 0)
+hipError_t launch_pyr_images(lsdg_frame f, bool fromU8, hipStream_t s);
 // undistorts the 8-bit image d_src (row length inW) through the table d_map
 // (one source position per level-0 pixel) into f's level 0, builds levels
 // 1-4, and writes the 8-bit undistorted image to d_undist unless it is null

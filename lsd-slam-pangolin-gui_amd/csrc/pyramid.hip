@@ -20,6 +20,7 @@ namespace lsdg {
 struct PyrImgArgs {
   float* img[PYR];
   int w[PYR], h[PYR];
+  const uint8_t* u8;  // the 8-bit level 0 when it holds the image (read instead of the float copy), or null
 };
 
 // Levels 2-4 from a block's 16x16 level-1 tile L1 (level 1 already stored).
@@ -59,11 +60,14 @@ __device__ inline void pyr_levels_2to4(const PyrImgArgs& p, const float (*L1)[17
   }
 }
 
-__device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __restrict__ u8, int bx, int by) {
+// Levels 1-4 of a 32x32 level-0 tile, read from the 8-bit image u8 when it
+// is set (writing level 0 from it too when writeL0) or from the float level 0.
+__device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __restrict__ u8, bool writeL0, int bx,
+                                       int by) {
   __shared__ float L1[16][17];
   const int t = threadIdx.x;
   const int w0 = p.w[0], h0 = p.h[0];
-  if (u8) {
+  if (u8 && writeL0) {
     for (int k = t; k < 1024; k += 256) {
       int x = bx * 32 + (k & 31), y = by * 32 + (k >> 5);
       if (x < w0 && y < h0) p.img[0][x + y * w0] = (float)u8[x + y * w0];
@@ -92,8 +96,8 @@ __device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __res
   pyr_levels_2to4(p, L1, bx, by);
 }
 
-__global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, const uint8_t* __restrict__ u8) {
-  pyr_images_tile(p, u8, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, int writeL0) {
+  pyr_images_tile(p, p.u8, writeL0 != 0, blockIdx.x, blockIdx.y);
 }
 
 // Undistorter::undistort for one output pixel: the source position m (from
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(256) void k_pyr_images_batch(const PyrImgArgs* __re
                                                           int nf) {
   int f = frame_of_block(off, nf, blockIdx.x);
   int local = blockIdx.x - off[f];
-  pyr_images_tile(args[f], nullptr, local % tx[f], local / tx[f]);
+  pyr_images_tile(args[f], args[f].u8, false, local % tx[f], local / tx[f]);
 }
 
 __global__ __launch_bounds__(256) void k_pyr_grads_batch(const PyrGradArgs* __restrict__ args,
@@ -435,15 +439,18 @@ __global__ __launch_bounds__(256) void k_pc_write(PcWriteArgs a, const int* __re
 }
 
 // ------------------------------------------------------------ launchers
-hipError_t launch_pyr_images(lsdg_frame f, const uint8_t* d_u8, hipStream_t s) {
+hipError_t launch_pyr_images(lsdg_frame f, bool fromU8, hipStream_t s) {
   PyrImgArgs p;
   for (int l = 0; l < PYR; ++l) {
     p.img[l] = f->d_image[l];
     p.w[l] = f->w[l];
     p.h[l] = f->h[l];
   }
+  // level 0 from the 8-bit staging (converted and written), or levels 1-4
+  // only, read from the 8-bit copy when it still holds level 0
+  p.u8 = (fromU8 || f->u8Level0) ? f->d_input_u8 : nullptr;
   dim3 grid((f->w[0] + 31) / 32, (f->h[0] + 31) / 32);
-  hipLaunchKernelGGL(k_pyr_images, grid, dim3(256), 0, s, p, d_u8);
+  hipLaunchKernelGGL(k_pyr_images, grid, dim3(256), 0, s, p, fromU8 ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -511,6 +518,7 @@ void pyr_batch_tables(lsdg_frame const* fs, const unsigned* need, int nf, void* 
       p.w[l] = f->w[l];
       p.h[l] = f->h[l];
     }
+    p.u8 = f->u8Level0 ? f->d_input_u8 : nullptr;
     int blk = pyr_grad_args(f, need[k], q);
     ia[k] = p;
     ga[k] = q;
