@@ -312,6 +312,13 @@ int32_t lsdg_depth_is_valid(lsdg_depthmap dm);
 /* DepthMap::initializeFromGTDepth(Frame*): the keyframe's idepth becomes the
  * hypotheses (var VAR_GT_INIT_INITIAL, validity 20). */
 lsdg_status lsdg_depth_initialize_from_gt(lsdg_depthmap dm, lsdg_frame keyframe);
+/* DepthMap::initializeRandomly(Frame*): every interior pixel whose level-0
+ * max-gradient exceeds MIN_ABS_GRAD_CREATE gets inverse depth
+ * 0.5 + (draw % 100001) / 100000, variance VAR_RANDOM_INIT_INITIAL and
+ * validity 20; the others lose their hypothesis.  The draws come from
+ * SplitMix64 seeded with `seed` (output i + 1 for pixel i, top 31 bits)
+ * instead of the reference's rand(), so a seed reproduces the map. */
+lsdg_status lsdg_depth_initialize_randomly(lsdg_depthmap dm, lsdg_frame keyframe, uint64_t seed);
 /* Prior hypotheses from arrays (idepth, var, validity) on the keyframe, then
  * setDepth.  Pixels with var <= 0 or non-finite idepth are invalid. */
 lsdg_status lsdg_depth_set_prior(lsdg_depthmap dm, lsdg_frame keyframe, const float* idepth,
@@ -526,6 +533,9 @@ lsdg_status lsdg_system_destroy(lsdg_system s);
  * keyframe, its depth map initialised from them (initializeFromGTDepth). */
 lsdg_status lsdg_system_init_gt(lsdg_system s, int32_t id, const uint8_t* image, int32_t stride,
                                 const float* idepth, const float* idepth_var);
+/* SlamSystem::randomInit: the first image becomes the first keyframe with a
+ * random depth map (lsdg_depth_initialize_randomly with `seed`). */
+lsdg_status lsdg_system_init_random(lsdg_system s, int32_t id, const uint8_t* image, int32_t stride, uint64_t seed);
 /* SlamSystem::trackFrame(image, id, blockUntilMapped = true). */
 lsdg_status lsdg_system_track(lsdg_system s, int32_t id, const uint8_t* image, int32_t stride,
                               lsdg_system_frame* out);

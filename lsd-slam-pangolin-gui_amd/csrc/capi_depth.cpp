@@ -156,6 +156,23 @@ extern "C" lsdg_status lsdg_depth_initialize_from_gt(lsdg_depthmap dm, lsdg_fram
   return LSDG_OK;
 }
 
+extern "C" lsdg_status lsdg_depth_initialize_randomly(lsdg_depthmap dm, lsdg_frame kf, uint64_t seed) {
+  if (!dm) return LSDG_EINVAL;
+  lsdg_status st = check_frame(dm, kf);
+  if (st != LSDG_OK) return st;
+  lsdg_ctx c = dm->ctx;
+  st = frame_require(kf, fd_bit(FD_MAXG, 0));
+  if (st != LSDG_OK) return st;
+  dm->kf = kf;
+  dm->kfReactivated = false;
+  dm->valid = true;
+  CK(c, launch_random_init(dm, c->P, (unsigned long long)seed, c->stream));
+  st = set_depth(dm, kf);
+  if (st != LSDG_OK) return st;
+  CK(c, hipStreamSynchronize(c->stream));
+  return LSDG_OK;
+}
+
 extern "C" lsdg_status lsdg_depth_set_prior(lsdg_depthmap dm, lsdg_frame kf, const float* idepth, const float* var,
                                             int32_t validity) {
   if (!dm || !idepth || !var) return LSDG_EINVAL;

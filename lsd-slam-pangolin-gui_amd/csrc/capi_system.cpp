@@ -109,6 +109,24 @@ extern "C" lsdg_status lsdg_system_init_gt(lsdg_system s, int32_t id, const uint
   return LSDG_OK;
 }
 
+extern "C" lsdg_status lsdg_system_init_random(lsdg_system s, int32_t id, const uint8_t* image, int32_t stride,
+                                               uint64_t seed) {
+  if (!s || !image) return LSDG_EINVAL;
+  if (s->initialised) return fail(s->ctx, LSDG_ESTATE, "system already initialised");
+  lsdg_frame f;
+  lsdg_status st = lsdg_frame_create_u8(s->ctx, id, &s->cam, image, stride, s->precision, &f);
+  if (st != LSDG_OK) return st;
+  FramePtr fp = own(f);
+  st = lsdg_depth_initialize_randomly(s->dm, f, seed);
+  if (st != LSDG_OK) return st;
+  f->camToWorld = pose_identity();
+  s->kf = fp;
+  s->keyframes.push_back(fp);
+  s->lastCamToWorld = f->camToWorld;
+  s->initialised = true;
+  return LSDG_OK;
+}
+
 // Frame::meanIdepth of the keyframe: the float mean, in raster order, of the
 // level-0 idepth setDepth wrote (pixels without depth hold -1).
 static lsdg_status mean_idepth(lsdg_system s, float* out) {

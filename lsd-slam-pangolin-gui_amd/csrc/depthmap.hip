@@ -1389,6 +1389,30 @@ __global__ void k_hyp_prior(HypBuf cur, int n, const float* __restrict__ id, con
   store_hyp(cur, i, h);
 }
 
+// DepthMap::initializeRandomly: interior pixels whose keyframe max-gradient
+// exceeds MIN_ABS_GRAD_CREATE get DepthMapPixelHypothesis(r, r, VAR_RANDOM_INIT_INITIAL,
+// VAR_RANDOM_INIT_INITIAL, 20) with r = 0.5 + (draw % 100001) / 100000, the
+// others lose their hypothesis (flag and blacklist cleared, the rest kept);
+// the border is untouched.  The draw is random_init_draw in place of the
+// reference's rand().
+__global__ void k_random_init(HypBuf cur, int w, int h, const float* __restrict__ maxg, float minGrad, float var,
+                              unsigned long long seed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w * h) return;
+  const int y = i / w, x = i - y * w;
+  if (x < 1 || x >= w - 1 || y < 1 || y >= h - 1) return;
+  if (maxg[i] > minGrad) {
+    const float id = 0.5f + 1.0f * ((float)(random_init_draw(seed, i) % 100001) / 100000.0f);
+    Hyp t = make_hyp(id, var, 20);
+    t.dv.z = id;
+    t.dv.w = var;
+    store_hyp(cur, i, t);
+  } else {
+    cur.valid[i] = 0;
+    cur.bl[i] = 0;
+  }
+}
+
 // ------------------------------------------------------------ launchers
 static inline dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }
 
@@ -1524,6 +1548,13 @@ hipError_t launch_hyp_prior(lsdg_depthmap dm, const float* d_id, const float* d_
                             float gtVar, hipStream_t s) {
   int n = dm->g.w * dm->g.h;
   k_hyp_prior<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(dm->buf[dm->cur], n, d_id, d_var, validity, gt, gtVar);
+  return hipGetLastError();
+}
+
+hipError_t launch_random_init(lsdg_depthmap dm, const lsdg_params& P, unsigned long long seed, hipStream_t s) {
+  const int n = dm->g.w * dm->g.h;
+  k_random_init<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(dm->buf[dm->cur], dm->g.w, dm->g.h, dm->kf->d_maxg[0],
+                                                            P.min_abs_grad_create, P.var_random_init_initial, seed);
   return hipGetLastError();
 }
 

@@ -307,6 +307,8 @@ hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool remove
 hipError_t launch_fill_regularize(lsdg_depthmap dm, const lsdg_params& P, int th, lsdg_frame setDepth,
                                   hipStream_t s);
 hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s);
+// DepthMap::initializeRandomly on dm->kf's level-0 max-gradients
+hipError_t launch_random_init(lsdg_depthmap dm, const lsdg_params& P, unsigned long long seed, hipStream_t s);
 hipError_t launch_propagate(lsdg_depthmap dm, const lsdg_params& P, lsdg_frame nkf, const Xf& oldToNew,
                             const uint8_t* d_wasGood, hipStream_t s);  // swaps buffers
 hipError_t launch_mean_idepth(lsdg_depthmap dm, hipStream_t s);  // h_red[0] = count, h_red[1] = sum

@@ -223,6 +223,17 @@ LSDG_HD void pose_roll_f(const Pose& T, float* r4) {
 
 // LDL^T solve of a symmetric N x N system (row-major); zero pivots give zero
 // solution components.
+// The 31-bit draw DepthMap::initializeRandomly takes from rand() upstream,
+// here deterministic: the high bits of SplitMix64's output number i + 1 from
+// state `seed` (Steele, Lea, Flood 2014).
+LSDG_HD int random_init_draw(unsigned long long seed, int i) {
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (int)(z >> 33);
+}
+
 template <int N>
 LSDG_HD void ldlt_solve(const double* A, const double* b, double* x) {
   double L[N][N];

@@ -267,6 +267,7 @@ def load():
         "lsdg_depth_reset": (i32, [vp]),
         "lsdg_depth_is_valid": (i32, [vp]),
         "lsdg_depth_initialize_from_gt": (i32, [vp, vp]),
+        "lsdg_depth_initialize_randomly": (i32, [vp, vp, C.c_uint64]),
         "lsdg_depth_set_prior": (i32, [vp, vp, fp, fp, i32]),
         "lsdg_depth_update": (i32, [vp, C.POINTER(vp), i32]),
         "lsdg_depth_create_keyframe": (i32, [vp, vp]),
@@ -294,6 +295,7 @@ def load():
         "lsdg_system_create": (i32, [vp, C.POINTER(Camera), i32, C.POINTER(SystemParams), C.POINTER(vp)]),
         "lsdg_system_destroy": (i32, [vp]),
         "lsdg_system_init_gt": (i32, [vp, i32, fp, i32, fp, fp]),
+        "lsdg_system_init_random": (i32, [vp, i32, fp, i32, C.c_uint64]),
         "lsdg_system_track": (i32, [vp, i32, fp, i32, C.POINTER(SystemFrame)]),
         "lsdg_system_keyframes": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_double), i32, C.POINTER(i32)]),
         "lsdg_system_depth_read": (i32, [vp, C.POINTER(Hypotheses)]),
@@ -494,6 +496,11 @@ class System:
         self.ctx.check(self.ctx.L.lsdg_system_init_gt(self.h, fid, img.ctypes.data, img.shape[1], pa, pb),
                        "lsdg_system_init_gt")
 
+    def init_random(self, fid, image, seed):
+        img = np.ascontiguousarray(image, np.uint8)
+        self.ctx.check(self.ctx.L.lsdg_system_init_random(self.h, fid, img.ctypes.data, img.shape[1], C.c_uint64(seed)),
+                       "lsdg_system_init_random")
+
     def track(self, fid, image):
         img = np.ascontiguousarray(image, np.uint8)
         out = SystemFrame()
@@ -675,6 +682,10 @@ class DepthMapH:
 
     def init_gt(self, kf):
         self.ctx.check(self.ctx.L.lsdg_depth_initialize_from_gt(self.h, kf.h), "lsdg_depth_initialize_from_gt")
+
+    def init_random(self, kf, seed):
+        self.ctx.check(self.ctx.L.lsdg_depth_initialize_randomly(self.h, kf.h, C.c_uint64(seed)),
+                       "lsdg_depth_initialize_randomly")
 
     def update(self, refs):
         arr = (C.c_void_p * len(refs))(*[r.h for r in refs])

@@ -34,6 +34,23 @@
 
 namespace orc {
 
+// SplitMix64 (Steele, Lea, Flood 2014): output number i + 1 from state
+// `seed`, top 31 bits: the draw DepthMap::initializeRandomly takes from
+// rand() upstream.
+inline int random_draw(uint64_t seed, int i) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (int)(z >> 33);
+}
+inline uint64_t splitmix64_output(uint64_t seed, int i) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 constexpr int PYR = LSDG_PYRAMID_LEVELS;
 
 void params_default(lsdg_params* p);
@@ -1508,6 +1525,25 @@ struct DepthMap {
         cur[i].blacklisted = 0;
       }
     }
+    setDepth(f);
+  }
+  // DepthMap::initializeRandomly, with random_draw (SplitMix64) in place of rand().
+  void initializeRandomly(Frame<S>* f, uint64_t seed) {
+    kf = f;
+    kfReactivated = false;
+    const S* maxg = f->maxg[0].data();
+    for (int y = 1; y < height - 1; ++y)
+      for (int x = 1; x < width - 1; ++x) {
+        const int i = x + y * width;
+        if (maxg[i] > (S)P.min_abs_grad_create) {
+          const float idf = 0.5f + 1.0f * ((float)(random_draw(seed, i) % 100001) / 100000.0f);
+          const S id = (S)idf;
+          cur[i] = Hyp<S>::make(id, id, (S)P.var_random_init_initial, (S)P.var_random_init_initial, 20);
+        } else {
+          cur[i].isValid = false;
+          cur[i].blacklisted = 0;
+        }
+      }
     setDepth(f);
   }
   void setPrior(Frame<S>* f, const float* id, const float* var, int validity) {

@@ -506,6 +506,12 @@ extern "C" void orc_depth_init_gt(void* d, void* kf) {
   if (D->dbl) D->d64.initializeFromGT(&((OFrame*)kf)->f64);
   else D->d32.initializeFromGT(&((OFrame*)kf)->f32);
 }
+extern "C" void orc_depth_init_random(void* d, void* kf, uint64_t seed) {
+  ODepth* D = (ODepth*)d;
+  if (D->dbl) D->d64.initializeRandomly(&((OFrame*)kf)->f64, seed);
+  else D->d32.initializeRandomly(&((OFrame*)kf)->f32, seed);
+}
+extern "C" uint64_t orc_splitmix64(uint64_t seed, int i) { return splitmix64_output(seed, i); }
 extern "C" void orc_depth_update(void* d, void** refs, int n) {
   ODepth* D = (ODepth*)d;
   if (D->dbl) {

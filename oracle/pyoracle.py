@@ -75,6 +75,8 @@ def load(fast=False):
         "orc_undistort": (None, [vp, i32, vp, i32, i32, vp]),
         "orc_fov_table": (None, [fp, f32, i32, i32, fp, i32, i32, vp]),
         "orc_pose_txt_line": (i32, [i32, dp, dp, C.c_char_p, i32]),
+        "orc_depth_init_random": (None, [vp, vp, C.c_uint64]),
+        "orc_splitmix64": (C.c_uint64, [C.c_uint64, i32]),
         "orc_constraint_test": (None, [vp, vp, i32, i32, dp, P, f32, C.POINTER(lsdg.ConstraintResult)]),
         "orc_sim3_log": (None, [dp, dp]),
         "orc_sim3_adj": (None, [dp, dp]),
@@ -335,6 +337,9 @@ class ODepth:
     def init_gt(self, kf):
         self.o.L.orc_depth_init_gt(self.h, kf.h)
 
+    def init_random(self, kf, seed):
+        self.o.L.orc_depth_init_random(self.h, kf.h, C.c_uint64(seed))
+
     def update(self, refs):
         arr = (C.c_void_p * len(refs))(*[r.h for r in refs])
         self.o.L.orc_depth_update(self.h, arr, len(refs))
@@ -426,6 +431,15 @@ class OSystem:
         f = self.o.frame(fid, self.cam, np.asarray(image, np.float32))
         f.set_depth(idepth, var)
         self.dm.init_gt(f)
+        c2w = np.r_[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
+        f.set_cam_to_world(c2w)
+        self.kf, self.kf_c2w = f, c2w
+        self.keyframes.append((f, c2w))
+        self.last_c2w = c2w
+
+    def init_random(self, fid, image, seed):
+        f = self.o.frame(fid, self.cam, np.asarray(image, np.float32))
+        self.dm.init_random(f, seed)
         c2w = np.r_[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
         f.set_cam_to_world(c2w)
         self.kf, self.kf_c2w = f, c2w

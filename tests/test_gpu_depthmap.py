@@ -201,3 +201,35 @@ def test_initialize_from_gt_and_finalize(ctx, depth_inst):
     assert not _diff(out[0][0], out[1][0])
     assert not _diff(out[0][1], out[1][1])
     assert np.array_equal(_bits(out[0][2]), _bits(out[1][2]))
+
+
+@pytest.mark.parametrize("size", [(640, 480), (200, 136)])
+def test_initialize_randomly_then_update_bit_exact(ctx, size):
+    """DepthMap::initializeRandomly (seeded draws) and an update on top of it,
+    every field of every pixel against the FP32 restatement."""
+    inst = synth.depth_instance(1023, cam=synth.camera(*size))
+    p = _params(100)
+
+    def run(sys_):
+        cam = inst["cam"]
+        f32 = (lambda a: a.astype(np.float32)) if isinstance(sys_, pyoracle.Oracle) else (lambda a: a)
+        kf = sys_.frame(0, cam, f32(inst["kf_img"]))
+        r = sys_.frame(1, cam, f32(inst["ref_img"]))
+        r.set_tracking(kf, list(inst["ref_pose"]) + [1.0], 0.4)
+        dm = sys_.depthmap(cam)
+        dm.init_random(kf, 99)
+        h0 = dm.read()
+        d0 = kf.read(lsdg.WHAT_IDEPTH, 0)
+        dm.update([r])
+        return h0, d0, dm.read()
+
+    ctx.set_params(p)
+    try:
+        g = run(ctx)
+    finally:
+        ctx.set_params(lsdg.default_params())
+    o = run(pyoracle.Oracle(0, params=p))
+    assert not _diff(g[0], o[0]), _diff(g[0], o[0])
+    assert np.array_equal(_bits(g[1]), _bits(o[1]))
+    assert not _diff(g[2], o[2]), _diff(g[2], o[2])
+    assert g[0]["is_valid"].mean() > 0.05

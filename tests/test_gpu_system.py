@@ -77,3 +77,20 @@ def test_system_state_errors(ctx, seq):
     with pytest.raises(lsdg.LsdgError):
         s.init_gt(0, seq["images"][0], seq["idepth"], seq["var"])
     s.destroy()
+
+
+def test_system_random_init_matches_oracle(ctx, seq):
+    """SlamSystem::randomInit (seeded) and the first frames after it."""
+    s = ctx.system(seq["cam"])
+    s.init_random(0, seq["images"][0], 5)
+    o = pyoracle.OSystem(pyoracle.Oracle(2), seq["cam"])
+    o.init_random(0, seq["images"][0], 5)
+    a, b = s.depth_read(), o.dm.read()
+    assert np.array_equal(a["is_valid"], b["is_valid"]) and np.array_equal(a["idepth"], b["idepth"])
+    for k in range(1, 8):
+        g = s.track(k, seq["images"][k])
+        e = o.track(k, seq["images"][k])
+        assert (g.status, g.keyframe_id, g.new_keyframe) == (e["status"], e["keyframe_id"], e["new_keyframe"]), k
+        if g.status == lsdg.SYSTEM_TRACKED:
+            assert np.max(np.abs(np.array(g.cam_to_world) - np.asarray(e["cam_to_world"]))) < 1e-6, k
+    s.destroy()

@@ -463,3 +463,40 @@ def test_propagation_along_the_optical_axis():
     assert v.sum() > 0.5 * n
     # after the rescale every propagated idepth is (1 / (z0 - tz)) * s = 1
     assert np.abs(hyp["idepth"][v] / s - 1.0 / (z0 - tz)).max() < 1e-6
+
+
+def test_random_init_draws_are_splitmix64():
+    # published SplitMix64 outputs (seed 0: first two; seed 1234567: first)
+    L = load()
+    assert L.orc_splitmix64(0, 0) == 0xE220A8397B1DCDAF
+    assert L.orc_splitmix64(0, 1) == 0x6E789E6AA1B965F4
+    assert L.orc_splitmix64(1234567, 0) == 6457827717110365317
+
+
+def test_random_init_rule():
+    """DepthMap::initializeRandomly: interior pixels above MIN_ABS_GRAD_CREATE
+    get idepth = smoothed in [0.5, 1.5], VAR_RANDOM_INIT_INITIAL and
+    validity 20; a seed reproduces the map and another seed changes it."""
+    cam, kfi, _ = _plane_stereo(0.05)
+    w, h = cam["w"], cam["h"]
+    o = Oracle(0)
+    p = pyoracle_params()
+    kf = o.frame(0, cam, kfi)
+    maps = []
+    for seed in (7, 7, 8):
+        dm = o.depthmap(cam)
+        dm.init_random(kf, seed)
+        maps.append(dm.read())
+    a = maps[0]
+    v = a["is_valid"].reshape(h, w).astype(bool)
+    mg = kf.read(lsdg.WHAT_MAXGRAD, 0)
+    expect = np.zeros((h, w), bool)
+    expect[1:-1, 1:-1] = mg[1:-1, 1:-1] > p.min_abs_grad_create
+    assert np.array_equal(v, expect) and v.any()
+    idp = a["idepth"].reshape(h, w)[v]
+    assert idp.min() >= 0.5 and idp.max() <= 1.5 and abs(idp.mean() - 1.0) < 0.02
+    assert np.array_equal(a["idepth_smoothed"].reshape(h, w)[v], idp)
+    assert np.all(a["idepth_var"].reshape(h, w)[v] == np.float32(p.var_random_init_initial))
+    assert np.all(a["validity_counter"].reshape(h, w)[v] == 20)
+    assert np.array_equal(maps[1]["idepth"], a["idepth"])
+    assert not np.array_equal(maps[2]["idepth"], a["idepth"])
