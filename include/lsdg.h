@@ -596,6 +596,11 @@ lsdg_status lsdg_frame_ingest(lsdg_frame f, int32_t id, lsdg_undistorter u, cons
  * significant digits), ending in a newline; NUL-terminated into buf.
  * LSDG_EINVAL when it does not fit in cap bytes. */
 lsdg_status lsdg_frame_pose_txt(lsdg_frame f, char* buf, size_t cap);
+/* The same line from an id and two poses (qx, qy, qz, qw, tx, ty, tz, s):
+ * for frames the caller does not hold as lsdg_frame, such as the system's
+ * (lsdg_system_frame: cam_to_world, and frame_to_keyframe with s = 1). */
+lsdg_status lsdg_pose_txt_line(int32_t id, const double cam_to_world[8], const double this_to_parent[8], char* buf,
+                               size_t cap);
 
 #ifdef __cplusplus
 }

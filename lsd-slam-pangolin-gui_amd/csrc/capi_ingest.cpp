@@ -173,16 +173,24 @@ extern "C" lsdg_status lsdg_frame_ingest(lsdg_frame f, int32_t id, lsdg_undistor
   return ingest_into(f, u, image, stride, undistorted);
 }
 
-extern "C" lsdg_status lsdg_frame_pose_txt(lsdg_frame f, char* buf, size_t cap) {
-  if (!f || !buf || cap == 0) return LSDG_EINVAL;
-  const Pose& a = f->camToWorld;
-  const Pose& b = f->thisToParent;
+extern "C" lsdg_status lsdg_pose_txt_line(int32_t id, const double cam_to_world[8], const double this_to_parent[8],
+                                          char* buf, size_t cap) {
+  if (!cam_to_world || !this_to_parent || !buf || cap == 0) return LSDG_EINVAL;
+  const double* a = cam_to_world + 4;
+  const double* b = this_to_parent + 4;
   // "%g" is the ostream default for a double (precision 6, no floatfield)
-  const int n = std::snprintf(buf, cap, "%d,%g,%g,%g,%g,%g,%g\n", (int)f->id, a.t[0], a.t[1], a.t[2], b.t[0], b.t[1],
-                              b.t[2]);
+  const int n = std::snprintf(buf, cap, "%d,%g,%g,%g,%g,%g,%g\n", (int)id, a[0], a[1], a[2], b[0], b[1], b[2]);
   if (n < 0 || (size_t)n >= cap) {
     buf[0] = '\0';
     return LSDG_EINVAL;
   }
   return LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_frame_pose_txt(lsdg_frame f, char* buf, size_t cap) {
+  if (!f) return LSDG_EINVAL;
+  double a[8], b[8];
+  pose_to8(f->camToWorld, a);
+  pose_to8(f->thisToParent, b);
+  return lsdg_pose_txt_line(f->id, a, b, buf, cap);
 }

@@ -290,6 +290,7 @@ def load():
         "lsdg_fov_undistort_table": (i32, [fp, f32, i32, i32, fp, i32, i32, fp]),
         "lsdg_frame_create_undistorted": (i32, [vp, i32, C.POINTER(Camera), vp, fp, i32, i32, fp, C.POINTER(vp)]),
         "lsdg_frame_pose_txt": (i32, [vp, C.c_char_p, C.c_size_t]),
+        "lsdg_pose_txt_line": (i32, [i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
         "lsdg_frame_ingest": (i32, [vp, i32, vp, fp, i32, fp]),
         "lsdg_system_params_default": (None, [C.POINTER(SystemParams)]),
         "lsdg_system_create": (i32, [vp, C.POINTER(Camera), i32, C.POINTER(SystemParams), C.POINTER(vp)]),
@@ -539,6 +540,17 @@ def fov_undistort_table(in_k, omega, in_size, out_k, out_size):
     if st < 0:
         raise LsdgError(f"lsdg_fov_undistort_table failed with status {st}")
     return out
+
+
+def pose_txt_line(fid, cam_to_world8, this_to_parent8):
+    """TextOutputIOWrapper::publishTrackedFrame's pose.txt line for an id and two poses."""
+    a = (C.c_double * 8)(*[float(v) for v in cam_to_world8])
+    b = (C.c_double * 8)(*[float(v) for v in this_to_parent8])
+    buf = C.create_string_buffer(512)
+    st = load().lsdg_pose_txt_line(int(fid), a, b, buf, 512)
+    if st < 0:
+        raise LsdgError(f"lsdg_pose_txt_line failed with status {st}")
+    return buf.value.decode()
 
 
 class Undistorter:

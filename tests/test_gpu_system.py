@@ -94,3 +94,25 @@ def test_system_random_init_matches_oracle(ctx, seq):
         if g.status == lsdg.SYSTEM_TRACKED:
             assert np.max(np.abs(np.array(g.cam_to_world) - np.asarray(e["cam_to_world"]))) < 1e-6, k
     s.destroy()
+
+
+def test_pose_txt_trajectory_regression(ctx, seq, tmp_path):
+    """The system's trajectory through TextOutputIOWrapper's pose.txt format
+    (one line per tracked frame), read back and checked against the
+    sequence's ground truth."""
+    s = ctx.system(seq["cam"], lsdg.FP16)
+    s.init_gt(0, seq["images"][0], seq["idepth"], seq["var"])
+    path = tmp_path / "pose.txt"
+    with open(path, "w") as fh:
+        for k in range(1, N):
+            g = s.track(k, seq["images"][k])
+            assert g.status == lsdg.SYSTEM_TRACKED
+            fh.write(lsdg.pose_txt_line(g.id, list(g.cam_to_world), list(g.frame_to_keyframe) + [1.0]))
+    s.destroy()
+    rows = [line.split(",") for line in open(path).read().splitlines()]
+    assert len(rows) == N - 1 and all(len(r) == 7 for r in rows)
+    ids = [int(r[0]) for r in rows]
+    assert ids == list(range(1, N))
+    t = np.array([[float(v) for v in r[1:4]] for r in rows])
+    err = np.linalg.norm(t - seq["poses"][1:, 4:7], axis=1)
+    assert err.max() < 6e-3, err.max()

@@ -134,3 +134,12 @@ def test_pose_line_formats_like_an_ostream(o, vals):
     exp = "42," + ",".join("%g" % v for v in (*a, *b)) + "\n"
     assert line == exp
     assert line.count(",") == 6 and line.endswith("\n")
+
+
+def test_product_pose_line_matches_the_oracle(o):
+    rng = np.random.default_rng(8)
+    for k in range(20):
+        a = rng.normal(size=3) * 10.0 ** rng.integers(-8, 8, 3)
+        b = rng.normal(size=3) * 10.0 ** rng.integers(-8, 8, 3)
+        line = lsdg.pose_txt_line(k, [0, 0, 0, 1, *a, 1.0], [0, 0, 0, 1, *b, 1.0])
+        assert line == o.pose_txt_line(k, a, b)
