@@ -96,6 +96,76 @@ __device__ inline void pyr_images_tile(const PyrImgArgs& p, const uint8_t* __res
   pyr_levels_2to4(p, L1, bx, by);
 }
 
+// Levels 1-4 of a 64x64 level-0 tile (the batched rebuild: a quarter of the
+// blocks of the 32x32 tiling, which leaves most threads idle at the coarse
+// levels), read from the 8-bit level 0 when u8 is set, else from the float
+// copy.  Same operations per pixel as pyr_images_tile.
+__device__ inline void pyr_images_tile64(const PyrImgArgs& p, const uint8_t* __restrict__ u8, int bx, int by) {
+  __shared__ float L1[32][33];
+  __shared__ float L2[16][17];
+  __shared__ float L3[8][9];
+  const int t = threadIdx.x;
+  const int w0 = p.w[0];
+  // level 1: 32x32, four pixels per thread, every load issued first
+  float q[4][4];
+  bool ok[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = t + 256 * i, lx = k & 31, ly = k >> 5;
+    const int x = bx * 32 + lx, y = by * 32 + ly;
+    ok[i] = x < p.w[1] && y < p.h[1];
+    const int s = ok[i] ? 2 * x + 2 * y * w0 : 0;
+    if (u8) {
+      q[i][0] = (float)u8[s]; q[i][1] = (float)u8[s + 1]; q[i][2] = (float)u8[s + w0]; q[i][3] = (float)u8[s + w0 + 1];
+    } else {
+      const float* I = p.img[0];
+      q[i][0] = I[s]; q[i][1] = I[s + 1]; q[i][2] = I[s + w0]; q[i][3] = I[s + w0 + 1];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = t + 256 * i, lx = k & 31, ly = k >> 5;
+    float v = 0.f;
+    if (ok[i]) {
+      v = (q[i][0] + q[i][1] + q[i][2] + q[i][3]) * 0.25f;
+      p.img[1][bx * 32 + lx + (by * 32 + ly) * p.w[1]] = v;
+    }
+    L1[ly][lx] = v;
+  }
+  __syncthreads();
+  {
+    const int lx = t & 15, ly = t >> 4;
+    const int x = bx * 16 + lx, y = by * 16 + ly;
+    float v = 0.f;
+    if (x < p.w[2] && y < p.h[2]) {
+      v = (L1[2 * ly][2 * lx] + L1[2 * ly][2 * lx + 1] + L1[2 * ly + 1][2 * lx] + L1[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[2][x + y * p.w[2]] = v;
+    }
+    L2[ly][lx] = v;
+  }
+  __syncthreads();
+  if (t < 64) {
+    const int lx = t & 7, ly = t >> 3;
+    const int x = bx * 8 + lx, y = by * 8 + ly;
+    float v = 0.f;
+    if (x < p.w[3] && y < p.h[3]) {
+      v = (L2[2 * ly][2 * lx] + L2[2 * ly][2 * lx + 1] + L2[2 * ly + 1][2 * lx] + L2[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[3][x + y * p.w[3]] = v;
+    }
+    L3[ly][lx] = v;
+  }
+  __syncthreads();
+  if (t < 16) {
+    const int lx = t & 3, ly = t >> 2;
+    const int x = bx * 4 + lx, y = by * 4 + ly;
+    if (x < p.w[4] && y < p.h[4]) {
+      const float v =
+          (L3[2 * ly][2 * lx] + L3[2 * ly][2 * lx + 1] + L3[2 * ly + 1][2 * lx] + L3[2 * ly + 1][2 * lx + 1]) * 0.25f;
+      p.img[4][x + y * p.w[4]] = v;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_pyr_images(PyrImgArgs p, int writeL0) {
   pyr_images_tile(p, p.u8, writeL0 != 0, blockIdx.x, blockIdx.y);
 }
@@ -249,7 +319,7 @@ __global__ __launch_bounds__(256) void k_pyr_images_batch(const PyrImgArgs* __re
                                                           int nf) {
   int f = frame_of_block(off, nf, blockIdx.x);
   int local = blockIdx.x - off[f];
-  pyr_images_tile(args[f], args[f].u8, false, local % tx[f], local / tx[f]);
+  pyr_images_tile64(args[f], args[f].u8, local % tx[f], local / tx[f]);
 }
 
 __global__ __launch_bounds__(256) void k_pyr_grads_batch(const PyrGradArgs* __restrict__ args,
@@ -522,7 +592,7 @@ void pyr_batch_tables(lsdg_frame const* fs, const unsigned* need, int nf, void* 
     int blk = pyr_grad_args(f, need[k], q);
     ia[k] = p;
     ga[k] = q;
-    int tx = (f->w[0] + 31) / 32, ty = (f->h[0] + 31) / 32;
+    int tx = (f->w[0] + 63) / 64, ty = (f->h[0] + 63) / 64;  // 64x64 tiles (pyr_images_tile64)
     ioff[k] = ib;
     itx[k] = tx;
     ib += tx * ty;
