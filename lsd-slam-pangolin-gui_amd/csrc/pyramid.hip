@@ -234,6 +234,7 @@ struct PyrGradArgs {
   int w[PYR], h[PYR];
   int blk0[PYR + 1];
   int prec;
+  int ppt;  // pixels per thread: a block covers ppt x 256 consecutive pixels of one level
 };
 
 __device__ inline float absgrad_at(const float* __restrict__ I, int j, int w, int h) {
@@ -272,32 +273,40 @@ __device__ inline uint2 pack_grad(float4 g, int prec) {
   return make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c);
 }
 
+// Pixels per thread of the batched gradient stage (a single frame's keeps
+// one per thread, for more blocks on its smaller grid).
+constexpr int GRAD_PPT_BATCH = 4;
+
 __device__ inline void pyr_grads_block(const PyrGradArgs& p, int b) {
   int l = 0;
   while (l < PYR - 1 && b >= p.blk0[l + 1]) l++;
-  int idx = (b - p.blk0[l]) * 256 + threadIdx.x;
-  int w = p.w[l], h = p.h[l];
-  if (idx >= w * h) return;
+  const int w = p.w[l], h = p.h[l];
   const float* I = p.img[l];
-  float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (idx >= w && idx < w * (h - 1)) {
-    g.x = 0.5f * (I[idx + 1] - I[idx - 1]);
-    g.y = 0.5f * (I[idx + w] - I[idx - w]);
-    g.z = I[idx];
+  // pixel i of thread t at t + 256 i
+  const int base = (b - p.blk0[l]) * 256 * p.ppt + threadIdx.x;
+  for (int i = 0; i < p.ppt; ++i) {
+    const int idx = base + 256 * i;
+    if (idx >= w * h) break;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (idx >= w && idx < w * (h - 1)) {
+      g.x = 0.5f * (I[idx + 1] - I[idx - 1]);
+      g.y = 0.5f * (I[idx + w] - I[idx - w]);
+      g.z = I[idx];
+    }
+    if (p.grad[l]) p.grad[l][idx] = g;
+    if (p.gradh[l]) p.gradh[l][idx] = pack_grad(g, p.prec);
+    if (!p.maxg[l]) continue;
+    float m;
+    if (idx >= w + 1 && idx < w * (h - 1) - 1) {
+      float g1 = vmax_at(I, idx - 1, w, h), g2 = vmax_at(I, idx, w, h);
+      if (g1 < g2) g1 = g2;
+      float g3 = vmax_at(I, idx + 1, w, h);
+      m = (g1 < g3) ? g3 : g1;
+    } else {
+      m = absgrad_at(I, idx, w, h);
+    }
+    p.maxg[l][idx] = m;
   }
-  if (p.grad[l]) p.grad[l][idx] = g;
-  if (p.gradh[l]) p.gradh[l][idx] = pack_grad(g, p.prec);
-  if (!p.maxg[l]) return;
-  float m;
-  if (idx >= w + 1 && idx < w * (h - 1) - 1) {
-    float g1 = vmax_at(I, idx - 1, w, h), g2 = vmax_at(I, idx, w, h);
-    if (g1 < g2) g1 = g2;
-    float g3 = vmax_at(I, idx + 1, w, h);
-    m = (g1 < g3) ? g3 : g1;
-  } else {
-    m = absgrad_at(I, idx, w, h);
-  }
-  p.maxg[l][idx] = m;
 }
 
 __global__ __launch_bounds__(256) void k_pyr_grads(PyrGradArgs p) { pyr_grads_block(p, blockIdx.x); }
@@ -537,9 +546,11 @@ hipError_t launch_ingest(lsdg_frame f, const uint8_t* d_src, int inW, const floa
   return hipGetLastError();
 }
 
-// Fills the gradient-stage arguments of frame f for the data in `need`;
+// Fills the gradient-stage arguments of frame f for the data in `need`, ppt
+// pixels per thread;
 // returns the number of blocks (levels with nothing to build get none).
-static int pyr_grad_args(lsdg_frame f, unsigned need, PyrGradArgs& p) {
+static int pyr_grad_args(lsdg_frame f, unsigned need, int ppt, PyrGradArgs& p) {
+  p.ppt = ppt;
   int blk = 0;
   for (int l = 0; l < PYR; ++l) {
     p.img[l] = f->d_image[l];
@@ -549,7 +560,7 @@ static int pyr_grad_args(lsdg_frame f, unsigned need, PyrGradArgs& p) {
     p.w[l] = f->w[l];
     p.h[l] = f->h[l];
     p.blk0[l] = blk;
-    if (p.grad[l] || p.gradh[l] || p.maxg[l]) blk += (f->w[l] * f->h[l] + 255) / 256;
+    if (p.grad[l] || p.gradh[l] || p.maxg[l]) blk += (f->w[l] * f->h[l] + 256 * ppt - 1) / (256 * ppt);
   }
   p.blk0[PYR] = blk;
   p.prec = f->precision;
@@ -558,7 +569,7 @@ static int pyr_grad_args(lsdg_frame f, unsigned need, PyrGradArgs& p) {
 
 hipError_t launch_pyr_grads(lsdg_frame f, unsigned need, hipStream_t s) {
   PyrGradArgs p;
-  int blk = pyr_grad_args(f, need, p);
+  int blk = pyr_grad_args(f, need, 1, p);
   if (blk == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pyr_grads, dim3(blk), dim3(256), 0, s, p);
   return hipGetLastError();
@@ -589,7 +600,7 @@ void pyr_batch_tables(lsdg_frame const* fs, const unsigned* need, int nf, void* 
       p.h[l] = f->h[l];
     }
     p.u8 = f->u8Level0 ? f->d_input_u8 : nullptr;
-    int blk = pyr_grad_args(f, need[k], q);
+    int blk = pyr_grad_args(f, need[k], GRAD_PPT_BATCH, q);
     ia[k] = p;
     ga[k] = q;
     int tx = (f->w[0] + 63) / 64, ty = (f->h[0] + 63) / 64;  // 64x64 tiles (pyr_images_tile64)
