@@ -547,8 +547,8 @@ hipError_t launch_ingest(lsdg_frame f, const uint8_t* d_src, int inW, const floa
 }
 
 // Fills the gradient-stage arguments of frame f for the data in `need`, ppt
-// pixels per thread;
-// returns the number of blocks (levels with nothing to build get none).
+// pixels per thread; returns the number of blocks (levels with nothing to
+// build get none).
 static int pyr_grad_args(lsdg_frame f, unsigned need, int ppt, PyrGradArgs& p) {
   p.ppt = ppt;
   int blk = 0;
