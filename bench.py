@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline's dominant kernel, from the FETCH_SIZE /
 # WRITE_SIZE counter passes scaled by a 256 MiB copy kernel (profiles/);
 # re-measured when the kernel changes.
-TRAFFIC_BYTES = {"k_batch_coop": 273062494, "k_batch_eval": 10134461}
+TRAFFIC_BYTES = {"k_batch_coop": 273371547, "k_batch_eval": 10134461}
 
 
 def tracking_params(its=20, crop=None):
