@@ -663,7 +663,16 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
                              (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (tid == 0) {
-          __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          // The row stores above are agent-scope atomic stores (sc1), coherent
+          // at agent scope once complete, so waiting for their completion
+          // orders them before the add.  An agent-scope release would also
+          // write back the XCD's L2 for ordinary stores, and this wave
+          // publishes none.  The signal fences keep the compiler from moving
+          // memory operations across the wait.
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __builtin_amdgcn_s_waitcnt(0);
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           const unsigned target = (unsigned)K * (gen + 1);
           unsigned spins = 0;
           while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
