@@ -663,15 +663,7 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
                              (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (tid == 0) {
-          // The row stores above are agent-scope atomic stores (sc1), coherent
-          // at agent scope once complete, so waiting for their completion
-          // orders them before the add.  An agent-scope release would also
-          // write back the XCD's L2 for ordinary stores, and this wave
-          // publishes none.  The signal fences keep the compiler from moving
-          // memory operations across the wait.
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
-          __builtin_amdgcn_s_waitcnt(0);
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          wait_own_stores();  // the row stores above are agent-scope atomics
           __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           const unsigned target = (unsigned)K * (gen + 1);
           unsigned spins = 0;

@@ -38,9 +38,10 @@ __device__ inline void reduce_and_merge(Accum<MODE, NS>& acc, double* __restrict
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < EVAL_WAVES; ++w) s += sh[w][threadIdx.x];
-    partials[(size_t)blockIdx.x * NS + threadIdx.x] = s;
+    __hip_atomic_store((unsigned long long*)&partials[(size_t)blockIdx.x * NS + threadIdx.x],
+                       (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __threadfence();
+  wait_own_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned prev = atomicAdd(counter, 1u);
@@ -48,7 +49,7 @@ __device__ inline void reduce_and_merge(Accum<MODE, NS>& acc, double* __restrict
   }
   __syncthreads();
   if (!amLast) return;
-  __threadfence();
+  // the rows are read with agent-scope atomic loads: no L2 invalidation needed
   if (threadIdx.x < NS) {
     double s = 0.0;
     for (unsigned b = 0; b < gridDim.x; ++b) {

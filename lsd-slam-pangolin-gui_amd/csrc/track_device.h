@@ -12,6 +12,17 @@
 
 namespace lsdg {
 
+// Waits until this wave's earlier memory operations have completed.  Stores
+// made as agent-scope atomics (sc1) are coherent at agent scope once complete,
+// so a wait orders them before a later counter update with no write-back of
+// the XCD's L2 (which an agent-scope release adds for ordinary stores).  The
+// signal fences keep the compiler from moving memory operations across it.
+__device__ inline void wait_own_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 constexpr int EVAL_BS = 256;
 constexpr int EVAL_WAVES = EVAL_BS / 64;
 constexpr int EVAL_U = 2;  // points per thread with loads in flight together (plain kernels)
