@@ -106,7 +106,10 @@ __device__ inline void pyr_images_tile64(const PyrImgArgs& p, const uint8_t* __r
   __shared__ float L3[8][9];
   const int t = threadIdx.x;
   const int w0 = p.w[0];
-  // level 1: 32x32, four pixels per thread, every load issued first
+  // level 1: 32x32, four pixels per thread, every load issued first.  An
+  // 8-bit level 0 of even width is read as byte pairs: s is even, so both
+  // rows' pairs are 2-byte aligned (the buffer itself is 256-byte aligned).
+  const bool pairs = u8 && (w0 & 1) == 0;
   float q[4][4];
   bool ok[4];
 #pragma unroll
@@ -115,7 +118,10 @@ __device__ inline void pyr_images_tile64(const PyrImgArgs& p, const uint8_t* __r
     const int x = bx * 32 + lx, y = by * 32 + ly;
     ok[i] = x < p.w[1] && y < p.h[1];
     const int s = ok[i] ? 2 * x + 2 * y * w0 : 0;
-    if (u8) {
+    if (pairs) {
+      const unsigned a = *(const unsigned short*)(u8 + s), b = *(const unsigned short*)(u8 + s + w0);
+      q[i][0] = (float)(a & 0xffu); q[i][1] = (float)(a >> 8); q[i][2] = (float)(b & 0xffu); q[i][3] = (float)(b >> 8);
+    } else if (u8) {
       q[i][0] = (float)u8[s]; q[i][1] = (float)u8[s + 1]; q[i][2] = (float)u8[s + w0]; q[i][3] = (float)u8[s + w0 + 1];
     } else {
       const float* I = p.img[0];
