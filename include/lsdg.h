@@ -302,6 +302,24 @@ lsdg_status lsdg_energy_grad(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, int32
                              const double ref_to_frame[7], double* energy, double grad_xi[6],
                              float* grad_idepth, int32_t max_points);
 
+/* Sim3 adjoint at referenceToFrame (double[8]): the gradient of the Sim3
+ * tracker's reweighted least-squares energy with its weights frozen at that
+ * pose,
+ *   E = 1/2 sum_i wh_i (w_p,i r_p,i^2 + w_d,i r_d,i^2),
+ * r_p the photometric and r_d = 1/z' - idepth_frame the depth residual of
+ * Sim3Tracker::calcSim3Buffers; wh the joint Huber weight at huber_d and
+ * w_p / w_d the variance weights of calcSim3WeightsAndResidual; the weights,
+ * the nearest-pixel frame depth sample and the in-image set are held fixed.
+ *   grad_xi[7]     = dE/dxi (v, w, sigma) for T <- exp(xi) T,
+ *   grad_idepth[i] = dE/d(idepth of point i) (0 outside the image).
+ * grad_xi equals calcSim3LGS's b (LGS7 of LGS6 + LGS4, not divided by n)
+ * with the frame's own gradient in place of the ESM average.  The frame must
+ * carry depth.  No counterpart in the reference; it is the reduction behind
+ * Sim3Tracker's normal equations. */
+lsdg_status lsdg_energy_grad_sim3(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, int32_t level,
+                                  const double ref_to_frame[8], double* energy, double grad_xi[7],
+                                  float* grad_idepth, int32_t max_points);
+
 /* ------------------------------------------------------------- depth map */
 
 lsdg_status lsdg_depthmap_create(lsdg_ctx ctx, const lsdg_camera* cam, lsdg_depthmap* out);
@@ -351,7 +369,8 @@ lsdg_status lsdg_depth_active_keyframe(lsdg_depthmap dm, int32_t* id);
 #define LSDG_KERNEL_BATCH_JOB 5   /* batched whole-job LM, one block per job */
 #define LSDG_KERNEL_BATCH_COOP 6  /* batched whole LM, several cooperating blocks per job */
 #define LSDG_KERNEL_INGEST 7      /* undistortion + conversion + image pyramid of one camera image */
-#define LSDG_KERNEL_KINDS 8
+#define LSDG_KERNEL_ADJOINT 8     /* SE3 / Sim3 adjoint (lsdg_energy_grad, lsdg_energy_grad_sim3) */
+#define LSDG_KERNEL_KINDS 9
 
 typedef struct lsdg_kernel_stats {
   int64_t launches;
@@ -387,7 +406,11 @@ typedef struct lsdg_job_result {
 
 /* Plans a batch: many alignments of uneven size run in one persistent launch
  * per LM step, the steps captured in a hipGraph.  The plan may be run many
- * times (frames may change content, not shape). */
+ * times.  Frames may change content between runs, not shape: each run
+ * rebuilds the pyramid levels a re-ingested frame's jobs track on (and resets
+ * its refPixelWasGood mask); a keyframe whose depth changed gets new point
+ * clouds; a Sim3 job's frame must carry depth when the plan runs
+ * (LSDG_ESTATE otherwise). */
 lsdg_status lsdg_batch_create(lsdg_ctx ctx, const lsdg_job* jobs, int32_t n, lsdg_batch* out);
 lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results);
 lsdg_status lsdg_batch_destroy(lsdg_batch b);

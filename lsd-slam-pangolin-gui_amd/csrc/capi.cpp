@@ -138,6 +138,7 @@ extern "C" lsdg_status lsdg_ctx_destroy(lsdg_ctx c) {
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->h_result) (void)hipHostFree(c->h_result);
   if (c->d_export) (void)hipFree(c->d_export);
+  if (c->d_adj) (void)hipFree(c->d_adj);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return LSDG_OK;
@@ -186,6 +187,11 @@ lsdg_status lsdg::frame_alloc(lsdg_ctx c, int32_t id, const lsdg_camera* cam, in
   if (cam->w < 32 || cam->h < 32 || cam->w > 16384 || cam->h > 16384 || (cam->w >> (PYR - 1)) < 4 ||
       (cam->h >> (PYR - 1)) < 4)
     return fail(c, LSDG_EINVAL, "frame size out of range (32..16384, >= 4 px at the coarsest level)");
+  // The level-1 refPixelWasGood mask is indexed with (x >> 1, y >> 1) of
+  // level-0 pixels (DepthMap::propagateDepth's tracking gate): an odd width
+  // or height would put the last column or row one past the mask.  Upstream
+  // requires multiples of 16 (SlamSystem's constructor); even suffices here.
+  if ((cam->w & 1) || (cam->h & 1)) return fail(c, LSDG_EINVAL, "frame width and height must be even");
   if (precision < LSDG_FP32 || precision > LSDG_BF16) return fail(c, LSDG_EINVAL, "unknown precision");
   lsdg_frame f = new (std::nothrow) lsdg_frame_s();
   if (!f) return LSDG_ENOMEM;
@@ -989,30 +995,84 @@ extern "C" lsdg_status lsdg_sim3_evaluate(lsdg_ctx c, lsdg_ref r, lsdg_frame f, 
 }
 
 // --------------------------------------------------------------- adjoint
-extern "C" lsdg_status lsdg_energy_grad(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int32_t level, const double pose[7],
-                                        double* energy, double grad_xi[6], float* grad_idepth, int32_t max_points) {
-  if (!c || !r || !f || !pose) return LSDG_EINVAL;
+// Device buffer of per-point inverse-depth gradients, grown on demand.
+static lsdg_status adjoint_buffer(lsdg_ctx c, int n, float** out) {
+  if (n > c->adjCap) {
+    if (c->d_adj) (void)hipFree(c->d_adj);
+    c->d_adj = nullptr;
+    c->adjCap = 0;
+    CK(c, hipMalloc(&c->d_adj, sizeof(float) * (size_t)n));
+    c->adjCap = n;
+  }
+  *out = c->d_adj;
+  return LSDG_OK;
+}
+
+// Shared body of the SE3 and Sim3 adjoints: point cloud, pyramid level,
+// launch, result row in host-mapped memory, per-point gradient copied out.
+static lsdg_status energy_grad_run(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int level, const Pose& T, bool sim3,
+                                   double* R, float* grad_idepth, int max_points) {
+  if (level < 0 || level >= PYR) return fail(c, LSDG_EINVAL, "level out of range");
+  if (r->ctx != c || f->ctx != c) return fail(c, LSDG_EINVAL, "objects belong to another context");
   if (f->w[0] != r->kf->w[0] || f->h[0] != r->kf->h[0]) return fail(c, LSDG_EINVAL, "frame / keyframe size mismatch");
+  if (sim3 && !f->hasDepth) return fail(c, LSDG_ESTATE, "the Sim3 adjoint needs a frame with depth");
   lsdg_status st = ref_points(r, level);
   if (st != LSDG_OK) return st;
   const PointsLevel& pl = r->lv[level];
   if (grad_idepth && max_points < pl.n) return fail(c, LSDG_EINVAL, "grad_idepth buffer too small");
+  const int ns = sim3 ? NS_ADJ_SIM3 : NS_ADJ;
+  std::memset(R, 0, sizeof(double) * ns);
+  if (pl.n == 0) return LSDG_OK;
   st = frame_require(f, fd_track(f, level, level));
   if (st != LSDG_OK) return st;
   float* d_g = nullptr;
-  CK(c, hipMalloc(&d_g, sizeof(float) * (pl.n > 0 ? pl.n : 1)));
-  TrackKParams kp = track_kparams(c->P, false);
-  hipError_t e = hipSuccess;
-  if (pl.n > 0) e = launch_energy_grad(c, pl, f, level, pose_xf(pose_from7(pose), false), kp, d_g, c->d_result);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e == hipSuccess && grad_idepth && pl.n > 0)
-    e = hipMemcpy(grad_idepth, d_g, sizeof(float) * pl.n, hipMemcpyDeviceToHost);
-  (void)hipFree(d_g);
+  st = adjoint_buffer(c, pl.n, &d_g);
+  if (st != LSDG_OK) return st;
+  TrackKParams kp = track_kparams(c->P, sim3);
+  kp.use_esm = 0;  // the exact derivative samples the frame's own gradient
+  KTimer kt = prof_timer(c);
+  hipError_t e = sim3 ? launch_energy_grad_sim3(c, pl, f, level, pose_xf(T, true), kp, d_g, c->d_result, &kt)
+                      : launch_energy_grad(c, pl, f, level, pose_xf(T, false), kp, d_g, c->d_result, &kt);
+  if (e != hipSuccess) return fail(c, LSDG_EDEVICE, "energy_grad launch", e);
+  prof_timer_done(c, LSDG_KERNEL_ADJOINT, kt, adjoint_bytes(pl.n, f, level, sim3));
+  if (grad_idepth) {
+    e = hipMemcpyAsync(grad_idepth, d_g, sizeof(float) * pl.n, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return fail(c, LSDG_EDEVICE, "energy_grad download", e);
+  }
+  e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, LSDG_EDEVICE, "energy_grad", e);
-  double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const double* R = pl.n > 0 ? c->h_result : zero;
+  std::memcpy(R, c->h_result, sizeof(double) * ns);
+  return LSDG_OK;
+}
+
+// Algorithmic bytes of one adjoint evaluation: the tracker evaluation's
+// (every point record and the level's image; Sim3 adds the depth terms) plus
+// the 4-byte dE/dd written per point.
+double lsdg::adjoint_bytes(int n, lsdg_frame f, int lvl, bool sim3) {
+  return eval_bytes(n, f, lvl, sim3) + 4.0 * n;
+}
+
+extern "C" lsdg_status lsdg_energy_grad(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int32_t level, const double pose[7],
+                                        double* energy, double grad_xi[6], float* grad_idepth, int32_t max_points) {
+  if (!c || !r || !f || !pose) return LSDG_EINVAL;
+  double R[NS_MAX];
+  lsdg_status st = energy_grad_run(c, r, f, level, pose_from7(pose), false, R, grad_idepth, max_points);
+  if (st != LSDG_OK) return st;
   if (energy) *energy = R[6];
   if (grad_xi)
     for (int a = 0; a < 6; ++a) grad_xi[a] = R[a];
+  return LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_energy_grad_sim3(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int32_t level,
+                                             const double pose[8], double* energy, double grad_xi[7],
+                                             float* grad_idepth, int32_t max_points) {
+  if (!c || !r || !f || !pose) return LSDG_EINVAL;
+  double R[NS_MAX];
+  lsdg_status st = energy_grad_run(c, r, f, level, pose_from8(pose), true, R, grad_idepth, max_points);
+  if (st != LSDG_OK) return st;
+  if (energy) *energy = 0.5 * (R[AS_EP] + R[AS_ED]);
+  if (grad_xi)
+    for (int a = 0; a < 7; ++a) grad_xi[a] = R[AS_G + a];
   return LSDG_OK;
 }

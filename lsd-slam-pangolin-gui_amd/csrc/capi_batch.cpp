@@ -59,6 +59,12 @@ struct lsdg_batch_s {
   int* h_remaining = nullptr;  // pinned
   double* d_partials = nullptr;
   size_t rows = 0;
+  // every distinct frame of the plan and the pyramid data its jobs track on
+  // (fd_bit flags), re-established at each run: a frame may have been given
+  // new content (lsdg_frame_ingest) since the plan was built
+  std::vector<lsdg_frame> frames;
+  std::vector<unsigned> needs;
+  std::vector<char> rebuilt;  // frames[k] is rebuilt by the graph itself
   // pyramid rebuild
   std::vector<lsdg_frame> pyrFrames;
   std::vector<unsigned> pyrNeed;  // data the rebuild produces per frame (fd_bit flags)
@@ -327,8 +333,8 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   }
   // pyramid data each frame is tracked on (union over its jobs), built now so
   // the frame holds it; frames the batch rebuilds get it again on every run
-  std::vector<lsdg_frame> frames;
-  std::vector<unsigned> needs;
+  std::vector<lsdg_frame>& frames = b->frames;
+  std::vector<unsigned>& needs = b->needs;
   for (auto& J : b->jobs) {
     unsigned m = fd_levels(FD_IMAGE, 0, PYR - 1) | fd_track(J.frame, J.final_level, J.start_level);
     auto it = std::find(frames.begin(), frames.end(), J.frame);
@@ -347,6 +353,8 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
       return st;
     }
   for (lsdg_frame f : b->pyrFrames) b->pyrNeed.push_back(needs[std::find(frames.begin(), frames.end(), f) - frames.begin()]);
+  for (lsdg_frame f : frames)
+    b->rebuilt.push_back(std::find(b->pyrFrames.begin(), b->pyrFrames.end(), f) != b->pyrFrames.end());
   if (!b->pyrFrames.empty()) {
     size_t bytes = pyr_batch_bytes((int)b->pyrFrames.size());
     if ((e = hipMalloc(&b->d_pyr, bytes)) != hipSuccess || (e = hipHostMalloc(&b->h_pyr, bytes)) != hipSuccess) {
@@ -498,9 +506,37 @@ static int remaining_jobs(lsdg_batch b) {
   return r;
 }
 
+// Brings every frame of the plan back to what the plan reads: a frame given
+// new content since the last run (lsdg_frame_ingest keeps only its image
+// levels, drops its depth and its refPixelWasGood mask) gets its derived
+// levels rebuilt unless the graph rebuilds it, and its mask reset; a Sim3
+// job's frame must still carry depth.  Frames whose data is in place cost
+// nothing here.
+static lsdg_status batch_refresh_frames(lsdg_batch b) {
+  lsdg_ctx c = b->ctx;
+  for (size_t k = 0; k < b->frames.size(); ++k) {
+    lsdg_frame f = b->frames[k];
+    if (b->kind == LSDG_JOB_SIM3 && !f->hasDepth)
+      return fail(c, LSDG_ESTATE, "a Sim3 job's frame has no depth (new content since the plan was built?)");
+    if (!b->rebuilt[k]) {
+      lsdg_status st = frame_require(f, b->needs[k]);
+      if (st != LSDG_OK) return st;
+    }
+    if (b->kind == LSDG_JOB_SE3) {
+      lsdg_status st = frame_ensure_was_good(f);
+      if (st != LSDG_OK) return st;
+    }
+  }
+  return LSDG_OK;
+}
+
 extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
   if (!b) return LSDG_EINVAL;
   lsdg_ctx c = b->ctx;
+  {
+    lsdg_status st = batch_refresh_frames(b);
+    if (st != LSDG_OK) return st;
+  }
   // keyframes whose depth changed since the plan was built need new point clouds
   bool stale = false;
   for (int j = 0; j < b->n; ++j) stale |= b->jobs[j].ref->kf->depthVersion != b->depthVersion[j];
@@ -541,6 +577,8 @@ extern "C" lsdg_status lsdg_batch_run(lsdg_batch b, lsdg_job_result* results) {
       if (st != LSDG_OK) return st;
     }
   }
+  // the graph rebuilt these frames' levels
+  for (size_t k = 0; k < b->pyrFrames.size(); ++k) b->pyrFrames[k]->valid |= b->pyrNeed[k];
   if (results || c->prof_on) {
     if (c->prof_on) CK(c, hipMemcpy(b->h_st, b->d_st, sizeof(JobState) * b->n, hipMemcpyDeviceToHost));
     if (results)

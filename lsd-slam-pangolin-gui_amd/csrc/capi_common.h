@@ -24,6 +24,7 @@ Xf pose_xf(const Pose& T, bool scaled);
 TrackKParams track_kparams(const lsdg_params& P, bool sim3);
 void esm_roll(const Pose& T, float* r4);
 double eval_bytes(int n, lsdg_frame f, int lvl, bool sim3);
+double adjoint_bytes(int n, lsdg_frame f, int lvl, bool sim3);
 
 // Kernel timing: begin() records the start event (if profiling is on) and
 // returns a handle for end(), which records the stop event and the bytes.

@@ -31,6 +31,8 @@ extern "C" lsdg_status lsdg_depthmap_create(lsdg_ctx c, const lsdg_camera* cam, 
   if (!c || !cam || !out) return LSDG_EINVAL;
   if (cam->w < 16 || cam->h < 16 || cam->w > 16384 || cam->h > 16384)
     return fail(c, LSDG_EINVAL, "depth map size out of range");
+  if ((cam->w & 1) || (cam->h & 1))  // the keyframes it maps are even-sized (frame_alloc)
+    return fail(c, LSDG_EINVAL, "depth map width and height must be even");
   lsdg_depthmap dm = new (std::nothrow) lsdg_depthmap_s();
   if (!dm) return LSDG_ENOMEM;
   dm->ctx = c;

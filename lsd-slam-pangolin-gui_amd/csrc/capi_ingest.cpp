@@ -22,6 +22,8 @@ extern "C" lsdg_status lsdg_undistorter_create(lsdg_ctx c, int32_t inW, int32_t 
   if (!c || !map || !out) return LSDG_EINVAL;
   if (inW < 2 || inH < 2 || outW < 1 || outH < 1 || inW > 16384 || inH > 16384 || outW > 16384 || outH > 16384)
     return fail(c, LSDG_EINVAL, "undistorter size out of range (2..16384 input, 1..16384 output)");
+  if ((outW & 1) || (outH & 1))  // the output is a frame's level 0 (even-sized)
+    return fail(c, LSDG_EINVAL, "undistorted width and height must be even");
   const size_t n = (size_t)outW * outH;
   // every source must keep both bilinear taps inside the input (NaN fails too)
   for (size_t i = 0; i < n; ++i) {

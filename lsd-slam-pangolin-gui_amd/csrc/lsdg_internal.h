@@ -43,6 +43,10 @@ enum Se3Slot { S3_A = 0, S3_B = 21, S3_ERR = 27, S3_RESUNW = 28, S3_SIGNED = 29,
                S3_SYY = 32, S3_SX = 33, S3_SY = 34, S3_SW = 35, S3_WARPED = 36, S3_GOOD = 37, S3_BAD = 38 };
 enum Sim3Slot { SM_A = 0, SM_B = 28, SM_RESP = 35, SM_RESD = 36, SM_NTD = 37, SM_WARPED = 38, SM_USAGE = 39,
                 SM_SXX = 40, SM_SYY = 41, SM_SX = 42, SM_SY = 43, SM_SW = 44 };
+// Sums of the adjoint evaluations.  SE3: dE/dxi (6), E, count.  Sim3: dE/dxi
+// (7), the photometric and depth parts of 2E, count.
+constexpr int NS_ADJ = 8;
+enum AdjSim3Slot { AS_G = 0, AS_EP = 7, AS_ED = 8, AS_COUNT = 9, NS_ADJ_SIM3 = 10 };
 
 struct LevelGeom {
   int w, h;
@@ -164,6 +168,9 @@ struct lsdg_ctx_s {
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> prof_events;
   lsdg_kernel_stats prof[LSDG_KERNEL_KINDS];
+  // per-point inverse-depth gradients of the adjoint, grown on demand
+  float* d_adj = nullptr;
+  int adjCap = 0;
   // keyframe export scratch (records, block offsets, vertices), grown on demand
   void* d_export = nullptr;
   size_t exportCap = 0;
@@ -287,7 +294,12 @@ hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int
 hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                             const TrackKParams& kp, double* d_out, const KTimer* t = nullptr);
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
-                              const TrackKParams& kp, float* d_grad_d, double* d_out);
+                              const TrackKParams& kp, float* d_grad_d, double* d_out, const KTimer* t = nullptr);
+// Sim3 adjoint (kp.use_esm must be 0): d_out receives dE/dxi[7], then the
+// photometric and depth parts of 2E and the in-image count.
+hipError_t launch_energy_grad_sim3(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                                   const TrackKParams& kp, float* d_grad_d, double* d_out,
+                                   const KTimer* t = nullptr);
 
 // depthmap.hip
 // Observe work list: slots (int4 each) for a w x h depth map, and the bytes of

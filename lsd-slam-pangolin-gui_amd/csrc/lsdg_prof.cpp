@@ -1,11 +1,11 @@
 // Profiling driver: the batched-tracking workload of bench.py (64 SE3
-// alignments of 640x480 frames, FP16 pyramids rebuilt per run, LM levels 4..1,
-// at most 20 iterations per level, seeds 1001 + 17 i) through the C-ABI alone,
+// alignments of 640x480 frames, pyramids rebuilt per run, LM levels 4..1, at
+// most 20 iterations per level, seeds 1001 + 17 i) through the C-ABI alone,
 // with no Python in the process, for counter passes of rocprofv3.  Optionally
 // first runs a plain device copy of a known byte count, the reference kernel
 // for scaling the HBM byte counters.
 //
-//   lsdg_prof [runs] [copy_mib]
+//   lsdg_prof [runs] [copy_mib] [precision: 0 FP32 (default), 1 FP16, 2 BF16]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -87,6 +87,12 @@ Instance make_instance(uint64_t seed, const lsdg_camera& cam) {
 int main(int argc, char** argv) {
   const int runs = argc > 1 ? std::atoi(argv[1]) : 5;
   const int copy_mib = argc > 2 ? std::atoi(argv[2]) : 0;
+  const int prec = argc > 3 ? std::atoi(argv[3]) : LSDG_FP32;
+  if (prec < LSDG_FP32 || prec > LSDG_BF16) {
+    std::fprintf(stderr, "precision must be 0, 1 or 2\n");
+    return 2;
+  }
+  static const char* kPrecName[] = {"FP32", "FP16", "BF16"};
   if (copy_mib > 0) {
     const size_t n4 = (size_t)copy_mib * (1u << 20) / 16;
     float4 *a = nullptr, *b = nullptr;
@@ -110,9 +116,9 @@ int main(int argc, char** argv) {
   std::vector<lsdg_job> jobs(B);
   for (int i = 0; i < B; ++i) {
     Instance in = make_instance(1001 + 17 * i, cam);
-    CHECK(lsdg_frame_create_u8(ctx, 2 * i, &cam, in.kf.data(), cam.w, LSDG_FP16, &kfs[i]));
+    CHECK(lsdg_frame_create_u8(ctx, 2 * i, &cam, in.kf.data(), cam.w, prec, &kfs[i]));
     CHECK(lsdg_frame_set_depth(kfs[i], in.idepth.data(), in.var.data()));
-    CHECK(lsdg_frame_create_u8(ctx, 2 * i + 1, &cam, in.fr.data(), cam.w, LSDG_FP16, &frs[i]));
+    CHECK(lsdg_frame_create_u8(ctx, 2 * i + 1, &cam, in.fr.data(), cam.w, prec, &frs[i]));
     CHECK(lsdg_ref_create(ctx, kfs[i], &refs[i]));
     lsdg_job& J = jobs[i];
     J.ref = refs[i];
@@ -137,8 +143,8 @@ int main(int argc, char** argv) {
   const double s = std::chrono::duration<double>(t1 - t0).count();
   int ok = 0;
   for (int i = 0; i < B; ++i) ok += res[i].status == LSDG_OK;
-  std::printf("batched SE3 640x480 x %d, FP16: %d runs, %.3f ms per run, %.1f frames/s, %d/%d jobs converged\n", B,
-              runs, 1e3 * s / runs, B * runs / s, ok, B);
+  std::printf("batched SE3 640x480 x %d, %s: %d runs, %.3f ms per run, %.1f frames/s, %d/%d jobs converged\n", B,
+              kPrecName[prec], runs, 1e3 * s / runs, B * runs / s, ok, B);
   lsdg_batch_destroy(b);
   for (int i = 0; i < B; ++i) {
     lsdg_ref_destroy(refs[i]);

@@ -92,7 +92,6 @@ __global__ __launch_bounds__(EVAL_BS) void k_sim3_eval(PointsLevel pts, const vo
 // --------------------------------------------------------------- adjoint
 // dE/dxi (6) and dE/d(idepth_i) of E = sum rho(sqrt(w_p) r) with w_p and the
 // in-image set frozen at x; rho Huber at kp.huber.  out: G[6], E, count.
-constexpr int NS_ADJ = 8;
 template <int PREC, int ACC>
 __global__ __launch_bounds__(EVAL_BS) void k_energy_grad(PointsLevel pts, const void* __restrict__ img, LevelGeom g,
                                                          Xf x, TrackKParams kp, float* __restrict__ grad_d,
@@ -102,6 +101,22 @@ __global__ __launch_bounds__(EVAL_BS) void k_energy_grad(PointsLevel pts, const 
   acc.zero();
   for (int i = blockIdx.x * EVAL_BS + threadIdx.x; i < pts.n; i += gridDim.x * EVAL_BS)
     adjoint_point<PREC>(pts, i, img, g, x, kp, grad_d, acc);
+  reduce_and_merge(acc, partials, counter, out);
+}
+
+// Sim3 adjoint: dE/dxi (7), 2E split into its photometric and depth parts,
+// count (AdjSim3Slot), dE/d(idepth_i) into grad_d.
+template <int PREC, int ACC>
+__global__ __launch_bounds__(EVAL_BS) void k_energy_grad_sim3(PointsLevel pts, const void* __restrict__ img,
+                                                              const float* __restrict__ fid,
+                                                              const float* __restrict__ fvar, LevelGeom g, Xf x,
+                                                              TrackKParams kp, float* __restrict__ grad_d,
+                                                              double* __restrict__ partials, unsigned* counter,
+                                                              double* __restrict__ out) {
+  Accum<ACC, NS_ADJ_SIM3> acc;
+  acc.zero();
+  for (int i = blockIdx.x * EVAL_BS + threadIdx.x; i < pts.n; i += gridDim.x * EVAL_BS)
+    adjoint_point_sim3<PREC>(pts, i, img, fid, fvar, g, x, kp, grad_d, acc);
   reduce_and_merge(acc, partials, counter, out);
 }
 
@@ -141,24 +156,27 @@ hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, in
 }
 
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
-                              const TrackKParams& kp, float* d_grad_d, double* d_out) {
+                              const TrackKParams& kp, float* d_grad_d, double* d_out, const KTimer* t) {
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
-  int nb = eval_blocks(c, pts.n);
+  const int nb = eval_blocks(c, pts.n);
   const void* img = level_image(f, level);
-  switch (f->precision) {
-    case LSDG_FP16:
-      k_energy_grad<LSDG_FP16, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, g, x, kp,
-                         d_grad_d, c->d_partials, c->d_counter, d_out);
-      break;
-    case LSDG_BF16:
-      k_energy_grad<LSDG_BF16, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, g, x, kp,
-                         d_grad_d, c->d_partials, c->d_counter, d_out);
-      break;
-    default:
-      k_energy_grad<LSDG_FP32, ACC_F64><<<dim3(nb), dim3(EVAL_BS), 0, c->stream>>>(pts, img, g, x, kp,
-                         d_grad_d, c->d_partials, c->d_counter, d_out);
-  }
-  return hipGetLastError();
+  auto k = f->precision == LSDG_FP16 ? k_energy_grad<LSDG_FP16, ACC_F64>
+           : f->precision == LSDG_BF16 ? k_energy_grad<LSDG_BF16, ACC_F64>
+                                       : k_energy_grad<LSDG_FP32, ACC_F64>;
+  return launch_k(k, dim3(nb), dim3(EVAL_BS), c->stream, t, pts, img, g, x, kp, d_grad_d, c->d_partials, c->d_counter,
+                  d_out);
+}
+
+hipError_t launch_energy_grad_sim3(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                                   const TrackKParams& kp, float* d_grad_d, double* d_out, const KTimer* t) {
+  LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
+  const void* img = level_image(f, level);
+  auto k = f->precision == LSDG_FP16 ? k_energy_grad_sim3<LSDG_FP16, ACC_F64>
+           : f->precision == LSDG_BF16 ? k_energy_grad_sim3<LSDG_BF16, ACC_F64>
+                                       : k_energy_grad_sim3<LSDG_FP32, ACC_F64>;
+  return launch_k(k, dim3(eval_blocks(c, pts.n)), dim3(EVAL_BS), c->stream, t, pts, img,
+                  (const float*)f->d_idepth[level], (const float*)f->d_ivar[level], g, x, kp, d_grad_d,
+                  c->d_partials, c->d_counter, d_out);
 }
 
 }  // namespace lsdg

@@ -545,4 +545,77 @@ __device__ inline void adjoint_point(const PointsLevel& pts, int i, const void* 
   grad_d[i] = -Wr * drpdd;
 }
 
+// One reference point of the Sim3 adjoint.  E is the Sim3 tracker's
+// reweighted least-squares energy with its weights frozen at x:
+//   E = 1/2 sum_i wh_i (w_p,i r_p,i^2 + w_d,i r_d,i^2),
+// wh the joint Huber weight, w_p / w_d the photometric / depth variance
+// weights, r_d = 1/z' - idepth_frame(nearest pixel); the frame's idepth
+// sample, the weights and the in-image set are constants of E.  Its twist
+// gradient is Sim3Tracker::calcSim3LGS's b (unnormalised) with the frame's own
+// gradient in place of the ESM average (kp.use_esm must be 0).  The
+// expressions are sim3_finish's, in the same order.
+//   dE/dd_i = -wh w_p r_p drpdd + wh w_d r_d g2   (g2 = d(1/z')/dd).
+template <int PREC, class ACC>
+__device__ inline void adjoint_point_sim3(const PointsLevel& pts, int i, const void* __restrict__ img,
+                                          const float* __restrict__ fid, const float* __restrict__ fvar,
+                                          const LevelGeom& g, const Xf& x, const TrackKParams& kp,
+                                          float* __restrict__ grad_d, ACC& acc) {
+  const float4 p = pts.posI[i];
+  const Proj w = project<PREC>(p, x, g);
+  if (!w.in) {
+    grad_d[i] = 0.0f;
+    return;
+  }
+  const float Wx = w.Wx, Wy = w.Wy, Wz = w.Wz;
+  Taps ri = interp3<PREC>(img, w.u, w.v, g.w);
+  const int ir = (int)(w.u + 0.5f) + g.w * (int)(w.v + 0.5f);
+  const float varF = fvar[ir], fidR = fid[ir];
+  float gx = g.fx * ri.gx, gy = g.fy * ri.gy;
+  float rp = (kp.aff_a * p.w + kp.aff_b) - ri.I;
+  float refId = frcp<PREC>(Wz);
+  float d = frcp<PREC>(p.z);
+  float rd, vard;
+  if (varF > 0.0f) {
+    rd = refId - fidR;
+    vard = varF;
+  } else {
+    rd = -1.0f;
+    vard = -1.0f;
+  }
+  float s = kp.var_weight * pts.var[i];
+  float sv = kp.var_weight * vard;
+  float g0 = fdiv<PREC>(x.t[0] * Wz - x.t[2] * Wx, Wz * Wz * d);
+  float g1 = fdiv<PREC>(x.t[1] * Wz - x.t[2] * Wy, Wz * Wz * d);
+  float g2 = fdiv<PREC>(Wz - x.t[2], Wz * Wz * d);
+  float drpdd = gx * g0 + gy * g1;
+  float w_p = frcp<PREC>(kp.noise2 + s * drpdd * drpdd);
+  float w_d = frcp<PREC>(sv + g2 * g2 * s);
+  float wrd = fabsf(rd * fsqrt<PREC>(w_d));
+  float wrp = fabsf(rp * fsqrt<PREC>(w_p));
+  float wabs = sv > 0.0f ? wrd + wrp : wrp;
+  float wh = fabsf(wabs < kp.huber ? 1.0f : fdiv<PREC>(kp.huber, wabs));
+  float wp = wh * w_p;
+  float wd = sv > 0.0f ? wh * w_d : 0.0f;
+  acc.add(AS_EP, wp * rp * rp);
+  if (sv > 0.0f) acc.add(AS_ED, wh * w_d * rd * rd);
+  acc.add(AS_COUNT, 1.0f);
+  float jv[6];
+  jac6<PREC>(Wx, Wy, Wz, gx, gy, jv);
+  float rpW = rp * wp;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) acc.add(AS_G + a, -(jv[a] * rpW));
+  float gd = -rpW * drpdd;
+  if (wd != 0.0f) {
+    float z = frcp<PREC>(Wz);
+    float z_sqr = frcp<PREC>(Wz * Wz);
+    float v4[4] = {z_sqr, z_sqr * Wy, -z_sqr * Wx, z};
+    const int remap[4] = {2, 3, 4, 6};
+    float rdW = rd * wd;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc.add(AS_G + remap[a], -(v4[a] * rdW));
+    gd = gd + rdW * g2;
+  }
+  grad_d[i] = gd;
+}
+
 }  // namespace lsdg

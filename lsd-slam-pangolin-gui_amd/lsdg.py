@@ -121,7 +121,7 @@ def hyp_struct(arrs):
 
 
 KERNEL_EVAL, KERNEL_PYRAMID, KERNEL_DEPTH, KERNEL_BATCH_EVAL, KERNEL_BATCH_SOLVE, KERNEL_BATCH_JOB, KERNEL_BATCH_COOP, \
-    KERNEL_INGEST = range(8)
+    KERNEL_INGEST, KERNEL_ADJOINT = range(9)
 
 
 class KernelStats(C.Structure):
@@ -262,6 +262,7 @@ def load():
         "lsdg_se3_evaluate": (i32, [vp, vp, vp, i32, dp, f32, f32, C.POINTER(Eval), fp]),
         "lsdg_sim3_evaluate": (i32, [vp, vp, vp, i32, dp, C.POINTER(Eval)]),
         "lsdg_energy_grad": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
+        "lsdg_energy_grad_sim3": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
         "lsdg_depthmap_create": (i32, [vp, C.POINTER(Camera), C.POINTER(vp)]),
         "lsdg_depthmap_destroy": (i32, [vp]),
         "lsdg_depth_reset": (i32, [vp]),
@@ -414,6 +415,16 @@ class Context:
         gd = np.zeros(n, np.float32)
         self.check(self.L.lsdg_energy_grad(self.h, ref.h, frame.h, level, pose7(pose), C.byref(e), g,
                                            gd.ctypes.data, n), "lsdg_energy_grad")
+        return e.value, np.array(g[:]), gd
+
+    def energy_grad_sim3(self, ref, frame, level, pose8_):
+        """Sim3 adjoint: (E, dE/dxi[7], dE/dd per point)."""
+        n = ref.num_points(level)
+        e = C.c_double()
+        g = (C.c_double * 7)()
+        gd = np.zeros(n, np.float32)
+        self.check(self.L.lsdg_energy_grad_sim3(self.h, ref.h, frame.h, level, pose8(pose8_), C.byref(e), g,
+                                                gd.ctypes.data, n), "lsdg_energy_grad_sim3")
         return e.value, np.array(g[:]), gd
 
     def points_vbo(self, records, w, h, cam4, params=None):

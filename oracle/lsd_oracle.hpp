@@ -923,6 +923,91 @@ struct Adjoint {
   }
 };
 
+// Sim3 adjoint: gradient of the Sim3 tracker's reweighted least-squares
+// energy with its weights frozen at refToFrame,
+//   E = 1/2 sum_i wh_i (w_p,i rp_i^2 + w_d,i rd_i^2),
+// with the per-point terms of calcSim3Buffers / calcSim3WeightsAndResidual
+// above (no ESM: the frame's own gradient) and the LGS6 + LGS4 -> LGS7 right
+// hand side of calcSim3LGS: dE/dxi = -sum (wp rp v + wd rd v4 at {2,3,4,6}),
+// dE/dd_i = -wp rp drpdd + wd rd g2.  out: gxi[7], E.
+template <class S, class Acc = S>
+struct AdjointSim3 {
+  lsdg_params P;
+  void eval(TrackingReference<S>& ref, Frame<S>& frame, const Sim3& refToFrame, int lvl, double* energy,
+            double* gxi, S* grad_d) {
+    ref.makePointCloud(lvl);
+    int W = frame.w[lvl], H = frame.h[lvl];
+    S fx = frame.fx[lvl], fy = frame.fy[lvl], cx = frame.cx[lvl], cy = frame.cy[lvl];
+    S R[9], t[3];
+    pose_RT(refToFrame, R, t, true);
+    const S* grad = frame.grad[lvl].data();
+    const S* fid = frame.idepth[lvl].data();
+    const S* fiv = frame.ivar[lvl].data();
+    S hd = (S)P.huber_d;
+    int N = ref.num[lvl];
+    Acc Ep = 0, Ed = 0, G[7] = {0};
+    const int remap[4] = {2, 3, 4, 6};
+    for (int i = 0; i < N; ++i) {
+      const S* p = &ref.pos[lvl][3 * i];
+      const S* cv = &ref.colVar[lvl][2 * i];
+      grad_d[i] = 0;
+      S Wx = R[0] * p[0] + R[1] * p[1] + R[2] * p[2] + t[0];
+      S Wy = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + t[1];
+      S Wz = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + t[2];
+      S u = (Wx / Wz) * fx + cx;
+      S v = (Wy / Wz) * fy + cy;
+      if (!(u > 1 && v > 1 && u < (S)(W - 2) && v < (S)(H - 2))) continue;
+      S ri[3];
+      interp4(grad, u, v, W, 3, ri);
+      S gx = fx * ri[0], gy = fy * ri[1];
+      S rp = cv[0] - ri[2];
+      int ir = (int)(u + (S)0.5) + W * (int)(v + (S)0.5);
+      S varF = fiv[ir];
+      S rd, vard;
+      if (varF > 0) {
+        rd = (S)1 / Wz - fid[ir];
+        vard = varF;
+      } else {
+        rd = -1;
+        vard = -1;
+      }
+      S d = (S)1 / p[2];
+      S s = (S)P.var_weight * cv[1];
+      S sv = (S)P.var_weight * vard;
+      S g0 = (t[0] * Wz - t[2] * Wx) / (Wz * Wz * d);
+      S g1 = (t[1] * Wz - t[2] * Wy) / (Wz * Wz * d);
+      S g2 = (Wz - t[2]) / (Wz * Wz * d);
+      S drpdd = gx * g0 + gy * g1;
+      S w_p = (S)1 / ((S)P.camera_pixel_noise2 + s * drpdd * drpdd);
+      S w_d = (S)1 / (sv + g2 * g2 * s);
+      S wrd = std::fabs(rd * std::sqrt(w_d));
+      S wrp = std::fabs(rp * std::sqrt(w_p));
+      S wabs = sv > 0 ? wrd + wrp : wrp;
+      S wh = std::fabs(wabs < hd ? (S)1 : hd / wabs);
+      S wp = wh * w_p;
+      S wd = sv > 0 ? wh * w_d : (S)0;
+      Ep += (Acc)(wp * rp * rp);
+      if (sv > 0) Ed += (Acc)(wh * w_d * rd * rd);
+      S vv[6];
+      SE3Tracker<S>::jac6(Wx, Wy, Wz, gx, gy, vv);
+      S rpW = rp * wp;
+      for (int a = 0; a < 6; ++a) G[a] -= (Acc)(vv[a] * rpW);
+      S gd = -rpW * drpdd;
+      if (wd != 0) {
+        S z = (S)1 / Wz;
+        S z_sqr = (S)1 / (Wz * Wz);
+        S v4[4] = {z_sqr, z_sqr * Wy, -z_sqr * Wx, z};
+        S rdW = rd * wd;
+        for (int a = 0; a < 4; ++a) G[remap[a]] -= (Acc)(v4[a] * rdW);
+        gd = gd + rdW * g2;
+      }
+      grad_d[i] = gd;
+    }
+    *energy = 0.5 * ((double)Ep + (double)Ed);
+    for (int a = 0; a < 7; ++a) gxi[a] = (double)G[a];
+  }
+};
+
 // ------------------------------------------------------------ DepthMap
 // Upstream DepthEstimation/DepthMap + DepthMapPixelHypothesis.  Output
 // evidence: the keyframe idepth / idepthVar arrays the wrapper reads

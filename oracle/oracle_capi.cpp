@@ -161,6 +161,9 @@ extern "C" void orc_params_default(lsdg_params* p) { params_default(p); }
 
 extern "C" void* orc_frame_new(int prec, int id, int w, int h, float fx, float fy, float cx, float cy, const float* img,
                     const lsdg_params* P) {
+  // propagateDepth indexes the level-1 tracking mask with (x >> 1, y >> 1):
+  // odd sizes would read past it (the C-ABI refuses them as well)
+  if (w < 2 || h < 2 || (w & 1) || (h & 1)) return nullptr;
   OFrame* f = new OFrame();
   f->prec = prec;
   f->dbl = prec == 1;
@@ -415,6 +418,115 @@ extern "C" int orc_energy_grad(void* r, void* f, const lsdg_params* P, int lvl, 
   return 0;
 }
 
+// Sim3 adjoint (frozen-weight reweighted least-squares energy); grad_d has
+// one entry per point.
+extern "C" int orc_energy_grad_sim3(void* r, void* f, const lsdg_params* P, int lvl, const double* pose8,
+                                    double* energy, double* gxi, double* grad_d) {
+  ORef* R = (ORef*)r;
+  Sim3 T = Sim3::from8(pose8);
+  const size_t n = (size_t)orc_ref_num_points(r, lvl);
+  if (R->dbl) {
+    AdjointSim3<double> a;
+    a.P = *P;
+    std::vector<double> g(n);
+    a.eval(R->r64, ((OFrame*)f)->f64, T, lvl, energy, gxi, g.data());
+    if (grad_d) std::copy(g.begin(), g.end(), grad_d);
+    return 0;
+  }
+  std::vector<float> g(n);
+  if (R->prec == 2) {
+    AdjointSim3<float, double> a;
+    a.P = *P;
+    a.eval(R->r32, ((OFrame*)f)->f32, T, lvl, energy, gxi, g.data());
+  } else {
+    AdjointSim3<float> a;
+    a.P = *P;
+    a.eval(R->r32, ((OFrame*)f)->f32, T, lvl, energy, gxi, g.data());
+  }
+  if (grad_d)
+    for (size_t i = 0; i < n; ++i) grad_d[i] = g[i];
+  return 0;
+}
+
+// Sim3 energy E = 1/2 sum_i wh_i (w_p,i rp_i^2 + w_d,i rd_i^2) at exp(xi7) * T
+// and point inverse depths d_i + delta_i, with the per-point weights
+// wp = wh w_p and wd = wh w_d, the nearest-pixel frame depth sample and the
+// in-image set frozen at T (FP64 only; for finite differences).  frozen holds
+// 3 doubles per point (wp, wd, frame idepth sample) and mask one byte; with
+// compute_frozen they are computed at T (xi and delta must then be zero).
+extern "C" int orc_energy_frozen_sim3(void* r, void* f, const lsdg_params* P, int lvl, const double* pose8,
+                                      const double* xi7, const double* delta_d, double* frozen,
+                                      uint8_t* mask_frozen, int compute_frozen, double* energy) {
+  ORef* R = (ORef*)r;
+  if (!R->dbl) return LSDG_EINVAL;
+  TrackingReference<double>& ref = R->r64;
+  Frame<double>& frame = ((OFrame*)f)->f64;
+  ref.makePointCloud(lvl);
+  Sim3 T0 = Sim3::from8(pose8);
+  Sim3 T = sim3_exp(xi7) * T0;
+  double Rm[9], t[3], R0[9], t0[3];
+  pose_RT(T, Rm, t, true);
+  pose_RT(T0, R0, t0, true);
+  const int W = frame.w[lvl], H = frame.h[lvl];
+  const double fx = frame.fx[lvl], fy = frame.fy[lvl], cx = frame.cx[lvl], cy = frame.cy[lvl];
+  const double hd = (double)P->huber_d;
+  const double* grad = frame.grad[lvl].data();
+  double E = 0;
+  const int N = ref.num[lvl];
+  for (int i = 0; i < N; ++i) {
+    const double* p0 = &ref.pos[lvl][3 * i];
+    const double* cv = &ref.colVar[lvl][2 * i];
+    double* fz = frozen + 3 * (size_t)i;
+    if (compute_frozen) {
+      double Wx = R0[0] * p0[0] + R0[1] * p0[1] + R0[2] * p0[2] + t0[0];
+      double Wy = R0[3] * p0[0] + R0[4] * p0[1] + R0[5] * p0[2] + t0[1];
+      double Wz = R0[6] * p0[0] + R0[7] * p0[1] + R0[8] * p0[2] + t0[2];
+      double u = (Wx / Wz) * fx + cx, v = (Wy / Wz) * fy + cy;
+      bool in = (u > 1 && v > 1 && u < W - 2 && v < H - 2);
+      mask_frozen[i] = in;
+      fz[0] = fz[1] = fz[2] = 0;
+      if (in) {
+        double ri[3];
+        interp4(grad, u, v, W, 3, ri);
+        double gx = fx * ri[0], gy = fy * ri[1], d = 1.0 / p0[2];
+        double rp = cv[0] - ri[2];
+        int ir = (int)(u + 0.5) + W * (int)(v + 0.5);
+        double varF = frame.ivar[lvl][ir];
+        double rd = varF > 0 ? 1.0 / Wz - frame.idepth[lvl][ir] : -1.0;
+        double vard = varF > 0 ? varF : -1.0;
+        double s = (double)P->var_weight * cv[1], sv = (double)P->var_weight * vard;
+        double g0 = (t0[0] * Wz - t0[2] * Wx) / (Wz * Wz * d);
+        double g1 = (t0[1] * Wz - t0[2] * Wy) / (Wz * Wz * d);
+        double g2 = (Wz - t0[2]) / (Wz * Wz * d);
+        double drpdd = gx * g0 + gy * g1;
+        double w_p = 1.0 / ((double)P->camera_pixel_noise2 + s * drpdd * drpdd);
+        double w_d = 1.0 / (sv + g2 * g2 * s);
+        double wabs = std::fabs(rp * std::sqrt(w_p)) + (sv > 0 ? std::fabs(rd * std::sqrt(w_d)) : 0.0);
+        double wh = wabs < hd ? 1.0 : hd / wabs;
+        fz[0] = wh * w_p;
+        fz[1] = sv > 0 ? wh * w_d : 0.0;
+        fz[2] = frame.idepth[lvl][ir];
+      }
+    }
+    if (!mask_frozen[i]) continue;
+    double d = 1.0 / p0[2] + (delta_d ? delta_d[i] : 0.0);
+    double sc = (1.0 / d) / p0[2];  // moves the point along its ray to the new inverse depth
+    double p[3] = {p0[0] * sc, p0[1] * sc, p0[2] * sc};
+    double Wx = Rm[0] * p[0] + Rm[1] * p[1] + Rm[2] * p[2] + t[0];
+    double Wy = Rm[3] * p[0] + Rm[4] * p[1] + Rm[5] * p[2] + t[1];
+    double Wz = Rm[6] * p[0] + Rm[7] * p[1] + Rm[8] * p[2] + t[2];
+    double u = (Wx / Wz) * fx + cx, v = (Wy / Wz) * fy + cy;
+    if (!(u > 1 && v > 1 && u < W - 2 && v < H - 2)) return LSDG_ESTATE;  // left the image: step too large
+    double ri[3];
+    interp4(grad, u, v, W, 3, ri);
+    double rp = cv[0] - ri[2];
+    double rd = 1.0 / Wz - fz[2];
+    E += 0.5 * (fz[0] * rp * rp + fz[1] * rd * rd);
+  }
+  *energy = E;
+  return 0;
+}
+
 // Energy with frozen per-point weights w_p and frozen in-image set, at a
 // perturbed pose exp(xi) * T and perturbed point inverse depths d_i + delta_i
 // (FP64 only; for finite differences).  w_frozen/mask_frozen come from the
@@ -480,6 +592,7 @@ extern "C" int orc_energy_frozen(void* r, void* f, const lsdg_params* P, int lvl
 
 extern "C" void* orc_depth_new(int prec, int w, int h, float fx, float fy, float cx, float cy, const lsdg_params* P,
                     int threads) {
+  if (w < 2 || h < 2 || (w & 1) || (h & 1)) return nullptr;
   ODepth* d = new ODepth();
   d->prec = prec;
   d->dbl = prec == 1;

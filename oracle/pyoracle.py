@@ -50,6 +50,8 @@ def load(fast=False):
         "orc_sim3_evaluate": (i32, [vp, vp, P, i32, dp, C.POINTER(lsdg.Eval)]),
         "orc_energy_grad": (i32, [vp, vp, P, i32, dp, dp, dp, fp]),
         "orc_energy_frozen": (i32, [vp, vp, P, i32, dp, dp, fp, fp, fp, i32, dp]),
+        "orc_energy_grad_sim3": (i32, [vp, vp, P, i32, dp, dp, dp, fp]),
+        "orc_energy_frozen_sim3": (i32, [vp, vp, P, i32, dp, dp, fp, fp, fp, i32, dp]),
         "orc_depth_new": (vp, [i32, i32, i32, f32, f32, f32, f32, P, i32]),
         "orc_depth_free": (None, [vp]),
         "orc_depth_set_threads": (None, [vp, i32]),
@@ -169,6 +171,8 @@ class Oracle:
         img = np.ascontiguousarray(image, np.float32)
         h = self.L.orc_frame_new(self.prec, fid, cam["w"], cam["h"], cam["fx"], cam["fy"], cam["cx"], cam["cy"],
                                  img.ctypes.data, C.byref(self.params))
+        if not h:
+            raise ValueError(f"oracle frame {cam['w']}x{cam['h']}: width and height must be even")
         return OFrame(self, h, fid, cam)
 
     def ref(self, kf):
@@ -210,9 +214,21 @@ class Oracle:
                                gd.ctypes.data)
         return e.value, np.array(g[:]), gd
 
+    def energy_grad_sim3(self, ref, frame, level, pose8_):
+        """Sim3 adjoint: (E, dE/dxi[7], dE/dd per point)."""
+        n = ref.num_points(level)
+        e = C.c_double()
+        g = (C.c_double * 7)()
+        gd = np.zeros(n, np.float64)
+        self.L.orc_energy_grad_sim3(ref.h, frame.h, C.byref(self.params), level, lsdg.pose8(pose8_), C.byref(e), g,
+                                    gd.ctypes.data)
+        return e.value, np.array(g[:]), gd
+
     def depthmap(self, cam, threads=1):
         h = self.L.orc_depth_new(self.prec, cam["w"], cam["h"], cam["fx"], cam["fy"], cam["cx"], cam["cy"],
                                  C.byref(self.params), threads)
+        if not h:
+            raise ValueError(f"oracle depth map {cam['w']}x{cam['h']}: width and height must be even")
         return ODepth(self, h, cam)
 
 

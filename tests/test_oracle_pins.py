@@ -15,6 +15,7 @@ from scipy.linalg import expm
 import lsdg
 import posemath
 import synth
+from helpers import frozen_energy, frozen_energy_sim3
 from pyoracle import Oracle, ldlt, load, se3_exp, sim3_exp, sim3_inv, sim3_mul
 
 
@@ -279,17 +280,6 @@ def _adjoint_setup():
     return o, ref, fr, ref_to_frame
 
 
-def _frozen_energy(o, ref, fr, lvl, pose, xi, delta, wf, mf, compute):
-    e = C.c_double()
-    dp = C.POINTER(C.c_double)
-    st = o.L.orc_energy_frozen(ref.h, fr.h, C.byref(o.params), lvl, lsdg.pose7(pose),
-                               np.ascontiguousarray(xi, np.float64).ctypes.data_as(dp),
-                               np.ascontiguousarray(delta, np.float64).ctypes.data_as(dp),
-                               wf.ctypes.data_as(dp), mf.ctypes.data, int(compute), C.byref(e))
-    assert st == 0, "a frozen point left the image"
-    return e.value
-
-
 @pytest.mark.parametrize("lvl", [1, 2])
 def test_adjoint_matches_central_differences(lvl):
     o, ref, fr, pose = _adjoint_setup()
@@ -298,23 +288,91 @@ def test_adjoint_matches_central_differences(lvl):
     wf = np.zeros(n, np.float64)
     mf = np.zeros(n, np.uint8)
     zero_d = np.zeros(n)
-    E0 = _frozen_energy(o, ref, fr, lvl, pose, np.zeros(6), zero_d, wf, mf, True)
+    E0 = frozen_energy(o, ref, fr, lvl, pose, np.zeros(6), zero_d, wf, mf, True)
     assert mf.sum() > 0.9 * n
     assert abs(E0 - E) <= 1e-12 * abs(E)
     h = 1e-6
     for k in range(6):
         e = np.zeros(6)
         e[k] = h
-        fd = (_frozen_energy(o, ref, fr, lvl, pose, e, zero_d, wf, mf, False)
-              - _frozen_energy(o, ref, fr, lvl, pose, -e, zero_d, wf, mf, False)) / (2 * h)
+        fd = (frozen_energy(o, ref, fr, lvl, pose, e, zero_d, wf, mf, False)
+              - frozen_energy(o, ref, fr, lvl, pose, -e, zero_d, wf, mf, False)) / (2 * h)
         assert abs(fd - g_xi[k]) <= 1e-6 * np.abs(g_xi).max(), (k, fd, g_xi[k])
     # E is separable in the point inverse depths: one directional derivative
     # along a random direction checks every dE/dd_i at once
     rng = np.random.default_rng(1)
     dirn = rng.normal(size=n)
-    fd = (_frozen_energy(o, ref, fr, lvl, pose, np.zeros(6), h * dirn, wf, mf, False)
-          - _frozen_energy(o, ref, fr, lvl, pose, np.zeros(6), -h * dirn, wf, mf, False)) / (2 * h)
+    fd = (frozen_energy(o, ref, fr, lvl, pose, np.zeros(6), h * dirn, wf, mf, False)
+          - frozen_energy(o, ref, fr, lvl, pose, np.zeros(6), -h * dirn, wf, mf, False)) / (2 * h)
     assert abs(fd - g_d @ dirn) <= 1e-6 * np.abs(g_d).sum()
+
+
+def _adjoint_setup_sim3(prec=1):
+    """Sim3 adjoint scene: textured keyframe points against an affine frame
+    image (exact central-difference gradients, as for SE3) whose inverse
+    depth is a smooth ramp with a hole of invalid pixels (no depth residual
+    there), at a Sim3 pose with a 3% scale.  ESM off: the adjoint's exact
+    derivative and the tracker's LGS then use the same gradient."""
+    inst = synth.tracking_instance(1011, cam=synth.camera(160, 120))
+    cam = inst["cam"]
+    p = lsdg.default_params()
+    p.use_esm_tracking = 0
+    o = Oracle(prec, params=p)
+    kf = o.frame(0, cam, inst["kf_img"])
+    kf.set_depth(inst["idepth"], inst["var"])
+    y, x = np.mgrid[0:cam["h"], 0:cam["w"]].astype(np.float64)
+    fr = o.frame(1, cam, (60.0 + 0.9 * x + 0.4 * y).astype(np.float32))
+    fid = (0.35 + 0.002 * x - 0.001 * y).astype(np.float32)
+    fvar = np.full(fid.shape, 2e-3, np.float32)
+    fvar[40:70, 60:100] = -1.0  # no frame depth: the point keeps only its photometric term
+    fr.set_depth(fid, fvar)
+    ref = o.ref(kf)
+    ref_to_frame = sim3_exp(np.array([0.01, -0.008, 0.005, 0.004, -0.003, 0.002, 0.03]))
+    return o, ref, fr, ref_to_frame
+
+
+@pytest.mark.parametrize("lvl", [1, 2])
+def test_sim3_adjoint_matches_central_differences(lvl):
+    o, ref, fr, pose = _adjoint_setup_sim3()
+    n = ref.num_points(lvl)
+    E, g_xi, g_d = o.energy_grad_sim3(ref, fr, lvl, pose)
+    fz = np.zeros(3 * n, np.float64)
+    mf = np.zeros(n, np.uint8)
+    zero_d = np.zeros(n)
+    E0 = frozen_energy_sim3(o, ref, fr, lvl, pose, np.zeros(7), zero_d, fz, mf, True)
+    assert mf.sum() > 0.9 * n
+    # both kinds of point are present: with and without a depth residual
+    wd = fz[1::3][mf.astype(bool)]
+    assert (wd > 0).sum() > 0.5 * mf.sum() and (wd == 0).sum() > 10
+    assert abs(E0 - E) <= 1e-12 * abs(E)
+    h = 1e-6
+    for k in range(7):
+        e = np.zeros(7)
+        e[k] = h
+        fd = (frozen_energy_sim3(o, ref, fr, lvl, pose, e, zero_d, fz, mf, False)
+              - frozen_energy_sim3(o, ref, fr, lvl, pose, -e, zero_d, fz, mf, False)) / (2 * h)
+        assert abs(fd - g_xi[k]) <= 1e-6 * np.abs(g_xi).max(), (k, fd, g_xi[k])
+    # the scale component is carried by the depth residual alone (the
+    # projection of a scaled point does not move): a real, non-zero term
+    assert abs(g_xi[6]) > 1e-3 * np.abs(g_xi).max()
+    rng = np.random.default_rng(2)
+    dirn = rng.normal(size=n)
+    fd = (frozen_energy_sim3(o, ref, fr, lvl, pose, np.zeros(7), h * dirn, fz, mf, False)
+          - frozen_energy_sim3(o, ref, fr, lvl, pose, np.zeros(7), -h * dirn, fz, mf, False)) / (2 * h)
+    assert abs(fd - g_d @ dirn) <= 1e-6 * np.abs(g_d).sum()
+
+
+def test_sim3_adjoint_gradient_is_the_lgs7_right_hand_side():
+    """dE/dxi is calcSim3LGS's b (LGS6 + LGS4 remapped to {2, 3, 4, 6}, not
+    divided by n) and E is half the weighted residual sums, at prec 1 (FP64)
+    and prec 2 (FP32 per-point terms, FP64 sums: the GPU's twin)."""
+    for prec in (1, 2):
+        o, ref, fr, pose = _adjoint_setup_sim3(prec)
+        for lvl in (1, 2, 3):
+            E, g_xi, _ = o.energy_grad_sim3(ref, fr, lvl, pose)
+            ev = o.sim3_evaluate(ref, fr, lvl, pose)
+            assert np.allclose(np.array(ev.b[:7]), g_xi, rtol=1e-12, atol=0), (prec, lvl)
+            assert abs(E - 0.5 * (ev.sum_res_p + ev.sum_res_d)) <= 1e-12 * abs(E)
 
 
 def test_adjoint_gradient_is_the_lgs_right_hand_side():
