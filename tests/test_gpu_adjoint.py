@@ -216,7 +216,9 @@ def test_sim3_adjoint_matches_fp64_central_differences(ctx):
         fd = central_differences(
             lambda e: frozen_energy_sim3(o, oref, ofr, level, pose, e, zero_d, fz, mf, False), 7)
         assert np.abs(g - fd).max() <= 1e-5 * np.abs(fd).max(), (level, g, fd)
-        assert abs(g[6]) > 1e-3 * np.abs(g).max()
+        # every component against its own size, the scale one included (it
+        # is carried by the depth residual alone and is ~1e-3 of the largest)
+        assert np.all(np.abs(g - fd) <= 5e-5 * np.abs(fd)), (level, np.abs(g - fd) / np.abs(fd))
         dirn = np.random.default_rng(5).normal(size=n)
         gdo = np.empty(n, np.float64)
         gdo[np.argsort(oref.points(level)[:, 7].astype(np.int64), kind="stable")] = _by_pixel(ref, level, gd)

@@ -192,3 +192,60 @@ def test_batch_sim3_1920x1080(ctx, schedule, precision):
         tol = 1e-9 if precision == lsdg.FP32 else 1e-5
         assert rig <= tol and sc <= tol, (j, rig, sc)
     assert list(res[0].frame_to_ref) == list(res[1].frame_to_ref)
+
+
+def _identity_table(w, h):
+    """Undistortion table that copies the image (integer source positions);
+    the last column and row have no two-tap source and stay 0."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    t = np.stack([x, y], -1)
+    t[:, -1] = -1.0
+    t[-1, :] = -1.0
+    return t
+
+
+def test_batch_rerun_after_new_frame_content(ctx, schedule):
+    """A plan run again after its frames were given new images
+    (lsdg_frame_ingest, no pyramid rebuild inside the batch) tracks the new
+    content: its poses equal a fresh plan's on frames made from the new
+    images, bit for bit.  A Sim3 plan whose frame lost its depth that way
+    reports LSDG_ESTATE."""
+    insts = _instances(3, 2091, [(320, 240)])
+    cam = insts[0]["cam"]
+    und = ctx.undistorter((cam["w"], cam["h"]), _identity_table(cam["w"], cam["h"]))
+    ctx.set_params(tracking_params(20))
+    try:
+        items = []
+        for i, inst in enumerate(insts):
+            kf = ctx.frame_undistorted(2 * i, cam, und, inst["kf_img"])
+            kf.set_depth(inst["idepth"], inst["var"])
+            fr = ctx.frame_undistorted(2 * i + 1, cam, und, insts[(i + 1) % 3]["fr_img"])
+            items.append((kf, fr, ctx.ref(kf)))
+        b = ctx.batch(_jobs(items, rebuild=0))
+        r_old = b.run()
+        for i, (kf, fr, ref) in enumerate(items):
+            fr.ingest(2 * i + 1, und, insts[i]["fr_img"])
+        r_new = b.run()
+        b.destroy()
+        fresh = []
+        for i, (kf, fr, ref) in enumerate(items):
+            fresh.append((kf, ctx.frame_undistorted(100 + i, cam, und, insts[i]["fr_img"]), ref))
+        bf = ctx.batch(_jobs(fresh, rebuild=0))
+        r_fresh = bf.run()
+        bf.destroy()
+        for i in range(3):
+            assert list(r_new[i].frame_to_ref) == list(r_fresh[i].frame_to_ref), i
+            assert list(r_new[i].stats.num_residual_calls) == list(r_fresh[i].stats.num_residual_calls)
+            assert list(r_new[i].frame_to_ref) != list(r_old[i].frame_to_ref), i
+        # Sim3: the frame's depth is gone after the new image
+        kf, fr, ref = items[0]
+        fr.set_depth(insts[0]["idepth"], insts[0]["var"])
+        bs = ctx.batch([lsdg.Job(ref.h, fr.h, lsdg.pose8(list(synth.IDENTITY7) + [1.0]), lsdg.JOB_SIM3, 4, 1, 0)])
+        bs.run()
+        fr.ingest(1, und, insts[0]["fr_img"])
+        with pytest.raises(lsdg.LsdgError, match="status -4"):
+            bs.run()
+        bs.destroy()
+    finally:
+        ctx.set_params(lsdg.default_params())
+        und.close()

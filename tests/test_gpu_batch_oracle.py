@@ -1,0 +1,163 @@
+"""The batched tracker against the oracle on the batched workloads.
+
+Mixed-resolution SE3 batch: 64 alignments (320x240 / 640x480 / 1280x960,
+seeded split, 15-35% semi-dense points, up to 5 cm / 3 deg), one persistent
+launch, levels 4..1, at most 20 LM iterations per level:
+  * FP32 pyramids: every job takes the oracle twin's LM path (FP32 per-point
+    terms, FP64 sums): same status and evaluation counts, pose within 1e-9;
+  * FP16 / BF16 pyramids: the pose error against the FP64 oracle is not worse
+    than the FP32 path's own (the spread of FP32 against FP64 comes from the
+    reference's stop rule and the non-convex energy, and bounds what any
+    precision can be held to per instance): the median stays within the
+    storage's quantisation floor and the largest error within the FP32
+    path's largest.
+
+Keyframe-pair Sim3 batch: 128 alignments of 1920x1080 keyframe pairs (~500k
+points each, scale drift U[0.9, 1.1], up to 10 cm / 5 deg), FP16 and BF16:
+pose, scale and depth-residual error against the FP64 oracle, next to the
+FP32 twin's own error against it."""
+import functools
+
+import numpy as np
+import pytest
+
+import lsdg
+import posemath
+import pyoracle
+import synth
+from helpers import tracking_params, twist
+
+pytestmark = pytest.mark.gpu
+
+
+@functools.lru_cache(maxsize=None)
+def _mixed():
+    return synth.batch_instances(64, seed=1003)
+
+
+@functools.lru_cache(maxsize=None)
+def _oracle_se3(prec):
+    o = pyoracle.Oracle(prec, params=tracking_params(20))
+    out = []
+    for inst in _mixed():
+        kf = o.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
+        kf.set_depth(inst["idepth"], inst["var"])
+        fr = o.frame(1, inst["cam"], inst["fr_img"].astype(np.float32))
+        out.append(o.se3_track(o.ref(kf), fr, synth.IDENTITY7))
+    return out
+
+
+def _batch_se3(ctx, precision):
+    ctx.set_params(tracking_params(20))
+    try:
+        items = []
+        for i, inst in enumerate(_mixed()):
+            kf = ctx.frame(2 * i, inst["cam"], inst["kf_img"], precision)
+            kf.set_depth(inst["idepth"], inst["var"])
+            fr = ctx.frame(2 * i + 1, inst["cam"], inst["fr_img"], precision)
+            items.append((kf, fr, ctx.ref(kf)))
+        jobs = [lsdg.Job(ref.h, fr.h, lsdg.pose8(list(synth.IDENTITY7) + [1.0]), lsdg.JOB_SE3, 4, 1, 1)
+                for kf, fr, ref in items]
+        b = ctx.batch(jobs)
+        res = b.run()
+        b.destroy()
+        for kf, fr, ref in items:
+            ref.destroy()
+            fr.destroy()
+            kf.destroy()
+        return res
+    finally:
+        ctx.set_params(lsdg.default_params())
+
+
+def test_mixed_batch_fp32_takes_the_oracle_path(ctx):
+    insts = _mixed()
+    assert {i["cam"]["w"] for i in insts} == {320, 640, 1280}
+    res = _batch_se3(ctx, lsdg.FP32)
+    twin = _oracle_se3(2)
+    for i, (r, (pose, st, status)) in enumerate(zip(res, twin)):
+        assert r.status == status, i
+        assert list(r.stats.num_residual_calls) == list(st.num_residual_calls), i
+        assert twist(r.frame_to_ref[:7], pose) <= 1e-9, (i, insts[i]["cam"]["w"])
+
+
+def _errors(poses, ref):
+    return np.array([twist(p, q[0]) for p, q in zip(poses, ref)])
+
+
+@pytest.mark.parametrize("precision,median_bar", [(lsdg.FP16, 1e-4), (lsdg.BF16, 1e-3)])
+def test_mixed_batch_reduced_precision_error_vs_fp64(ctx, precision, median_bar):
+    fp64 = _oracle_se3(1)
+    e32 = _errors([p[0] for p in _oracle_se3(2)], fp64)
+    res = _batch_se3(ctx, precision)
+    assert all(r.status == lsdg.OK for r in res)
+    e = _errors([list(r.frame_to_ref[:7]) for r in res], fp64)
+    name = {lsdg.FP16: "FP16", lsdg.BF16: "BF16"}[precision]
+    print(f"{name} vs FP64: max {e.max():.3e} median {np.median(e):.3e} >1e-5: {(e > 1e-5).sum()}/64; "
+          f"FP32 vs FP64: max {e32.max():.3e} median {np.median(e32):.3e} >1e-5: {(e32 > 1e-5).sum()}/64")
+    assert np.median(e) <= median_bar
+    assert e.max() <= max(2.0 * e32.max(), 1e-3)
+
+
+@functools.lru_cache(maxsize=None)
+def _pairs():
+    return [synth.sim3_instance(1005 + k) for k in range(8)]
+
+
+@functools.lru_cache(maxsize=None)
+def _oracle_sim3(prec):
+    o = pyoracle.Oracle(prec)
+    out = []
+    for d in _pairs():
+        a = o.frame(0, d["cam"], d["a_img"].astype(np.float32))
+        a.set_depth(d["a_idepth"], d["a_var"])
+        b = o.frame(1, d["cam"], d["b_img"].astype(np.float32))
+        b.set_depth(d["b_idepth"], d["b_var"])
+        out.append(o.sim3_track(o.ref(a), b, list(synth.IDENTITY7) + [1.0], 4, 1))
+    return out
+
+
+def _sim3_errors(poses, stats, ref):
+    rig, sc, dres = [], [], []
+    for p, st, (q, qst, _) in zip(poses, stats, ref):
+        r, s = posemath.sim3_error(list(p), list(q))
+        rig.append(r)
+        sc.append(s)
+        dres.append(abs(st.last_depth_residual - qst.last_depth_residual) / abs(qst.last_depth_residual))
+    return np.array(rig), np.array(sc), np.array(dres)
+
+
+@pytest.mark.parametrize("precision", [lsdg.FP16, lsdg.BF16])
+def test_sim3_batch_1920x1080_error_vs_fp64(ctx, precision):
+    """128 jobs in one batch (8 keyframe pairs x 16 jobs each): identical jobs
+    give identical results, and each pair's pose, scale and final depth
+    residual lie within the stated bars of the FP64 oracle's."""
+    pairs = _pairs()
+    items = []
+    for i, d in enumerate(pairs):
+        a = ctx.frame(2 * i, d["cam"], d["a_img"], precision)
+        a.set_depth(d["a_idepth"], d["a_var"])
+        b = ctx.frame(2 * i + 1, d["cam"], d["b_img"], precision)
+        b.set_depth(d["b_idepth"], d["b_var"])
+        items.append((a, b, ctx.ref(a)))
+    ident = list(synth.IDENTITY7) + [1.0]
+    jobs = [lsdg.Job(ref.h, b.h, lsdg.pose8(ident), lsdg.JOB_SIM3, 4, 1, 0) for a, b, ref in items for _ in range(16)]
+    assert len(jobs) == 128
+    bt = ctx.batch(jobs)
+    res = bt.run()
+    bt.destroy()
+    assert all(r.status == lsdg.OK for r in res)
+    for k in range(8):
+        for j in range(1, 16):
+            assert list(res[16 * k + j].frame_to_ref) == list(res[16 * k].frame_to_ref)
+    first = [res[16 * k] for k in range(8)]
+    fp64 = _oracle_sim3(1)
+    rig, sc, dres = _sim3_errors([r.frame_to_ref for r in first], [r.stats for r in first], fp64)
+    twin = _oracle_sim3(2)
+    rig32, sc32, dres32 = _sim3_errors([t[0] for t in twin], [t[1] for t in twin], fp64)
+    name = {lsdg.FP16: "FP16", lsdg.BF16: "BF16"}[precision]
+    print(f"Sim3 {name} vs FP64: pose max {rig.max():.3e} scale max {sc.max():.3e} depth-res max {dres.max():.3e}; "
+          f"FP32 twin vs FP64: pose {rig32.max():.3e} scale {sc32.max():.3e} depth-res {dres32.max():.3e}")
+    assert rig.max() <= max(3.0 * rig32.max(), 5e-3)
+    assert sc.max() <= max(3.0 * sc32.max(), 5e-3)
+    assert dres.max() <= max(3.0 * dres32.max(), 5e-2)
