@@ -320,6 +320,23 @@ lsdg_status lsdg_energy_grad_sim3(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, 
                                   const double ref_to_frame[8], double* energy, double grad_xi[7],
                                   float* grad_idepth, int32_t max_points);
 
+/* lsdg_energy_grad with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Its reduction rows, counter, merged row and
+ * per-point dE/dd staging live in a fixed device array of about 4.1 MB per
+ * GPU (held until the process ends), so:
+ *   - the reference's point cloud of `level` and the frame's pyramid level
+ *     must already be built (lsdg_ref_make_point_cloud,
+ *     lsdg_frame_build_pyramid), else LSDG_ESTATE;
+ *   - at most 1048576 points (LSDG_EINVAL beyond);
+ *   - every call on one GPU shares that GPU's array: calls that overlap on
+ *     one GPU (several contexts or threads) race.
+ * Results equal lsdg_energy_grad's bit for bit. */
+#define LSDG_NOALLOC_MAX_POINTS 1048576
+lsdg_status lsdg_energy_grad_noalloc(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, int32_t level,
+                                     const double ref_to_frame[7], double* energy, double grad_xi[6],
+                                     float* grad_idepth, int32_t max_points);
+
 /* ------------------------------------------------------------- depth map */
 
 lsdg_status lsdg_depthmap_create(lsdg_ctx ctx, const lsdg_camera* cam, lsdg_depthmap* out);

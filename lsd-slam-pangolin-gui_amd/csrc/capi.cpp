@@ -122,7 +122,8 @@ extern "C" lsdg_status lsdg_ctx_create(int32_t device, const lsdg_params* params
       hipMalloc(&c->d_partials, sizeof(double) * NS_MAX * c->partials_blocks) != hipSuccess ||
       hipMalloc(&c->d_counter, 64) != hipSuccess || hipMemset(c->d_counter, 0, 64) != hipSuccess ||
       hipHostMalloc(&c->h_result, sizeof(double) * 64, hipHostMallocMapped) != hipSuccess ||
-      hipHostGetDevicePointer((void**)&c->d_result, c->h_result, 0) != hipSuccess) {
+      hipHostGetDevicePointer((void**)&c->d_result, c->h_result, 0) != hipSuccess ||
+      noalloc_scratch_init(device) != hipSuccess) {  // the allocation-free entry point's device array
     lsdg_ctx_destroy(c);
     return LSDG_EDEVICE;
   }
@@ -1074,5 +1075,38 @@ extern "C" lsdg_status lsdg_energy_grad_sim3(lsdg_ctx c, lsdg_ref r, lsdg_frame 
   if (energy) *energy = 0.5 * (R[AS_EP] + R[AS_ED]);
   if (grad_xi)
     for (int a = 0; a < 7; ++a) grad_xi[a] = R[AS_G + a];
+  return LSDG_OK;
+}
+
+// The SE3 adjoint with no device allocation: its reduction rows, counter,
+// merged row and per-point staging live in a file-scope device array
+// (track.hip), so the point cloud and the frame's pyramid level it reads must
+// already exist (building them allocates).
+extern "C" lsdg_status lsdg_energy_grad_noalloc(lsdg_ctx c, lsdg_ref r, lsdg_frame f, int32_t level,
+                                                const double pose[7], double* energy, double grad_xi[6],
+                                                float* grad_idepth, int32_t max_points) {
+  if (!c || !r || !f || !pose) return LSDG_EINVAL;
+  if (level < 0 || level >= PYR) return fail(c, LSDG_EINVAL, "level out of range");
+  if (r->ctx != c || f->ctx != c) return fail(c, LSDG_EINVAL, "objects belong to another context");
+  if (f->w[0] != r->kf->w[0] || f->h[0] != r->kf->h[0]) return fail(c, LSDG_EINVAL, "frame / keyframe size mismatch");
+  if (!r->kf->hasDepth || r->depthVersion != r->kf->depthVersion || r->num[level] < 0)
+    return fail(c, LSDG_ESTATE, "point cloud of this level not built (lsdg_ref_make_point_cloud first)");
+  const unsigned need = fd_track(f, level, level);
+  if ((f->valid & need) != need)
+    return fail(c, LSDG_ESTATE, "frame pyramid level not built (lsdg_frame_build_pyramid first)");
+  const PointsLevel& pl = r->lv[level];
+  if (pl.n > NOALLOC_MAX_POINTS)
+    return fail(c, LSDG_EINVAL, "more points than the allocation-free adjoint holds (1048576)");
+  if (grad_idepth && max_points < pl.n) return fail(c, LSDG_EINVAL, "grad_idepth buffer too small");
+  double R[NS_ADJ] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (pl.n > 0) {
+    TrackKParams kp = track_kparams(c->P, false);
+    kp.use_esm = 0;
+    hipError_t e = launch_energy_grad_noalloc(c, pl, f, level, pose_xf(pose_from7(pose), false), kp, R, grad_idepth);
+    if (e != hipSuccess) return fail(c, LSDG_EDEVICE, "energy_grad_noalloc", e);
+  }
+  if (energy) *energy = R[6];
+  if (grad_xi)
+    for (int a = 0; a < 6; ++a) grad_xi[a] = R[a];
   return LSDG_OK;
 }

@@ -295,6 +295,16 @@ hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, in
                             const TrackKParams& kp, double* d_out, const KTimer* t = nullptr);
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                               const TrackKParams& kp, float* d_grad_d, double* d_out, const KTimer* t = nullptr);
+// lsdg_energy_grad_noalloc: limits of its file-scope device array, the
+// once-per-device lookup of that array's address (lsdg_ctx_create calls it
+// with the device current) and the launch: clear the counter, run
+// k_energy_grad over the array, copy the merged row (NS_ADJ doubles) and, if
+// h_grad is set, the per-point dE/dd to host memory, synchronise.
+constexpr int NOALLOC_MAX_POINTS = 1 << 20;
+constexpr int NOALLOC_MAX_BLOCKS = 1024;
+hipError_t noalloc_scratch_init(int device);
+hipError_t launch_energy_grad_noalloc(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                                      const TrackKParams& kp, double* h_row, float* h_grad);
 // Sim3 adjoint (kp.use_esm must be 0): d_out receives dE/dxi[7], then the
 // photometric and depth parts of 2E and the in-image count.
 hipError_t launch_energy_grad_sim3(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,

@@ -16,6 +16,8 @@
 //               written straight to the caller's (possibly host-mapped) buffer.
 // The per-point float expressions keep the reference's operation order so the
 // isGood mask and counts match the CPU restatement bit for bit.
+#include <mutex>
+
 #include "track_device.h"
 
 namespace lsdg {
@@ -177,6 +179,79 @@ hipError_t launch_energy_grad_sim3(lsdg_ctx c, const PointsLevel& pts, lsdg_fram
   return launch_k(k, dim3(eval_blocks(c, pts.n)), dim3(EVAL_BS), c->stream, t, pts, img,
                   (const float*)f->d_idepth[level], (const float*)f->d_ivar[level], g, x, kp, d_grad_d,
                   c->d_partials, c->d_counter, d_out);
+}
+
+// ------------------------------------------------ allocation-free adjoint
+// lsdg_energy_grad_noalloc's device memory: one file-scope array per GPU,
+// loaded with the code object and held until the process ends.  Layout:
+//   [0, PARTIALS)              FP64 partial rows, NOALLOC_MAX_BLOCKS x NS_ADJ
+//   [COUNTER, +256)            the last-block counter (unsigned)
+//   [RESULT, +256)             the merged row (NS_ADJ doubles)
+//   [GRAD, +4 * max points)    per-point dE/dd staging
+// Host code uses the address hipGetSymbolAddress returns for the current
+// device, never the array's name (which is a host-side stand-in).  Every
+// call on a device shares that device's array: calls that overlap on one
+// GPU race.
+namespace {
+constexpr size_t NOALLOC_PARTIALS = (size_t)NOALLOC_MAX_BLOCKS * NS_ADJ * sizeof(double);
+constexpr size_t NOALLOC_COUNTER = NOALLOC_PARTIALS;
+constexpr size_t NOALLOC_RESULT = NOALLOC_COUNTER + 256;
+constexpr size_t NOALLOC_GRAD = NOALLOC_RESULT + 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_GRAD + (size_t)NOALLOC_MAX_POINTS * sizeof(float);
+static_assert(NOALLOC_PARTIALS % 256 == 0, "sections stay 256-byte aligned");
+static_assert(NS_ADJ * sizeof(double) <= 256, "the merged row fits its section");
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = (unsigned char*)ptr;
+  return hipSuccess;
+}
+
+__global__ void k_noalloc_clear(unsigned* counter) { *counter = 0u; }
+
+hipError_t launch_energy_grad_noalloc(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
+                                      const TrackKParams& kp, double* h_row, float* h_grad) {
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) base = g_scratch_ptr[c->device];
+  }
+  if (!base) return hipErrorNotInitialized;
+  if (pts.n > NOALLOC_MAX_POINTS) return hipErrorInvalidValue;
+  double* partials = (double*)base;
+  unsigned* counter = (unsigned*)(base + NOALLOC_COUNTER);
+  double* row = (double*)(base + NOALLOC_RESULT);
+  float* grad = (float*)(base + NOALLOC_GRAD);
+  LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
+  int nb = eval_blocks(c, pts.n);
+  if (nb > NOALLOC_MAX_BLOCKS) nb = NOALLOC_MAX_BLOCKS;
+  // the counter must start at zero whatever the last call on this device left
+  k_noalloc_clear<<<dim3(1), dim3(1), 0, c->stream>>>(counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  auto k = f->precision == LSDG_FP16 ? k_energy_grad<LSDG_FP16, ACC_F64>
+           : f->precision == LSDG_BF16 ? k_energy_grad<LSDG_BF16, ACC_F64>
+                                       : k_energy_grad<LSDG_FP32, ACC_F64>;
+  e = launch_k(k, dim3(nb), dim3(EVAL_BS), c->stream, nullptr, pts, level_image(f, level), g, x, kp, grad, partials,
+               counter, row);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(h_row, row, sizeof(double) * NS_ADJ, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && h_grad)
+    e = hipMemcpyAsync(h_grad, grad, sizeof(float) * pts.n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
 }
 
 }  // namespace lsdg

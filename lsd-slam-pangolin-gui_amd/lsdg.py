@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PYR = 5
 
 OK, DIVERGED, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, -1, -2, -3, -4
+NOALLOC_MAX_POINTS = 1 << 20  # LSDG_NOALLOC_MAX_POINTS
 FP32, FP16, BF16 = 0, 1, 2
 WHAT_IMAGE, WHAT_GRADIENTS, WHAT_MAXGRAD, WHAT_IDEPTH, WHAT_IDEPTH_VAR, WHAT_REF_PIXEL_WAS_GOOD = range(6)
 JOB_SE3, JOB_SIM3 = 0, 1
@@ -263,6 +264,7 @@ def load():
         "lsdg_sim3_evaluate": (i32, [vp, vp, vp, i32, dp, C.POINTER(Eval)]),
         "lsdg_energy_grad": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
         "lsdg_energy_grad_sim3": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
+        "lsdg_energy_grad_noalloc": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
         "lsdg_depthmap_create": (i32, [vp, C.POINTER(Camera), C.POINTER(vp)]),
         "lsdg_depthmap_destroy": (i32, [vp]),
         "lsdg_depth_reset": (i32, [vp]),
