@@ -7,9 +7,12 @@ plain.  A step is one pass of the hot path over one batch of 64 synthetic
 1 deg perturbation with sigma-2 noise, 5-level pyramid, levels 4..1, up to 20
 LM iterations per level): every frame's pyramid is rebuilt from its resident
 level-0 image and every frame is tracked.  `value` is the batched path with
-FP16 pyramids (one persistent launch per LM step, hipGraph-replayed); the plain
-path (one alignment per call) and FP32 pyramids are reported beside it.
-Inputs are resident in HBM when the timed region starts.
+FP32 pyramids (the reference's image precision; one persistent launch per
+batch, hipGraph-replayed, in the shape the plan tuner picked); FP16 and BF16
+pyramids, the plain path (one alignment per call) and the CPU baseline are
+reported beside it, with every path's pose error against the FP64 oracle
+over all 64 instances.  Inputs are resident in HBM when the timed region
+starts.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -31,10 +34,59 @@ import posemath  # noqa: E402
 import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the headline's dominant kernel, from the FETCH_SIZE /
-# WRITE_SIZE counter passes scaled by a 256 MiB copy kernel (profiles/);
-# re-measured when the kernel changes.
-TRAFFIC_BYTES = {"k_batch_coop": 275632378, "k_batch_eval": 10134461}
+# Sources that define the batched kernels' code: a counter record of HBM
+# traffic (profiles/traffic.json) counts only for the sources it was measured on.
+KERNEL_SOURCES = ("batch.hip", "batch_internal.h", "track_device.h", "lsdg_internal.h", "lsdg_lie.h",
+                  "capi_batch.cpp")
+
+
+def kernel_sources_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "lsd-slam-pangolin-gui_amd", "csrc", name), "rb") as f:
+            h.update(name.encode())
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def measured_traffic(record):
+    """HBM bytes per launch (FETCH_SIZE and WRITE_SIZE passes, scaled by a
+    copy kernel of known size) from the committed counter record of this
+    kernel and workload, when it was measured on the current kernel sources;
+    (None, reason) otherwise."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None, "no counter record"
+    r = data.get(record)
+    if not r:
+        return None, f"no counter record for {record}"
+    if r.get("sources_sha256") != kernel_sources_sha():
+        return None, "the counter record was taken on other kernel sources"
+    return r["bytes"], r["profile"]
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The host threads a CPU baseline may use: the process's CPU set, at most
+    16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def tracking_params(its=20, crop=None):
@@ -68,10 +120,19 @@ class Tracking:
             fr.invalidate()  # the tracker rebuilds the levels it reads, as the batch does
             self.poses[i] = self.ctx.se3_track(ref, fr, synth.IDENTITY7)[0]
 
-    def make_batch(self):
-        jobs = [lsdg.Job(ref.h, fr.h, lsdg.pose8(synth.IDENTITY7), lsdg.JOB_SE3, 4, 1, 1)
+    def jobs(self):
+        return [lsdg.Job(ref.h, fr.h, lsdg.pose8(synth.IDENTITY7), lsdg.JOB_SE3, 4, 1, 1)
                 for kf, fr, ref in self.items]
-        self.batch = self.ctx.batch(jobs)
+
+    def tune(self, runs=5):
+        """Searches the plan shapes for these jobs (lsdg_batch_tune); later
+        plans of this class take the winner."""
+        best, default_ms, best_ms = self.ctx.batch_tune(self.jobs(), runs)
+        return {"shape": best, "default_ms": round(default_ms, 4), "tuned_ms": round(best_ms, 4),
+                "gain": round(default_ms / best_ms, 3)}
+
+    def make_batch(self):
+        self.batch = self.ctx.batch(self.jobs())
 
     def batch_step(self):
         """One batch run; the C call copies every job's pose and statistics
@@ -84,33 +145,112 @@ class Tracking:
         return self.poses
 
 
-def sim3_leg(ctx, steps, warmup, pairs=16, per_pair=8):
+def sim3_leg(ctx, steps, warmup, pairs=16, per_pair=8, check_fp64=True):
     """Keyframe-to-keyframe Sim3 at 1920x1080 (~500k points per keyframe,
-    scale drift U[0.9, 1.1], up to 10 cm / 5 deg, FP16 pyramids, default LM
-    settings): 16 keyframe pairs x 8 jobs per batch, and the same pairs
-    through the plain path."""
+    scale drift U[0.9, 1.1], up to 10 cm / 5 deg, default LM settings): 16
+    keyframe pairs x 8 jobs per batch, FP16 and BF16 pyramids, the FP16 pairs
+    through the plain path beside them; each pair's pose, scale and final
+    depth residual against the FP64 oracle, next to the FP32 oracle twin's."""
     ident = list(synth.IDENTITY7) + [1.0]
-    items = []
-    for k in range(pairs):
-        d = synth.sim3_instance(1005 + k)
-        a = ctx.frame(2 * k, d["cam"], d["a_img"], lsdg.FP16)
-        a.set_depth(d["a_idepth"], d["a_var"])
-        b = ctx.frame(2 * k + 1, d["cam"], d["b_img"], lsdg.FP16)
-        b.set_depth(d["b_idepth"], d["b_var"])
-        items.append((a, b, ctx.ref(a)))
-    for a, b, ref in items:
-        for l in range(1, 5):
-            ref.num_points(l)
-    jobs = [lsdg.Job(ref.h, b.h, lsdg.pose8(ident), lsdg.JOB_SIM3, 4, 1, 0) for a, b, ref in items
-            for _ in range(per_pair)]
-    bt = ctx.batch(jobs)
-    t = timed(ctx, bt.run, steps, warmup)
-    batched = len(jobs) * steps / t
-    t = timed(ctx, lambda: [ctx.sim3_track(ref, b, ident, 4, 1) for a, b, ref in items], 1, 1)
-    plain = pairs / t
-    bt.destroy()
-    return {"batched_alignments_per_s": round(batched, 1), "plain_alignments_per_s": round(plain, 1),
-            "jobs_per_batch": len(jobs)}
+    data = [synth.sim3_instance(1005 + k) for k in range(pairs)]
+    out = {"jobs_per_batch": pairs * per_pair}
+    results = {}
+    for name, prec in (("fp16", lsdg.FP16), ("bf16", lsdg.BF16)):
+        items = []
+        for k, d in enumerate(data):
+            a = ctx.frame(2 * k, d["cam"], d["a_img"], prec)
+            a.set_depth(d["a_idepth"], d["a_var"])
+            b = ctx.frame(2 * k + 1, d["cam"], d["b_img"], prec)
+            b.set_depth(d["b_idepth"], d["b_var"])
+            items.append((a, b, ctx.ref(a)))
+        for a, b, ref in items:
+            for l in range(1, 5):
+                ref.num_points(l)
+        jobs = [lsdg.Job(ref.h, b.h, lsdg.pose8(ident), lsdg.JOB_SIM3, 4, 1, 0) for a, b, ref in items
+                for _ in range(per_pair)]
+        bt = ctx.batch(jobs)
+        t = timed(ctx, bt.run, steps, warmup)
+        out[f"batched_{name}_alignments_per_s"] = round(len(jobs) * steps / t, 1)
+        res = bt.run()
+        results[name] = [(list(res[per_pair * k].frame_to_ref), res[per_pair * k].stats) for k in range(pairs)]
+        bt.destroy()
+        if prec == lsdg.FP16:
+            t = timed(ctx, lambda: [ctx.sim3_track(ref, b, ident, 4, 1) for a, b, ref in items], 1, 1)
+            out["plain_fp16_alignments_per_s"] = round(pairs / t, 1)
+        for a, b, ref in items:
+            ref.destroy()
+            b.destroy()
+            a.destroy()
+    if check_fp64:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        ref64 = {}
+        for prec in (1, 2):
+            o = pyoracle.Oracle(prec)
+            ref64[prec] = []
+            for d in data:
+                a = o.frame(0, d["cam"], d["a_img"].astype(np.float32))
+                a.set_depth(d["a_idepth"], d["a_var"])
+                b = o.frame(1, d["cam"], d["b_img"].astype(np.float32))
+                b.set_depth(d["b_idepth"], d["b_var"])
+                q, st, _ = o.sim3_track(o.ref(a), b, ident, 4, 1)
+                ref64[prec].append((q, st))
+
+        def errs(rows):
+            rig, sc, dr = [], [], []
+            for (p, st), (q, qst) in zip(rows, ref64[1]):
+                e = posemath.sim3_error(p, q)
+                rig.append(e[0])
+                sc.append(e[1])
+                dr.append(abs(st.last_depth_residual - qst.last_depth_residual) / abs(qst.last_depth_residual))
+            return {"pose_twist_max": float(np.max(rig)), "pose_twist_median": float(np.median(rig)),
+                    "scale_max": float(np.max(sc)), "depth_residual_rel_max": float(np.max(dr))}
+        out["error_vs_fp64_oracle"] = {"fp16_batched": errs(results["fp16"]), "bf16_batched": errs(results["bf16"]),
+                                       "fp32_oracle_twin": errs(ref64[2]), "pairs": pairs}
+    return out
+
+
+def adjoint_leg(ctx, steps, warmup, cpu_seconds):
+    """Gradient of the Huber-weighted photometric energy (lsdg_energy_grad)
+    at a 1280x960 keyframe with ~300k semi-dense points (level 0) against a
+    frame at a 3 cm / 2 deg perturbation, FP32: evaluations per second of the
+    C call (launch, reduction, copy of the ~300k per-point gradients to host
+    memory), the adjoint kernel's device time and algorithmic bytes
+    (B_eval + 4 N per launch); the oracle's FP32 restatement, one thread,
+    beside it."""
+    inst = synth.tracking_instance(1004, cam=synth.camera(1280, 960), trans_m=0.03, rot_deg=2.0, threshold=None,
+                                   fraction=0.25)
+    kf = ctx.frame(0, inst["cam"], inst["kf_img"])
+    kf.set_depth(inst["idepth"], inst["var"])
+    fr = ctx.frame(1, inst["cam"], inst["fr_img"])
+    ref = ctx.ref(kf)
+    n = ref.num_points(0)
+    fr.build_pyramid()
+    pose = list(synth.IDENTITY7)
+    t = timed(ctx, lambda: ctx.energy_grad(ref, fr, 0, pose), steps, warmup)
+    achieved, ks = kernel_roofline(ctx, lambda: [ctx.energy_grad(ref, fr, 0, pose) for _ in range(20)],
+                                   lsdg.KERNEL_ADJOINT)
+    out = {"gradient_evaluations_per_s": round(steps / t, 1), "points": n,
+           "kernel_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 3), "kernel_gbs": round(achieved, 2),
+           "kernel_bytes_per_launch": round(ks["bytes"] / max(ks["launches"], 1), 1),
+           "kernel_frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 5)}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(0, fast=True)
+        okf = o.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
+        okf.set_depth(inst["idepth"], inst["var"])
+        ofr = o.frame(1, inst["cam"], inst["fr_img"].astype(np.float32))
+        oref = o.ref(okf)
+        o.energy_grad(oref, ofr, 0, pose)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            o.energy_grad(oref, ofr, 0, pose)
+            k += 1
+        out["cpu_gradient_evaluations_per_s"] = round(k / (time.perf_counter() - t0), 2)
+    for x in (ref, fr, kf):
+        x.destroy()
+    return out
 
 
 def timed(ctx, fn, steps, warmup):
@@ -322,32 +462,65 @@ def system_leg(ctx, cpu_seconds, n=60):
     return out
 
 
-def cpu_tracking_baseline(insts, params, seconds):
+def cpu_tracking_baseline(insts, params, seconds_per_run, runs=5):
     """The oracle's FP32 restatement of the reference CPU path (timing build,
-    upstream FP32 sums), one thread, on a bounded sample of the same
-    alignments; the per-frame pyramid build is included as on the GPU."""
+    upstream FP32 sums) on the same 64 alignments, per-frame pyramid build
+    included as on the GPU: one thread, and one alignment per thread on every
+    host thread this process may use; each figure the median of `runs` runs
+    of about `seconds_per_run` after a warm-up run."""
+    import threading
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    o = pyoracle.Oracle(0, params=params, fast=True)
-    pairs = []
-    for i, inst in enumerate(insts):
-        kf = o.frame(2 * i, inst["cam"], inst["kf_img"].astype(np.float32))
-        kf.set_depth(inst["idepth"], inst["var"])
-        ref = o.ref(kf)
-        for l in range(1, 5):
-            ref.num_points(l)
-        pairs.append((ref, inst["cam"], inst["fr_img"].astype(np.float32)))
-    n = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        ref, cam, img = pairs[n % len(pairs)]
-        fr = o.frame(10000 + n, cam, img)
-        o.se3_track(ref, fr, synth.IDENTITY7)
-        n += 1
-    dt = time.perf_counter() - t0
-    return dict(value=round(n / dt, 2), unit="frames/s", cores=1, kind="port",
-                sample=f"{n} plain SE3 alignments of the 64 640x480 instances (oracle FP32 restatement of the "
-                       f"reference CPU path, -O3 x86-64-v3, 1 thread, {dt:.1f} s)")
+
+    def workers(nthreads):
+        ws = []
+        for t in range(nthreads):
+            o = pyoracle.Oracle(0, params=params, fast=True)
+            pairs = []
+            for inst in insts[t::nthreads] or insts:
+                kf = o.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
+                kf.set_depth(inst["idepth"], inst["var"])
+                ref = o.ref(kf)
+                for l in range(1, 5):
+                    ref.num_points(l)
+                pairs.append((kf, ref, inst["cam"], inst["fr_img"].astype(np.float32)))
+            ws.append((o, pairs))
+        return ws
+
+    def rate(ws, seconds):
+        counts = [0] * len(ws)
+        stop = time.perf_counter() + seconds
+
+        def work(k):
+            o, pairs = ws[k]
+            n = 0
+            while time.perf_counter() < stop:
+                kf, ref, cam, img = pairs[n % len(pairs)]
+                o.se3_track(ref, o.frame(100000 + n, cam, img), synth.IDENTITY7)
+                n += 1
+            counts[k] = n
+
+        th = [threading.Thread(target=work, args=(k,)) for k in range(len(ws))]
+        t0 = time.perf_counter()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        return sum(counts) / (time.perf_counter() - t0), sum(counts)
+
+    out = {}
+    for nthreads in sorted({1, cpu_threads()}):
+        ws = workers(nthreads)
+        rate(ws, min(1.0, seconds_per_run))
+        rs = [rate(ws, seconds_per_run) for _ in range(runs)]
+        out[nthreads] = (float(np.median([r[0] for r in rs])), sum(r[1] for r in rs))
+    n_max = max(out)
+    return dict(value=round(out[n_max][0], 2), unit="frames/s", cores=n_max, kind="port",
+                single_thread_value=round(out[1][0], 2), cpu_model=cpu_model(),
+                sample=f"plain SE3 alignments of the 64 640x480 instances (oracle FP32 restatement of the "
+                       f"reference CPU path, -O3 x86-64-v3, per-frame pyramid build included): median of {runs} "
+                       f"runs of {seconds_per_run:.1f} s each on {n_max} threads ({out[n_max][1]} alignments) and "
+                       f"on 1 thread ({out[1][1]} alignments)")
 
 
 def cpu_depth_baseline(d, params, seconds):
@@ -359,7 +532,7 @@ def cpu_depth_baseline(d, params, seconds):
     ref = o.frame(1, cam, d["ref_img"].astype(np.float32))
     ref.set_tracking(kf, list(d["ref_pose"]) + [1.0], 0.4)
     out = {}
-    for threads in (1, os.cpu_count() or 1):
+    for threads in sorted({1, cpu_threads()}):
         dm = o.depthmap(cam, threads=threads)
         n, t = 0, 0.0
         while t < seconds / 2:
@@ -372,18 +545,34 @@ def cpu_depth_baseline(d, params, seconds):
     return out
 
 
-def fp64_poses(insts, params, k=4):
+def oracle_poses(insts, params, prec=1):
+    """Every instance's pose from the oracle (prec 1: FP64 throughout)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    o = pyoracle.Oracle(1, params=params)
+    o = pyoracle.Oracle(prec, params=params)
     out = []
-    for i in range(min(k, len(insts))):
-        inst = insts[i]
+    for inst in insts:
         kf = o.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
         kf.set_depth(inst["idepth"], inst["var"])
         fr = o.frame(1, inst["cam"], inst["fr_img"].astype(np.float32))
         out.append(o.se3_track(o.ref(kf), fr, synth.IDENTITY7)[0])
     return out
+
+
+def error_stats(poses, ref):
+    e = np.array([posemath.twist_error(p, q) for p, q in zip(poses, ref)])
+    return {"max": float(e.max()), "median": float(np.median(e)), "p90": float(np.quantile(e, 0.9)),
+            "above_1e-5": int((e > 1e-5).sum()), "instances": int(len(e)), "argmax": int(e.argmax())}
+
+
+def converged_params(its=100):
+    """The LM run to its minimum: the error-ratio stop off (eps 1), steps
+    down to 1e-6 in twist norm, up to `its` iterations per level."""
+    p = tracking_params(its)
+    for l in range(5):
+        p.convergence_eps[l] = 1.0
+        p.step_size_min[l] = 1e-12
+    return p
 
 
 def main():
@@ -395,12 +584,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="plans in their default shape")
     ap.add_argument("--headline-only", action="store_true",
-                    help="run only the batched FP16 leg (for kernel profiling)")
+                    help="run only the batched FP32 leg (for kernel profiling)")
+    ap.add_argument("--precision", choices=("fp32", "fp16", "bf16"), default="fp32",
+                    help="pyramid storage of the --headline-only leg")
     ap.add_argument("--depth-only", action="store_true",
                     help="run only the depth-map update leg (for kernel profiling)")
     ap.add_argument("--ingest-only", action="store_true",
                     help="run only the ingest leg (for kernel profiling)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg (for kernel profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -429,73 +623,88 @@ def main():
     if args.ingest_only:
         print(json.dumps(ingest_leg(ctx, args.steps, args.warmup, 0.0)))
         return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(ctx, args.steps, args.warmup, 0.0)))
+        return
     insts = [synth.tracking_instance(1001 + 17 * i) for i in range(args.batch)]
     B = args.batch
+    prec_of = {"fp32": lsdg.FP32, "fp16": lsdg.FP16, "bf16": lsdg.BF16}
 
-    # headline: batched, FP16 pyramids
-    t16 = Tracking(ctx, insts, lsdg.FP16)
-    t16.make_batch()
-    t = timed(ctx, t16.batch_step, args.steps, args.warmup)
-    batched16_fps = B * args.steps / t
-    ms_per_step = 1e3 * t / args.steps
-    res16 = t16.batch_step()
-    t16.collect()
+    def batched(precision, steps, warmup):
+        """The 64-alignment batch in the shape the tuner picks (tuned outside
+        the timed region): frames/s, the plan's shape, the poses."""
+        tr = Tracking(ctx, insts, precision)
+        tune = None if args.no_tune else tr.tune(runs=5)
+        tr.make_batch()
+        t = timed(ctx, tr.batch_step, steps, warmup)
+        tr.batch_step()
+        return tr, B * steps / t, 1e3 * t / steps, tune
+
+    # headline: batched, FP32 pyramids (or the --precision of --headline-only)
+    head_prec = prec_of[args.precision] if args.headline_only else lsdg.FP32
+    t32, batched32_fps, ms_per_step, tune32 = batched(head_prec, args.steps, args.warmup)
+    poses32 = list(t32.collect())
     # HIP events around every launch, over a profiled repeat of the timed steps
-    achieved, ks = kernel_roofline(ctx, t16.batch_step, lsdg.KERNEL_BATCH_EVAL, reps=min(args.steps, 20))
-    solve = ctx.kernel_stats(lsdg.KERNEL_BATCH_SOLVE)
-    # the dominant kernel: the cooperative whole-LM kernel when the plan uses it
+    achieved, ks = kernel_roofline(ctx, t32.batch_step, lsdg.KERNEL_BATCH_EVAL, reps=min(args.steps, 20))
     dom = ctx.kernel_stats(lsdg.KERNEL_BATCH_COOP)
+    shape = t32.batch.config()
     dom_name = "k_batch_coop"
-    dom_desc = ("k_batch_coop (every job's whole LM loop, 4 blocks per job: fused residual + weights + LGS, "
-                "block reductions, row exchange, LM step; FP16 taps)")
+    dom_desc = (f"k_batch_coop (every job's whole LM loop, {shape['coop_k']} blocks of {shape['coop_block']} threads "
+                "per job: fused residual + weights + LGS, block reductions, row exchange, LM step; FP32 taps)")
     if dom["launches"] == 0:
         dom, dom_name = ks, "k_batch_eval"
-        dom_desc = "k_batch_eval (fused residual + weights + LGS over all jobs' tiles, FP16 taps)"
+        dom_desc = "k_batch_eval (fused residual + weights + LGS over all jobs' tiles, FP32 taps)"
     dom_gbs = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
-    jobk = ctx.kernel_stats(lsdg.KERNEL_BATCH_JOB)
-    coopk = ctx.kernel_stats(lsdg.KERNEL_BATCH_COOP)
-    poses16 = list(t16.poses)
     if args.headline_only:
-        print(json.dumps({"value": round(batched16_fps, 2), "unit": "frames/s", "ms_per_step": round(ms_per_step, 3),
-                          "eval_avg_launch_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 3),
-                          "eval_launches": ks["launches"], "eval_gbs": round(achieved, 2),
-                          "solve_avg_us": round(1e3 * solve["ms"] / max(solve["launches"], 1), 3),
-                          "job_kernel_us": round(1e3 * jobk["ms"] / max(jobk["launches"], 1), 3),
-                          "job_kernel_gbs": round(jobk["bytes"] / (jobk["ms"] * 1e-3) / 1e9, 2) if jobk["ms"] else 0,
-                          "coop_kernel_us": round(1e3 * coopk["ms"] / max(coopk["launches"], 1), 3),
-                          "coop_kernel_gbs": round(coopk["bytes"] / (coopk["ms"] * 1e-3) / 1e9, 2) if coopk["ms"] else 0}))
+        print(json.dumps({"value": round(batched32_fps, 2), "unit": "frames/s", "precision": args.precision,
+                          "ms_per_step": round(ms_per_step, 3), "shape": shape, "tune": tune32,
+                          "dominant_kernel": dom_name, "dominant_avg_launch_us":
+                          round(1e3 * dom["ms"] / max(dom["launches"], 1), 3), "dominant_gbs": round(dom_gbs, 2),
+                          "dominant_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1), 1)}))
         return
+    traffic, traffic_src = measured_traffic(f"{dom_name}_se3_640x480_x64_fp32")
 
-    # batched, FP32 pyramids (FP64 sums: the plain path's LM path)
-    t32 = Tracking(ctx, insts, lsdg.FP32)
-    t32.make_batch()
-    t = timed(ctx, t32.batch_step, args.steps, args.warmup)
-    batched32_fps = B * args.steps / t
-    t32.batch_step()
-    poses32 = list(t32.collect())
+    # reduced-precision pyramids beside it
+    t16, batched16_fps, _, tune16 = batched(lsdg.FP16, args.steps, args.warmup)
+    poses16 = list(t16.collect())
+    tb16, batchedbf_fps, _, tunebf = batched(lsdg.BF16, args.steps, args.warmup)
+    posesbf = list(tb16.collect())
 
     out_plain = {}
     if not args.no_plain:
         t = timed(ctx, t32.plain_step, max(1, args.steps // 2), 1)
         plain_fps = B * max(1, args.steps // 2) / t
         plain_ach, plain_ks = kernel_roofline(ctx, t32.plain_step, lsdg.KERNEL_EVAL)
-        diff32 = max(posemath.twist_error(a, b) for a, b in zip(poses32, t32.poses))
-        diff16 = max(posemath.twist_error(a, b) for a, b in zip(poses16, t32.poses))
-        out_plain = {"plain_frames_per_s": round(plain_fps, 2),
-                     "batched_vs_plain_speedup": round(batched16_fps / plain_fps, 1),
+        out_plain = {"plain_fp32_frames_per_s": round(plain_fps, 2),
+                     "batched_fp32_vs_plain_speedup": round(batched32_fps / plain_fps, 1),
+                     "batched_fp16_vs_plain_speedup": round(batched16_fps / plain_fps, 1),
                      "plain_eval_kernel_gbs": round(plain_ach, 2),
                      "plain_eval_avg_launch_us": round(1e3 * plain_ks["ms"] / max(plain_ks["launches"], 1), 3),
-                     "max_twist_batched_fp32_vs_plain": diff32,
-                     "max_twist_batched_fp16_vs_plain": diff16}
+                     "max_twist_batched_fp32_vs_plain": max(posemath.twist_error(a, b)
+                                                            for a, b in zip(poses32, t32.poses))}
 
-    # mixed-resolution batch (320x240 / 640x480 / 1280x960, up to 5 cm / 3 deg)
+    # converged variant (error-ratio stop off): FP32 and FP16 batches
+    conv = {}
+    ctx.set_params(converged_params())
+    try:
+        for name, prec in (("fp32", lsdg.FP32), ("fp16", lsdg.FP16)):
+            tr = Tracking(ctx, insts, prec)
+            tr.make_batch()
+            tr.batch_step()
+            conv[name] = list(tr.collect())
+    finally:
+        ctx.set_params(params)
+
+    # mixed-resolution batch (320x240 / 640x480 / 1280x960, up to 5 cm / 3 deg), FP32
     mixed = synth.batch_instances(64, seed=1003)
-    tm = Tracking(ctx, mixed, lsdg.FP16)
+    tm = Tracking(ctx, mixed, lsdg.FP32)
+    tune_mixed = None if args.no_tune else tm.tune(runs=5)
     tm.make_batch()
     t = timed(ctx, tm.batch_step, args.steps, args.warmup)
     mixed_aps = 64 * args.steps / t
 
-    sim3 = sim3_leg(ctx, args.steps, args.warmup)
+    sim3 = sim3_leg(ctx, args.steps, args.warmup, check_fp64=not args.no_cpu)
+    adjoint = adjoint_leg(ctx, max(args.steps, 20), args.warmup, 0.0 if args.no_cpu else 2.0)
 
     depth, dinst = depth_leg(ctx, args.steps, args.warmup)
     ctx.set_params(params)
@@ -506,7 +715,7 @@ def main():
 
     out = {
         "metric": "SE3 track frames/s + depth updates/s @640x480, batched vs plain, 1 GPU",
-        "value": round(batched16_fps, 2),
+        "value": round(batched32_fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -519,12 +728,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched SE3 alignment: {B} x 640x480 frames per step (pyramid rebuild + LM "
                                "levels 4..1, <=20 iterations/level), ~25% semi-dense points, 2 cm / 1 deg, "
-                               "FP16 (I, gx, gy) pyramids, FP32 residuals, FP64 sums, FP64 "
-                               "merge and LDLT", "batch": B, "resolution": "640x480", "storage": "fp16"},
-        "batched_fp32_frames_per_s": round(batched32_fps, 2),
+                               "FP32 (I, gx, gy) pyramids, FP32 residuals, FP64 sums, FP64 merge and LDLT",
+                   "batch": B, "resolution": "640x480", "storage": "fp32"},
+        "plan_shape": shape,
+        "tune_fp32": tune32,
+        "batched_fp16_frames_per_s": round(batched16_fps, 2),
+        "batched_bf16_frames_per_s": round(batchedbf_fps, 2),
+        "tune_fp16": tune16,
+        "tune_bf16": tunebf,
         **out_plain,
         "mixed_resolution_batch_alignments_per_s": round(mixed_aps, 2),
+        "tune_mixed_resolution": tune_mixed,
         "sim3_1920x1080": sim3,
+        "adjoint_1280x960": adjoint,
         "depth_updates_per_s": round(depth["updates_per_s"], 2),
         "depth_update_ms": round(depth["ms_per_update"], 4),
         "depth_device_ms_per_update": round(depth["device_ms_per_update"], 4),
@@ -534,25 +750,26 @@ def main():
         "constraint_search_640x480": constraint,
         "slam_system_640x480": system,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES[dom_name],
-                     "kernel": dom_desc,
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_src, "kernel": dom_desc,
                      "avg_launch_us": round(1e3 * dom["ms"] / max(dom["launches"], 1), 3),
                      "launches": dom["launches"],
                      "bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1), 1)},
     }
     if not args.no_cpu:
-        cb = cpu_tracking_baseline(insts, params, args.cpu_seconds * 0.6)
-        out["cpu_baseline"] = cb
+        out["cpu_baseline"] = cpu_tracking_baseline(insts, params, args.cpu_seconds * 0.1)
         cd = cpu_depth_baseline(dinst, tracking_params(20, crop=100), args.cpu_seconds * 0.4)
         out["cpu_depth_updates_per_s"] = {str(k): round(v, 2) for k, v in cd.items()}
-        ref64 = fp64_poses(insts, params)
-        errs16 = [posemath.twist_error(poses16[i], q) for i, q in enumerate(ref64)]
-        errs32 = [posemath.twist_error(poses32[i], q) for i, q in enumerate(ref64)]
-        # FP32 batched poses equal the plain path's to ~1e-16: the plain path's own error
-        out["pose_error_vs_fp64_oracle"] = {"batched_fp16_max": max(errs16), "batched_fp16_median":
-                                            float(np.median(errs16)), "fp32_plain_max": max(errs32),
-                                            "fp32_plain_median": float(np.median(errs32)),
-                                            "instances": len(errs16)}
+        ref64 = oracle_poses(insts, params)
+        ref64c = oracle_poses(insts, converged_params())
+        out["pose_error_vs_fp64_oracle"] = {
+            "fp32_batched_and_plain": error_stats(poses32, ref64),
+            "fp16_batched": error_stats(poses16, ref64),
+            "bf16_batched": error_stats(posesbf, ref64),
+            "converged_fp32": error_stats(conv["fp32"], ref64c),
+            "converged_fp16": error_stats(conv["fp16"], ref64c),
+            "note": "twist norm of every instance's frameToRef against the FP64 oracle's; the FP32 batch equals the "
+                    "plain path to 1e-16 and the oracle's FP32 twin to 1e-9"}
     print(json.dumps(out))
 
 

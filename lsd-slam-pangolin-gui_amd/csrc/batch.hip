@@ -589,20 +589,16 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
 // LDS copy of the control block, so no result is broadcast; part 0 alone
 // counts finished jobs and writes the control block back.  Rows are double
 // buffered by iteration parity: a block can be at most one iteration ahead of
-// the slowest block of its job.  The wait is bounded: if the K blocks of a job
-// cannot all be resident (the grid is sized from the occupancy, so this does
-// not happen on an idle device), the blocks stop and set *remaining negative,
-// which the host reports as an error.
-#ifndef LSDG_COOP_BS
-#define LSDG_COOP_BS 512
-#endif
-constexpr int COOP_BS = LSDG_COOP_BS;
-constexpr unsigned COOP_SPIN_MAX = 1u << 22;
+// the slowest block of its job.  The wait is bounded (a.spinMax polls): if the
+// K blocks of a job cannot all be resident (the grid is sized from the
+// occupancy, so this does not happen on an idle device), the blocks stop and
+// set *remaining negative, which the host reports as an error.  The block
+// size BS (256 or 512 threads) is one of the shapes the plan's tuner picks.
 #ifndef LSDG_SPIN_SLEEP
 #define LSDG_SPIN_SLEEP 2
 #endif
 
-template <int KIND, int PREC>
+template <int KIND, int PREC, int COOP_BS>
 __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
   constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
   constexpr int WAVES = COOP_BS / 64;
@@ -613,7 +609,19 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
   __shared__ double lmsc[LM_SCRATCH];
   __shared__ int arrived;
   const int K = a.coopK;
-  const int j = blockIdx.x / K, part = blockIdx.x % K;
+  // Blocks are dispatched round-robin over the device's XCDs.  With
+  // a.xcds > 1, block b belongs to group b % xcds, and the K blocks of job j
+  // all fall in group j % xcds: a job's points and taps stay in one XCD's L2.
+  int j, part;
+  if (a.xcds > 1) {
+    const int slot = blockIdx.x / a.xcds;
+    j = (slot / K) * a.xcds + (int)(blockIdx.x % a.xcds);
+    part = slot % K;
+  } else {
+    j = blockIdx.x / K;
+    part = blockIdx.x % K;
+  }
+  if (j >= a.njobs) return;  // padding of the last group (the whole block leaves)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int* remaining = part == 0 ? a.remaining : nullptr;
   double* rows = a.coopRows + (size_t)j * 2 * K * NS;
@@ -665,12 +673,15 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
         if (tid == 0) {
           wait_own_stores();  // the row stores above are agent-scope atomics
           __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          // at most spinMax polls after the first look; arrived only if the
+          // count was reached
           const unsigned target = (unsigned)K * (gen + 1);
-          unsigned spins = 0;
-          while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
-                 ++spins < COOP_SPIN_MAX)
+          bool ok = __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+          for (unsigned spins = 0; !ok && spins < a.spinMax; ++spins) {
             __builtin_amdgcn_s_sleep(LSDG_SPIN_SLEEP);
-          arrived = spins < COOP_SPIN_MAX;
+            ok = __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+          }
+          arrived = ok;
         }
       }
       __syncthreads();
@@ -718,28 +729,43 @@ hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hip
   return launch_k(k, dim3(a.njobs), dim3(BATCH_JOB_BS), s, t, a);
 }
 
-static void (*coop_kernel(int kind, int prec))(BatchArgs) {
+template <int BS>
+static void (*coop_kernel_bs(int kind, int prec))(BatchArgs) {
   if (kind == LSDG_JOB_SIM3)
-    return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_FP16>
-           : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_BF16>
-                               : k_batch_coop<LSDG_JOB_SIM3, LSDG_FP32>;
-  return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_FP16>
-         : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_BF16>
-                             : k_batch_coop<LSDG_JOB_SE3, LSDG_FP32>;
+    return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_FP16, BS>
+           : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_BF16, BS>
+                               : k_batch_coop<LSDG_JOB_SIM3, LSDG_FP32, BS>;
+  return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_FP16, BS>
+         : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_BF16, BS>
+                             : k_batch_coop<LSDG_JOB_SE3, LSDG_FP32, BS>;
 }
 
-hipError_t batch_coop_capacity(int kind, int prec, int* blocks) {
+static void (*coop_kernel(int kind, int prec, int bs))(BatchArgs) {
+  return bs == 256 ? coop_kernel_bs<256>(kind, prec) : coop_kernel_bs<512>(kind, prec);
+}
+
+bool batch_coop_block_size_ok(int bs) { return bs == 256 || bs == 512; }
+
+int batch_coop_grid(int njobs, int K, int xcds) {
+  if (xcds <= 1) return njobs * K;
+  return ((njobs + xcds - 1) / xcds) * xcds * K;
+}
+
+hipError_t batch_coop_capacity(int kind, int prec, int bs, int* blocks) {
+  *blocks = 0;
+  if (!batch_coop_block_size_ok(bs)) return hipErrorInvalidValue;
   int dev = 0, perCU = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coop_kernel(kind, prec), COOP_BS, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coop_kernel(kind, prec, bs), bs, 0);
   *blocks = e == hipSuccess ? perCU * cus : 0;
   return e;
 }
 
-hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, hipStream_t s, const KTimer* t) {
-  return launch_k(coop_kernel(kind, prec), dim3(a.njobs * a.coopK), dim3(COOP_BS), s, t, a);
+hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, int bs, hipStream_t s, const KTimer* t) {
+  if (!batch_coop_block_size_ok(bs)) return hipErrorInvalidValue;
+  return launch_k(coop_kernel(kind, prec, bs), dim3(batch_coop_grid(a.njobs, a.coopK, a.xcds)), dim3(bs), s, t, a);
 }
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {

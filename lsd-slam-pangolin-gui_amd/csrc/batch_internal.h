@@ -1,6 +1,7 @@
 // Device-side layout of a batched tracking plan (jobs, per-job LM control
 // blocks) shared by batch.hip and capi_batch.cpp.
 #pragma once
+#include "../../include/lsdg_tune.h"
 #include "lsdg_internal.h"
 
 namespace lsdg {
@@ -66,15 +67,26 @@ struct BatchArgs {
   double* coopRows;   // k_batch_coop: [njobs][2][coopK] rows of NS doubles
   int coopK;          // k_batch_coop: blocks per job
   int soloMax;        // k_batch_coop: levels of at most this many points are not split
+  unsigned spinMax;   // k_batch_coop: polls of the row-exchange wait before the blocks give up
+  int xcds;           // k_batch_coop: > 1: block b belongs to dispatch group b % xcds, which holds
+                      // every block of jobs j with j % xcds == b % xcds (grid padded to a multiple)
+  int affineStats;    // accumulate the affine-brightness sums (only read when estimation is on)
   lsdg_params P;
   TrackKParams kp;
 };
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s);
-// Cooperative whole-LM kernel (a.coopK blocks per job, all levels) and the
-// number of its blocks the device holds at once.
-hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, hipStream_t s, const KTimer* t = nullptr);
-hipError_t batch_coop_capacity(int kind, int prec, int* blocks);
+// Cooperative whole-LM kernel (a.coopK blocks of `bs` threads per job, all
+// levels; bs is 256 or 512) and the number of its blocks the device holds at
+// once.
+constexpr unsigned COOP_SPIN_MAX = 1u << 22;
+bool batch_coop_block_size_ok(int bs);
+// Blocks of a cooperative launch: njobs * K, or with xcds > 1 the jobs
+// padded to a multiple of xcds (the padding blocks exit at once).
+int batch_coop_grid(int njobs, int K, int xcds);
+hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, int bs, hipStream_t s,
+                             const KTimer* t = nullptr);
+hipError_t batch_coop_capacity(int kind, int prec, int bs, int* blocks);
 // Launches with an optional kernel timer (see launch_k).
 hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s,
                             const KTimer* t = nullptr);
@@ -82,5 +94,12 @@ hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, in
                              const KTimer* t = nullptr);
 hipError_t launch_batch_solve(const BatchArgs& a, int kind, int parity, hipStream_t s, const KTimer* t = nullptr);
 hipError_t launch_batch_step(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s);
+
+// Plan shapes (capi_batch.cpp) and the tuning table (capi_tune.cpp).
+lsdg_batch_config batch_default_shape(int n);
+lsdg_status batch_create_shaped(lsdg_ctx c, const lsdg_job* jobs, int32_t n, const lsdg_batch_config& shape,
+                                lsdg_batch* out);
+// The table's entry for key into *out; false when there is none.
+bool tune_table_lookup(const char* key, lsdg_batch_config* out);
 
 }  // namespace lsdg

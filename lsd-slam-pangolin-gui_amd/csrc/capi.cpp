@@ -579,6 +579,7 @@ TrackKParams lsdg::track_kparams(const lsdg_params& P, bool sim3) {
   k.aff_a = 1.0f;
   k.aff_b = 0.0f;
   k.use_esm = P.use_esm_tracking;
+  k.aff_stats = 1;  // the plain evaluation reports them (lsdg_eval)
   k.roll[0] = 1.f; k.roll[1] = 0.f; k.roll[2] = 0.f; k.roll[3] = 1.f;
   return k;
 }

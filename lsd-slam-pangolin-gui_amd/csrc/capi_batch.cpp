@@ -46,7 +46,10 @@ struct lsdg_batch_s {
   // Cooperative schedule: coopK blocks per job run every level's LM loop in
   // one launch (0: the whole-job kernel and multi-block steps instead).
   int coopK = 0;
+  int coopBS = 512;  // threads per cooperative block (256 or 512)
+  int xcds = 1;      // dispatch groups a job's cooperative blocks are kept in (1: job order)
   int soloMax = 2048;
+  unsigned spinMax = COOP_SPIN_MAX;
   double* d_coop = nullptr;
   std::vector<lsdg_job> jobs;
   std::vector<int> depthVersion;
@@ -168,8 +171,13 @@ static BatchArgs batch_args(lsdg_batch b, int c) {
   a.coopRows = b->d_coop + (size_t)j0 * 2 * std::max(b->coopK, 1) * (b->kind == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3);
   a.coopK = b->coopK;
   a.soloMax = b->soloMax;
+  a.spinMax = b->spinMax;
   a.P = b->ctx->P;
   a.kp = track_kparams(b->ctx->P, b->kind == LSDG_JOB_SIM3);
+  // the batched LM reads the affine sums only when it estimates the affine
+  // brightness; the results report the estimate, not the sums
+  a.kp.aff_stats = b->ctx->P.use_affine_lightning_estimation;
+  a.xcds = b->coopK > 0 ? b->xcds : 1;
   return a;
 }
 
@@ -190,7 +198,7 @@ static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec
     BatchArgs a = batch_args(b, k);
     if (first) {
       e = launch_batch_init(a, st);
-      if (e == hipSuccess && b->coopK > 0) e = launch_batch_coop(a, b->kind, b->prec, st);
+      if (e == hipSuccess && b->coopK > 0) e = launch_batch_coop(a, b->kind, b->prec, b->coopBS, st);
       else if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, st);
     }
     for (int s = 0; s < steps && e == hipSuccess; ++s)
@@ -232,12 +240,11 @@ static lsdg_status batch_graphs(lsdg_batch b) {
   return capture(b, false, &b->g_cont, &b->cont);
 }
 
-extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32_t n, lsdg_batch* out) {
-  if (!c || !jobs || !out || n < 1) return LSDG_EINVAL;
+static lsdg_status validate_jobs(lsdg_ctx c, const lsdg_job* jobs, int32_t n) {
+  if (!c || !jobs || n < 1) return LSDG_EINVAL;
   if (n > MAX_BATCH_JOBS) return fail(c, LSDG_EINVAL, "too many jobs in one batch");
-  const lsdg_params& P = c->P;
-  int kind = jobs[0].kind;
-  int prec = jobs[0].frame ? jobs[0].frame->precision : LSDG_FP32;
+  const int kind = jobs[0].kind;
+  const int prec = jobs[0].frame ? jobs[0].frame->precision : LSDG_FP32;
   for (int j = 0; j < n; ++j) {
     const lsdg_job& J = jobs[j];
     if (!J.ref || !J.frame) return fail(c, LSDG_EINVAL, "job without reference or frame");
@@ -255,6 +262,50 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
       return fail(c, LSDG_EINVAL, "unknown job kind");
     }
   }
+  return LSDG_OK;
+}
+
+// The default shape: the cooperative kernel with 512-thread blocks, as many
+// blocks per job as the device holds at once (at most 16), levels above 2048
+// points split; when the device cannot hold two blocks per job, the
+// whole-job kernel for levels up to 6144 points and multi-block steps above,
+// in two chains from 16 jobs on.
+lsdg_batch_config lsdg::batch_default_shape(int n) {
+  lsdg_batch_config s{};
+  s.schedule = LSDG_SCHEDULE_COOP;
+  s.coop_block = 512;
+  s.coop_k = 16;
+  s.solo_max = 2048;
+  s.job_max_points = 12 * BATCH_JOB_BS;
+  s.chains = n >= 16 ? 2 : 1;
+  s.points_per_thread = 4;
+  s.xcd_groups = 1;
+  return s;
+}
+
+// Environment knobs that pin a schedule (tests and measurements); each
+// overrides the shape it names.
+static void env_shape(lsdg_batch_config& s, int n) {
+  if (const char* e = std::getenv("LSDG_BATCH_COOP")) s.schedule = std::atoi(e) ? LSDG_SCHEDULE_COOP : LSDG_SCHEDULE_STEPS;
+  if (const char* e = std::getenv("LSDG_BATCH_COOP_K")) s.coop_k = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("LSDG_BATCH_COOP_BS")) s.coop_block = std::atoi(e);
+  if (const char* e = std::getenv("LSDG_BATCH_SOLO")) s.solo_max = std::max(0, std::atoi(e));
+  if (const char* e = std::getenv("LSDG_BATCH_PPT")) s.points_per_thread = std::max(1, std::min(64, std::atoi(e)));
+  if (const char* e = std::getenv("LSDG_BATCH_JOB_POINTS")) s.job_max_points = std::max(0, std::atoi(e));
+  if (const char* e = std::getenv("LSDG_BATCH_CHAINS")) s.chains = std::max(1, std::min(8, std::min(n, std::atoi(e))));
+  if (const char* e = std::getenv("LSDG_BATCH_XCDS")) s.xcd_groups = std::max(1, std::min(64, std::atoi(e)));
+}
+
+lsdg_status lsdg::batch_create_shaped(lsdg_ctx c, const lsdg_job* jobs, int32_t n, const lsdg_batch_config& shape,
+                                      lsdg_batch* out) {
+  if (!out) return LSDG_EINVAL;
+  lsdg_status vst = validate_jobs(c, jobs, n);
+  if (vst != LSDG_OK) return vst;
+  if (shape.schedule == LSDG_SCHEDULE_COOP && !batch_coop_block_size_ok(shape.coop_block))
+    return fail(c, LSDG_EINVAL, "cooperative block size must be 256 or 512");
+  const lsdg_params& P = c->P;
+  const int kind = jobs[0].kind;
+  const int prec = jobs[0].frame->precision;
   lsdg_batch b = new (std::nothrow) lsdg_batch_s();
   if (!b) return LSDG_ENOMEM;
   b->ctx = c;
@@ -270,28 +321,32 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   // sums in FP64.
   b->acc = BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_ACC")) b->acc = std::atoi(e) ? BATCH_ACC_F32 : BATCH_ACC_F64;
-  if (const char* e = std::getenv("LSDG_BATCH_PPT")) b->ppt = std::max(1, std::min(64, std::atoi(e)));
   if (const char* e = std::getenv("LSDG_BATCH_STEPS")) b->steps = std::max(0, std::min(512, std::atoi(e) & ~1));
-  if (const char* e = std::getenv("LSDG_BATCH_JOB_POINTS")) b->jobMaxPoints = std::max(0, std::atoi(e));
-  b->chains = n >= 16 ? 2 : 1;
-  if (const char* e = std::getenv("LSDG_BATCH_CHAINS")) b->chains = std::max(1, std::min(8, std::min(n, std::atoi(e))));
-  // The cooperative schedule when the device holds at least two blocks per
-  // job at once (LSDG_BATCH_COOP=0 turns it off, LSDG_BATCH_COOP_K caps the
-  // blocks per job, LSDG_BATCH_SOLO sets the largest level not split).
-  {
-    bool coop = true;
-    if (const char* e = std::getenv("LSDG_BATCH_COOP")) coop = std::atoi(e) != 0;
+  // polls of the cooperative row-exchange wait before a job's blocks give up
+  // (LSDG_EDEVICE); a small value makes the give-up path run on purpose
+  if (const char* e = std::getenv("LSDG_BATCH_SPIN_MAX")) b->spinMax = (unsigned)std::max(0, std::atoi(e));
+  b->ppt = std::max(1, std::min(64, shape.points_per_thread));
+  b->jobMaxPoints = std::max(0, shape.job_max_points);
+  b->chains = std::max(1, std::min(8, std::min(n, shape.chains)));
+  b->soloMax = std::max(0, shape.solo_max);
+  // The cooperative schedule needs the K blocks of every job resident
+  // together: K is capped by the blocks the device holds at once; below two
+  // blocks per job the plan falls back to the whole-job kernel and steps.
+  if (shape.schedule == LSDG_SCHEDULE_COOP) {
     int cap = 0;
-    if (coop && batch_coop_capacity(kind, prec, &cap) == hipSuccess) {
-      int K = std::min(16, cap / n);
-      if (const char* e = std::getenv("LSDG_BATCH_COOP_K")) K = std::min(K, std::max(1, std::atoi(e)));
+    if (batch_coop_capacity(kind, prec, shape.coop_block, &cap) == hipSuccess) {
+      // with dispatch groups the jobs are padded to a multiple of the groups
+      const int xcds = std::max(1, std::min(64, shape.xcd_groups));
+      const int nPad = xcds > 1 ? ((n + xcds - 1) / xcds) * xcds : n;
+      const int K = std::min(std::max(1, shape.coop_k), std::min(16, cap / nPad));
       if (K >= 2) {
         b->coopK = K;
+        b->coopBS = shape.coop_block;
+        b->xcds = xcds;
         b->chains = 1;
         b->jobMaxPoints = 1 << 30;
       }
     }
-    if (const char* e = std::getenv("LSDG_BATCH_SOLO")) b->soloMax = std::max(0, std::atoi(e));
   }
   b->grid = 1024 / b->chains;  // evaluation blocks per chain
   if (const char* e = std::getenv("LSDG_BATCH_GRID")) b->grid = std::max(1, std::min(65536, std::atoi(e)));
@@ -391,6 +446,40 @@ extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32
   return LSDG_OK;
 }
 
+extern "C" lsdg_status lsdg_batch_create(lsdg_ctx c, const lsdg_job* jobs, int32_t n, lsdg_batch* out) {
+  if (!c || !jobs || !out || n < 1) return LSDG_EINVAL;
+  lsdg_status st = validate_jobs(c, jobs, n);
+  if (st != LSDG_OK) return st;
+  // the tuned shape of this class of batch when the table has one
+  lsdg_batch_config shape = batch_default_shape(n);
+  char key[LSDG_TUNE_KEY_BYTES];
+  if (lsdg_batch_tune_key(c, jobs, n, key) == LSDG_OK) tune_table_lookup(key, &shape);
+  env_shape(shape, n);
+  return batch_create_shaped(c, jobs, n, shape, out);
+}
+
+extern "C" lsdg_status lsdg_batch_create_with_config(lsdg_ctx c, const lsdg_job* jobs, int32_t n,
+                                                     const lsdg_batch_config* config, lsdg_batch* out) {
+  if (!c || !jobs || !config || !out || n < 1) return LSDG_EINVAL;
+  if (config->schedule != LSDG_SCHEDULE_COOP && config->schedule != LSDG_SCHEDULE_STEPS)
+    return fail(c, LSDG_EINVAL, "unknown schedule");
+  return batch_create_shaped(c, jobs, n, *config, out);
+}
+
+extern "C" lsdg_status lsdg_batch_get_config(lsdg_batch b, lsdg_batch_config* out) {
+  if (!b || !out) return LSDG_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  out->schedule = b->coopK > 0 ? LSDG_SCHEDULE_COOP : LSDG_SCHEDULE_STEPS;
+  out->coop_block = b->coopBS;
+  out->coop_k = b->coopK;
+  out->solo_max = b->soloMax;
+  out->job_max_points = b->jobMaxPoints;
+  out->chains = b->chains;
+  out->points_per_thread = b->ppt;
+  out->xcd_groups = b->xcds;
+  return LSDG_OK;
+}
+
 extern "C" lsdg_status lsdg_batch_destroy(lsdg_batch b) {
   if (!b) return LSDG_EINVAL;
   (void)hipStreamSynchronize(b->ctx->stream);
@@ -472,7 +561,7 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
     CK(c, launch_batch_init(a, c->stream));
     if (b->coopK > 0) {
       KTimer kt = prof_timer(c);
-      CK(c, launch_batch_coop(a, b->kind, b->prec, c->stream, &kt));
+      CK(c, launch_batch_coop(a, b->kind, b->prec, b->coopBS, c->stream, &kt));
       prof_timer_done(c, LSDG_KERNEL_BATCH_COOP, kt, 0.0);
     } else if (b->jobMaxPoints > 0) {
       KTimer kt = prof_timer(c);

@@ -67,6 +67,7 @@ struct TrackKParams {
   float max_diff_grad_mult;
   float aff_a, aff_b;
   int use_esm;
+  int aff_stats;      // accumulate the affine-brightness sums (sxx, syy, sx, sy, sw)
   float roll[4];      // Sim3 ESM roll (xRoll0, xRoll1, yRoll0, yRoll1)
 };
 

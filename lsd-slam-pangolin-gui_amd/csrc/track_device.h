@@ -300,12 +300,14 @@ __device__ inline void se3_finish(const float4& p, float pvar, int pidx, const P
   float c1 = kp.aff_a * p.w + kp.aff_b;
   float c2 = ri.I;
   float r = c1 - c2;
-  float wgt = fabsf(r) < 5.0f ? 1.0f : fdiv<PREC>(5.0f, fabsf(r));
-  acc.add(S3_SXX, c1 * c1 * wgt);
-  acc.add(S3_SYY, c2 * c2 * wgt);
-  acc.add(S3_SX, c1 * wgt);
-  acc.add(S3_SY, c2 * wgt);
-  acc.add(S3_SW, wgt);
+  if (kp.aff_stats) {  // affine-brightness statistics (read only when that estimation is on)
+    float wgt = fabsf(r) < 5.0f ? 1.0f : fdiv<PREC>(5.0f, fabsf(r));
+    acc.add(S3_SXX, c1 * c1 * wgt);
+    acc.add(S3_SYY, c2 * c2 * wgt);
+    acc.add(S3_SX, c1 * wgt);
+    acc.add(S3_SY, c2 * wgt);
+    acc.add(S3_SW, wgt);
+  }
   bool good = fdiv<PREC>(r * r, kp.max_diff_constant + kp.max_diff_grad_mult * (ri.gx * ri.gx + ri.gy * ri.gy)) < 1.0f;
   if (isGood) isGood[pidx] = good ? 1 : 0;
   float gx = g.fx * ri.gx, gy = g.fy * ri.gy;
@@ -401,12 +403,14 @@ __device__ inline void sim3_finish(const float4& p, float pvar, float2 rg, const
   float c1 = kp.aff_a * p.w + kp.aff_b;
   float c2 = ri.I;
   float rp = c1 - c2;
-  float wgt = fabsf(rp) < 2.0f ? 1.0f : fdiv<PREC>(2.0f, fabsf(rp));
-  acc.add(SM_SXX, c1 * c1 * wgt);
-  acc.add(SM_SYY, c2 * c2 * wgt);
-  acc.add(SM_SX, c1 * wgt);
-  acc.add(SM_SY, c2 * wgt);
-  acc.add(SM_SW, wgt);
+  if (kp.aff_stats) {
+    float wgt = fabsf(rp) < 2.0f ? 1.0f : fdiv<PREC>(2.0f, fabsf(rp));
+    acc.add(SM_SXX, c1 * c1 * wgt);
+    acc.add(SM_SYY, c2 * c2 * wgt);
+    acc.add(SM_SX, c1 * wgt);
+    acc.add(SM_SY, c2 * wgt);
+    acc.add(SM_SW, wgt);
+  }
   float refId = frcp<PREC>(Wz);
   float d = frcp<PREC>(p.z);
   float rd, vard;

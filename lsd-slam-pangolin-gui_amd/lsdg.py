@@ -129,6 +129,23 @@ class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
 
 
+SCHEDULE_COOP, SCHEDULE_STEPS = 0, 1
+_CONFIG_FIELDS = ["schedule", "coop_block", "coop_k", "solo_max", "job_max_points", "chains", "points_per_thread",
+                  "xcd_groups"]
+
+
+class BatchConfig(C.Structure):
+    """lsdg_batch_config (include/lsdg_tune.h): the shape of a batch plan."""
+    _fields_ = [(n, C.c_int32) for n in _CONFIG_FIELDS]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n in _CONFIG_FIELDS}
+
+    @staticmethod
+    def of(d):
+        return BatchConfig(*[int(d[n]) for n in _CONFIG_FIELDS])
+
+
 class Job(C.Structure):
     _fields_ = [("ref", C.c_void_p), ("frame", C.c_void_p), ("frame_to_ref_init", C.c_double * 8),
                 ("kind", C.c_int32), ("start_level", C.c_int32), ("final_level", C.c_int32),
@@ -265,6 +282,16 @@ def load():
         "lsdg_energy_grad": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
         "lsdg_energy_grad_sim3": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
         "lsdg_energy_grad_noalloc": (i32, [vp, vp, vp, i32, dp, dp, dp, fp, i32]),
+        "lsdg_batch_tune_key": (i32, [vp, vp, i32, C.c_char_p]),
+        "lsdg_batch_create_with_config": (i32, [vp, vp, i32, C.POINTER(BatchConfig), C.POINTER(vp)]),
+        "lsdg_batch_get_config": (i32, [vp, C.POINTER(BatchConfig)]),
+        "lsdg_batch_tune": (i32, [vp, vp, i32, i32, C.POINTER(BatchConfig), dp, dp]),
+        "lsdg_tune_table_put": (i32, [C.c_char_p, C.POINTER(BatchConfig)]),
+        "lsdg_tune_table_get": (i32, [C.c_char_p, C.POINTER(BatchConfig)]),
+        "lsdg_tune_table_clear": (i32, []),
+        "lsdg_tune_table_size": (i32, []),
+        "lsdg_tune_table_save": (i32, [C.c_char_p]),
+        "lsdg_tune_table_load": (i32, [C.c_char_p]),
         "lsdg_depthmap_create": (i32, [vp, C.POINTER(Camera), C.POINTER(vp)]),
         "lsdg_depthmap_destroy": (i32, [vp]),
         "lsdg_depth_reset": (i32, [vp]),
@@ -491,6 +518,30 @@ class Context:
         cc = Camera.of(cam)
         self.check(self.L.lsdg_depthmap_create(self.h, C.byref(cc), C.byref(h)), "lsdg_depthmap_create")
         return DepthMapH(self, h, cam)
+
+    def batch_tune(self, jobs, runs=5):
+        """lsdg_batch_tune: (best shape, default median ms, best median ms);
+        later plans of this class of batch take the best shape."""
+        arr = (Job * len(jobs))(*jobs)
+        best = BatchConfig()
+        d, b = C.c_double(), C.c_double()
+        self.check(self.L.lsdg_batch_tune(self.h, arr, len(jobs), runs, C.byref(best), C.byref(d), C.byref(b)),
+                   "lsdg_batch_tune")
+        return best.as_dict(), d.value, b.value
+
+    def batch_key(self, jobs):
+        arr = (Job * len(jobs))(*jobs)
+        buf = C.create_string_buffer(64)
+        self.check(self.L.lsdg_batch_tune_key(self.h, arr, len(jobs), buf), "lsdg_batch_tune_key")
+        return buf.value.decode()
+
+    def batch_with_config(self, jobs, config):
+        arr = (Job * len(jobs))(*jobs)
+        cfg = BatchConfig.of(config)
+        h = C.c_void_p()
+        self.check(self.L.lsdg_batch_create_with_config(self.h, arr, len(jobs), C.byref(cfg), C.byref(h)),
+                   "lsdg_batch_create_with_config")
+        return Batch(self, h, len(jobs))
 
     def batch(self, jobs):
         arr = (Job * len(jobs))(*jobs)
@@ -747,7 +798,45 @@ class Batch:
         self.ctx.check(self.ctx.L.lsdg_batch_run(self.h, res), "lsdg_batch_run")
         return res
 
+    def config(self):
+        """The shape this plan took."""
+        c = BatchConfig()
+        self.ctx.check(self.ctx.L.lsdg_batch_get_config(self.h, C.byref(c)), "lsdg_batch_get_config")
+        return c.as_dict()
+
     def destroy(self):
         if self.h:
             self.ctx.L.lsdg_batch_destroy(self.h)
             self.h = None
+
+
+def tune_table_put(key, config):
+    st = load().lsdg_tune_table_put(key.encode(), C.byref(BatchConfig.of(config)))
+    if st != OK:
+        raise LsdgError(f"lsdg_tune_table_put failed with status {st}")
+
+
+def tune_table_get(key):
+    c = BatchConfig()
+    st = load().lsdg_tune_table_get(key.encode(), C.byref(c))
+    return c.as_dict() if st == OK else None
+
+
+def tune_table_clear():
+    load().lsdg_tune_table_clear()
+
+
+def tune_table_size():
+    return int(load().lsdg_tune_table_size())
+
+
+def tune_table_save(path):
+    st = load().lsdg_tune_table_save(os.fsencode(path))
+    if st != OK:
+        raise LsdgError(f"lsdg_tune_table_save({path}) failed with status {st}")
+
+
+def tune_table_load(path):
+    st = load().lsdg_tune_table_load(os.fsencode(path))
+    if st != OK:
+        raise LsdgError(f"lsdg_tune_table_load({path}) failed with status {st}")

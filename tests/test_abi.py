@@ -1,6 +1,7 @@
 """CPU checks of the C-ABI boundary: the HIP library loads, exports every
 entry point include/lsdg.h declares, and the ctypes mirror has the C layout."""
 import ctypes as C
+import glob
 import os
 import re
 import subprocess
@@ -10,11 +11,11 @@ import pytest
 import lsdg
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HDR = os.path.join(ROOT, "include", "lsdg.h")
+HDRS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared_functions():
-    txt = open(HDR).read()
+    txt = "".join(open(h).read() for h in HDRS)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(lsdg_[a-z0-9_]+)\s*\(", txt)))
 
@@ -22,7 +23,7 @@ def declared_functions():
 def test_header_declares_entry_points():
     fns = declared_functions()
     for must in ("lsdg_se3_track", "lsdg_sim3_track", "lsdg_depth_update", "lsdg_depth_create_keyframe",
-                 "lsdg_energy_grad", "lsdg_batch_track"):
+                 "lsdg_energy_grad", "lsdg_energy_grad_sim3", "lsdg_batch_track", "lsdg_batch_tune"):
         assert must in fns
 
 
@@ -35,15 +36,16 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sizes.c"
-    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lsdg.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lsdg_tune.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
                    'sizeof(lsdg_params), '
                    'sizeof(lsdg_track_stats), sizeof(lsdg_eval), sizeof(lsdg_hypotheses), sizeof(lsdg_camera), '
                    'sizeof(lsdg_job), sizeof(lsdg_job_result), offsetof(lsdg_params, stereo_max_steps), '
                    'sizeof(lsdg_point_dense), sizeof(lsdg_vertex), sizeof(lsdg_vbo_params), '
                    'sizeof(lsdg_constraint_candidate), sizeof(lsdg_constraint_edge), sizeof(lsdg_constraint_result), '
                    'offsetof(lsdg_constraint_result, frame_on_candidate), sizeof(lsdg_system_params), '
-                   'sizeof(lsdg_system_frame), offsetof(lsdg_system_frame, stats));return 0;}\n')
+                   'sizeof(lsdg_system_frame), offsetof(lsdg_system_frame, stats), sizeof(lsdg_batch_config));'
+                   'return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     got = [int(v) for v in subprocess.check_output([str(exe)]).split()]
@@ -52,7 +54,7 @@ def test_struct_layouts_match_c(tmp_path):
             C.sizeof(lsdg.PointDense), C.sizeof(lsdg.Vertex), C.sizeof(lsdg.VboParams),
             C.sizeof(lsdg.ConstraintCandidate), C.sizeof(lsdg.ConstraintEdge), C.sizeof(lsdg.ConstraintResult),
             lsdg.ConstraintResult.frame_on_candidate.offset, C.sizeof(lsdg.SystemParams), C.sizeof(lsdg.SystemFrame),
-            lsdg.SystemFrame.stats.offset]
+            lsdg.SystemFrame.stats.offset, C.sizeof(lsdg.BatchConfig)]
     assert got == want
 
 

@@ -249,3 +249,30 @@ def test_batch_rerun_after_new_frame_content(ctx, schedule):
     finally:
         ctx.set_params(lsdg.default_params())
         und.close()
+
+
+def test_cooperative_wait_gives_up_cleanly(ctx, schedule, monkeypatch):
+    """The cooperative kernel's row-exchange wait is bounded: with the poll
+    limit forced to 0 (LSDG_BATCH_SPIN_MAX, every level split over the job's
+    blocks) the first block of a job to arrive at an exchange gives up, every
+    block of the launch exits, and the run reports LSDG_EDEVICE instead of
+    hanging.  The context stays usable: a plan with the normal limit then
+    tracks as before."""
+    if not (schedule.startswith("coop") or schedule == "default"):
+        pytest.skip("the bounded wait belongs to the cooperative schedule")
+    insts = _instances(8, 2101, [(320, 240)])
+    items = _setup(ctx, insts, lsdg.FP32)
+    monkeypatch.setenv("LSDG_BATCH_SOLO", "0")
+    monkeypatch.setenv("LSDG_BATCH_SPIN_MAX", "0")
+    b = ctx.batch(_jobs(items))
+    assert b.config()["schedule"] == lsdg.SCHEDULE_COOP
+    with pytest.raises(lsdg.LsdgError, match="status -2"):
+        b.run()
+    b.destroy()
+    monkeypatch.delenv("LSDG_BATCH_SPIN_MAX")
+    b = ctx.batch(_jobs(items))
+    r = b.run()
+    b.destroy()
+    plain = [ctx.se3_track(ref, fr, synth.IDENTITY7) for kf, fr, ref in items]
+    for i, (pose, st, status) in enumerate(plain):
+        assert r[i].status == status and twist(r[i].frame_to_ref[:7], pose) <= 1e-9, i
