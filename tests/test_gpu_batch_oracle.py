@@ -15,7 +15,8 @@ launch, levels 4..1, at most 20 LM iterations per level:
 Keyframe-pair Sim3 batch: 128 alignments of 1920x1080 keyframe pairs (~500k
 points each, scale drift U[0.9, 1.1], up to 10 cm / 5 deg), FP16 and BF16:
 pose, scale and depth-residual error against the FP64 oracle, next to the
-FP32 twin's own error against it."""
+FP32 twin's own error against it (the depth residual evaluated in FP64 at
+each final pose)."""
 import functools
 
 import numpy as np
@@ -105,7 +106,7 @@ def _pairs():
 
 
 @functools.lru_cache(maxsize=None)
-def _oracle_sim3(prec):
+def _oracle_scenes(prec):
     o = pyoracle.Oracle(prec)
     out = []
     for d in _pairs():
@@ -113,17 +114,33 @@ def _oracle_sim3(prec):
         a.set_depth(d["a_idepth"], d["a_var"])
         b = o.frame(1, d["cam"], d["b_img"].astype(np.float32))
         b.set_depth(d["b_idepth"], d["b_var"])
-        out.append(o.sim3_track(o.ref(a), b, list(synth.IDENTITY7) + [1.0], 4, 1))
+        out.append((o, o.ref(a), b, a))
     return out
 
 
-def _sim3_errors(poses, stats, ref):
+@functools.lru_cache(maxsize=None)
+def _oracle_sim3(prec):
+    return [o.sim3_track(r, b, list(synth.IDENTITY7) + [1.0], 4, 1) for o, r, b, a in _oracle_scenes(prec)]
+
+
+def _mean_d(k, pose8):
+    """The FP64 depth residual (mean over terms) at a final pose, level 1:
+    the residual a result leaves.  (The trackers' reported last residual is
+    the last accepted step's, which can be a coarser level's when no step at
+    the finest level is accepted, as in upstream's trackFrameSim3.)"""
+    o, r, b, _ = _oracle_scenes(1)[k]
+    ev = o.sim3_evaluate(r, b, 1, pyoracle.sim3_inv(np.asarray(pose8, np.float64)))
+    return ev.sum_res_d / ev.num_terms_d
+
+
+def _sim3_errors(poses, ref):
     rig, sc, dres = [], [], []
-    for p, st, (q, qst, _) in zip(poses, stats, ref):
+    for k, (p, (q, qst, _)) in enumerate(zip(poses, ref)):
         r, s = posemath.sim3_error(list(p), list(q))
         rig.append(r)
         sc.append(s)
-        dres.append(abs(st.last_depth_residual - qst.last_depth_residual) / abs(qst.last_depth_residual))
+        d64 = _mean_d(k, q)
+        dres.append(abs(_mean_d(k, p) - d64) / d64)
     return np.array(rig), np.array(sc), np.array(dres)
 
 
@@ -152,12 +169,12 @@ def test_sim3_batch_1920x1080_error_vs_fp64(ctx, precision):
             assert list(res[16 * k + j].frame_to_ref) == list(res[16 * k].frame_to_ref)
     first = [res[16 * k] for k in range(8)]
     fp64 = _oracle_sim3(1)
-    rig, sc, dres = _sim3_errors([r.frame_to_ref for r in first], [r.stats for r in first], fp64)
+    rig, sc, dres = _sim3_errors([r.frame_to_ref for r in first], fp64)
     twin = _oracle_sim3(2)
-    rig32, sc32, dres32 = _sim3_errors([t[0] for t in twin], [t[1] for t in twin], fp64)
+    rig32, sc32, dres32 = _sim3_errors([t[0] for t in twin], fp64)
     name = {lsdg.FP16: "FP16", lsdg.BF16: "BF16"}[precision]
     print(f"Sim3 {name} vs FP64: pose max {rig.max():.3e} scale max {sc.max():.3e} depth-res max {dres.max():.3e}; "
           f"FP32 twin vs FP64: pose {rig32.max():.3e} scale {sc32.max():.3e} depth-res {dres32.max():.3e}")
     assert rig.max() <= max(3.0 * rig32.max(), 5e-3)
     assert sc.max() <= max(3.0 * sc32.max(), 5e-3)
-    assert dres.max() <= max(3.0 * dres32.max(), 5e-2)
+    assert dres.max() <= max(3.0 * dres32.max(), 1e-2)
