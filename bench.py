@@ -680,6 +680,21 @@ def main():
     tb16, batchedbf_fps, _, tunebf = batched(lsdg.BF16, args.steps, args.warmup)
     posesbf = list(tb16.collect())
 
+    # compensated FP32 sums (Kahan per thread, TwoSum trees, FP64 across
+    # blocks) beside the FP64 per-thread default: FP32 pyramids, same batch
+    ctx.set_sums(lsdg.SUMS_KAHAN)
+    try:
+        tk, batchedk_fps, _, tunek = batched(lsdg.FP32, args.steps, args.warmup)
+        posesk = list(tk.collect())
+        kahan = {"batched_fp32_frames_per_s": round(batchedk_fps, 2),
+                 "vs_fp64_sums": round(batchedk_fps / batched32_fps, 3), "tune": tunek,
+                 "max_twist_vs_fp64_sums": max(posemath.twist_error(a, b) for a, b in zip(posesk, poses32))}
+        if not args.no_plain:
+            _, k_ks = kernel_roofline(ctx, tk.plain_step, lsdg.KERNEL_EVAL)
+            kahan["plain_eval_avg_launch_us"] = round(1e3 * k_ks["ms"] / max(k_ks["launches"], 1), 3)
+    finally:
+        ctx.set_sums(lsdg.SUMS_FP64)
+
     out_plain = {}
     if not args.no_plain:
         t = timed(ctx, t32.plain_step, max(1, args.steps // 2), 1)
@@ -746,6 +761,7 @@ def main():
         "batched_bf16_frames_per_s": round(batchedbf_fps, 2),
         "tune_fp16": tune16,
         "tune_bf16": tunebf,
+        "sums_kahan": kahan,
         **out_plain,
         "mixed_resolution_batch_alignments_per_s": round(mixed_aps, 2),
         "tune_mixed_resolution": tune_mixed,
@@ -776,6 +792,7 @@ def main():
             "fp32_batched_and_plain": error_stats(poses32, ref64),
             "fp16_batched": error_stats(poses16, ref64),
             "bf16_batched": error_stats(posesbf, ref64),
+            "kahan_fp32_batched": error_stats(posesk, ref64),
             "converged_fp32": error_stats(conv["fp32"], ref64c),
             "converged_fp16": error_stats(conv["fp16"], ref64c),
             "note": "twist norm of every instance's frameToRef against the FP64 oracle's; the FP32 batch equals the "

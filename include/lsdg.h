@@ -209,6 +209,24 @@ lsdg_status lsdg_ctx_create(int32_t device, const lsdg_params* params, lsdg_ctx*
 lsdg_status lsdg_ctx_destroy(lsdg_ctx ctx);
 lsdg_status lsdg_ctx_set_params(lsdg_ctx ctx, const lsdg_params* params);
 lsdg_status lsdg_ctx_synchronize(lsdg_ctx ctx);
+
+/* How the tracker evaluations sum their FP32 per-point terms.
+ *   LSDG_SUMS_FP64  (default) each term is widened to FP64 and summed in FP64
+ *                   per thread and through every tree: the CPU restatement's
+ *                   sums to ~1e-16, so the GPU takes its LM path.
+ *   LSDG_SUMS_KAHAN compensated FP32: Kahan per-thread sums, TwoSum-merged
+ *                   wave and block trees, FP64 only in the cross-block merge
+ *                   and the solve.  Sums within ~1e-7 relative of the FP64
+ *                   ones, which can flip a borderline LM accept.
+ * Applies to the SE3 and Sim3 evaluations (lsdg_se3_*, lsdg_sim3_*) and to
+ * batch plans created afterwards; the adjoint kernels always sum in FP64.
+ * No counterpart in the reference, whose LGS6/LGS7 accumulations (upstream
+ * SE3Tracker::calculateWarpUpdate, Sim3Tracker::calcSim3LGS) sum in FP32 on
+ * the CPU. */
+#define LSDG_SUMS_FP64 0
+#define LSDG_SUMS_KAHAN 1
+lsdg_status lsdg_ctx_set_sums(lsdg_ctx ctx, int32_t mode);
+lsdg_status lsdg_ctx_get_sums(lsdg_ctx ctx, int32_t* mode);
 /* Last error message of this context (static storage, valid until next call). */
 const char* lsdg_ctx_last_error(lsdg_ctx ctx);
 

@@ -8,10 +8,11 @@
  * per job, the largest pyramid level a job's blocks evaluate whole rather
  * than split, and for the step schedule the whole-job threshold, the number
  * of parallel chains and the points per thread of a reduction tile.  Every
- * shape computes the same FP32 per-point terms and sums them in FP64; shapes
- * differ only in the order of that FP64 sum (~1e-16 relative), which leaves
- * the LM path unchanged: the tests hold every shape to 1e-9 twist of the
- * plain path with the same evaluation counts.  Only the time differs.
+ * shape computes the same FP32 per-point terms and sums them in the context's
+ * summation mode (lsdg_ctx_set_sums); shapes differ only in the order of the
+ * sum (~1e-16 relative under LSDG_SUMS_FP64), which leaves the LM path
+ * unchanged: the tests hold every shape to 1e-9 twist of the plain path with
+ * the same evaluation counts.  Only the time differs.
  *
  * lsdg_batch_tune searches the shapes for a set of jobs and records the best
  * in a process-wide table keyed by the batch's class (job kind, pyramid
@@ -51,9 +52,9 @@ typedef struct lsdg_batch_config {
 #define LSDG_TUNE_KEY_BYTES 64
 
 /* The class a batch belongs to, as the tuning table's key, e.g.
- * "se3/fp32/p19/single/n64": job kind, pyramid precision, ceil(log2) of the
- * largest level-0 pixel count, whether the batch mixes resolutions, and
- * ceil(log2) of the job count as a power of two. */
+ * "se3/fp32/p19/single/n64": job kind, pyramid precision (with "+kahan" under
+ * LSDG_SUMS_KAHAN), ceil(log2) of the largest level-0 pixel count, whether the
+ * batch mixes resolutions, and ceil(log2) of the job count. */
 lsdg_status lsdg_batch_tune_key(lsdg_ctx ctx, const lsdg_job* jobs, int32_t n, char key[LSDG_TUNE_KEY_BYTES]);
 
 /* A plan with an explicit shape (a cooperative schedule falls back to the

@@ -62,6 +62,91 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane l's value of v, wave-uniform (two readlanes, no LDS round trip).
+__device__ inline double lane_value(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// N independent FP64 divisions in one: lane i divides num[i] by den[i] and
+// every lane reads the N quotients back.  A division is a chain of about ten
+// dependent instructions that also serialise on VCC, so N of them one after
+// another cost N chains; spread over the lanes they cost one.  Each quotient
+// is the IEEE one, as a division on one lane gives.
+template <int N>
+__device__ inline void wave_div(const double (&num)[N], const double (&den)[N], double (&out)[N]) {
+  const int lane = lm_lane();
+  double a = num[0], b = den[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+    const bool m = lane == i;
+    a = m ? num[i] : a;
+    b = m ? den[i] : b;
+  }
+  const double q = a / b;
+#pragma unroll
+  for (int i = 0; i < N; ++i) out[i] = lane_value(q, i);
+}
+
+// sin and cos of |x| <= pi/4 from their Taylor series through x^17 / x^18
+// (the first omitted term is below 2e-19 of the result there, against 2^-53
+// rounding), in Horner form over x^2 with fused multiply-adds; other
+// arguments go to the library's sincos.
+__device__ inline void sincos_wave(double x, double* sv, double* cv) {
+  if (fabs(x) > 0.78539816339744831) {
+    sincos(x, sv, cv);
+    return;
+  }
+  const double y = x * x;
+  double ps = 1.0 / 355687428096000.0;       // 1/17!
+  ps = fma(ps, y, -1.0 / 1307674368000.0);   // 1/15!
+  ps = fma(ps, y, 1.0 / 6227020800.0);       // 1/13!
+  ps = fma(ps, y, -1.0 / 39916800.0);        // 1/11!
+  ps = fma(ps, y, 1.0 / 362880.0);           // 1/9!
+  ps = fma(ps, y, -1.0 / 5040.0);            // 1/7!
+  ps = fma(ps, y, 1.0 / 120.0);              // 1/5!
+  ps = fma(ps, y, -1.0 / 6.0);               // 1/3!
+  ps = fma(ps, y, 1.0);
+  double pc = -1.0 / 6402373705728000.0;     // 1/18!
+  pc = fma(pc, y, 1.0 / 20922789888000.0);   // 1/16!
+  pc = fma(pc, y, -1.0 / 87178291200.0);     // 1/14!
+  pc = fma(pc, y, 1.0 / 479001600.0);        // 1/12!
+  pc = fma(pc, y, -1.0 / 3628800.0);         // 1/10!
+  pc = fma(pc, y, 1.0 / 40320.0);            // 1/8!
+  pc = fma(pc, y, -1.0 / 720.0);             // 1/6!
+  pc = fma(pc, y, 1.0 / 24.0);               // 1/4!
+  pc = fma(pc, y, -0.5);                     // 1/2!
+  pc = fma(pc, y, 1.0);
+  *sv = x * ps;
+  *cv = pc;
+}
+
+// quat_normalize (lsdg_lie.h) with its four divisions spread over lanes 0-3.
+__device__ inline void quat_normalize_wave(double* q) {
+  const double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  double o[4];
+  wave_div<4>({q[0], q[1], q[2], q[3]}, {n, n, n, n}, o);
+  q[0] = o[0]; q[1] = o[1]; q[2] = o[2]; q[3] = o[3];
+}
+
+// pose_mul (lsdg_lie.h), the same arithmetic, normalised with quat_normalize_wave.
+__device__ inline Pose pose_mul_wave(const Pose& a, const Pose& b) {
+  Pose r;
+  r.q[0] = a.q[3] * b.q[0] + a.q[0] * b.q[3] + a.q[1] * b.q[2] - a.q[2] * b.q[1];
+  r.q[1] = a.q[3] * b.q[1] - a.q[0] * b.q[2] + a.q[1] * b.q[3] + a.q[2] * b.q[0];
+  r.q[2] = a.q[3] * b.q[2] + a.q[0] * b.q[1] - a.q[1] * b.q[0] + a.q[2] * b.q[3];
+  r.q[3] = a.q[3] * b.q[3] - a.q[0] * b.q[0] - a.q[1] * b.q[1] - a.q[2] * b.q[2];
+  quat_normalize_wave(r.q);
+  double M[9];
+  quat_R(a.q, M);
+  for (int i = 0; i < 3; ++i)
+    r.t[i] = a.s * (M[3 * i] * b.t[0] + M[3 * i + 1] * b.t[1] + M[3 * i + 2] * b.t[2]) + a.t[i];
+  r.s = a.s * b.s;
+  return r;
+}
+
 // remaining may be null: blocks that repeat another block's LM step (the
 // cooperative kernel) leave the count to that block.
 __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* remaining) {
@@ -71,40 +156,38 @@ __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* rem
 }
 
 // se3_exp (lsdg_lie.h) with sin and cos of theta / 2 (lane 0) and theta
-// (lane 1) evaluated at once, one sincos per lane.
-__device__ inline Pose se3_exp_wave(const double* __restrict__ xi, double* __restrict__ sc) {
+// (lane 1) evaluated at once, its three divisions in one (wave_div) and the
+// quaternion normalised with quat_normalize_wave.
+__device__ inline Pose se3_exp_wave(const double* __restrict__ xi) {
   const double* w = xi + 3;
   const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   const double th = sqrt(th2);
-  if (th >= 1e-10) {
-    const int lane = lm_lane();
-    if (lane < 2) {
-      double sv, cv;
-      sincos(lane == 0 ? 0.5 * th : th, &sv, &cv);
-      sc[lane] = sv;
-      sc[2 + lane] = cv;
-    }
-    wave_sync();
-  }
-  Pose r;
-  double k, c;
+  double k, c, bb, cc;
   if (th < 1e-10) {
     k = 0.5 - th2 / 48.0;
     c = 1.0 - th2 / 8.0;
-  } else {
-    k = sc[0] / th;
-    c = sc[2];
-  }
-  r.q[0] = k * w[0]; r.q[1] = k * w[1]; r.q[2] = k * w[2]; r.q[3] = c;
-  quat_normalize(r.q);
-  double bb, cc;
-  if (th < 1e-5) {
     bb = 0.5 - th2 / 24.0;
     cc = 1.0 / 6.0 - th2 / 120.0;
   } else {
-    bb = (1.0 - sc[3]) / th2;
-    cc = (th - sc[1]) / (th2 * th);
+    double sv, cv;
+    sincos_wave(lm_lane() == 0 ? 0.5 * th : th, &sv, &cv);
+    const double sh = lane_value(sv, 0), ch = lane_value(cv, 0);
+    const double sf = lane_value(sv, 1), cf = lane_value(cv, 1);
+    double q[3];
+    wave_div<3>({sh, 1.0 - cf, th - sf}, {th, th2, th2 * th}, q);
+    k = q[0];
+    c = ch;
+    if (th < 1e-5) {
+      bb = 0.5 - th2 / 24.0;
+      cc = 1.0 / 6.0 - th2 / 120.0;
+    } else {
+      bb = q[1];
+      cc = q[2];
+    }
   }
+  Pose r;
+  r.q[0] = k * w[0]; r.q[1] = k * w[1]; r.q[2] = k * w[2]; r.q[3] = c;
+  quat_normalize_wave(r.q);
   apply_abc(1.0, bb, cc, w, xi, r.t);
   r.s = 1.0;
   return r;
@@ -147,7 +230,7 @@ __device__ inline bool solve_and_propose(JobState& __restrict__ s, double* __res
     if (!(dd >= 0 && dd < 1)) return false;
     s.Tn = pose_mul(sim3_exp(inc), s.T);
   } else {
-    s.Tn = pose_mul(se3_exp_wave(inc, sc), s.T);
+    s.Tn = pose_mul_wave(se3_exp_wave(inc), s.T);
   }
   s.phase = 1;
   set_eval_pose(s, s.Tn, KIND == LSDG_JOB_SIM3);
@@ -206,11 +289,15 @@ __device__ void lm_step_se3(JobState& __restrict__ s, const JobDesc& __restrict_
   s.evalsAt[lvl]++;
   s.good = S[S3_GOOD];
   s.bad = S[S3_BAD];
-  s.meanRes = S[S3_SIGNED] / S[S3_GOOD];
-  s.usage = S[S3_USAGE] / (double)L.pts.n;
-  {
-    double sxx = S[S3_SXX], syy = S[S3_SYY], sx = S[S3_SX], sy = S[S3_SY], sw = S[S3_SW];
-    s.affAIt = (float)sqrt((syy - sy * sy / sw) / (sxx - sx * sx / sw));
+  const double sxx = S[S3_SXX], syy = S[S3_SYY], sx = S[S3_SX], sy = S[S3_SY], sw = S[S3_SW];
+  double q[5];  // the step's independent divisions, one per lane
+  wave_div<5>({S[S3_SIGNED], S[S3_USAGE], S[S3_ERR], sy * sy, sx * sx},
+              {S[S3_GOOD], (double)L.pts.n, S[S3_WARPED], sw, sw}, q);
+  s.meanRes = q[0];
+  s.usage = q[1];
+  // the affine iterate is read only when the affine brightness is estimated
+  if (P.use_affine_lightning_estimation) {
+    s.affAIt = (float)sqrt((syy - q[3]) / (sxx - q[4]));
     s.affBIt = (float)((sy - (double)s.affAIt * sx) / sw);
   }
   const float lim = P.min_goodperall_pixel_absmin * (float)(L.g.w * L.g.h);
@@ -218,7 +305,7 @@ __device__ void lm_step_se3(JobState& __restrict__ s, const JobDesc& __restrict_
     finish_job(s, 2, ntNext, remaining);
     return;
   }
-  const double err = S[S3_ERR] / S[S3_WARPED];
+  const double err = q[2];
   s.numRes[lvl]++;
   if (s.phase == 0) {
     if (P.use_affine_lightning_estimation) { s.affA = s.affAIt; s.affB = s.affBIt; }
@@ -349,8 +436,21 @@ __device__ void lm_step_sim3(JobState& __restrict__ s, const JobDesc& __restrict
 // Start state of every job: T = (frameToRef_init)^-1, first level, counters
 // zero.  One block, so the remaining-jobs counter can be reset first.
 __global__ __launch_bounds__(256) void k_batch_init(BatchArgs a) {
-  if (threadIdx.x == 0) *a.remaining = 0;
+  __shared__ bool clearRows;
+  if (threadIdx.x == 0) {
+    *a.remaining = 0;
+    // the run's epoch tags the cooperative row exchange; its low 16 bits
+    // never take the value 0, which the rows are cleared to on the first run
+    // and whenever the low bits wrap, so no word left by an earlier run can
+    // carry a tag of this one
+    unsigned e = *a.epoch + 1;
+    if ((e & 0xFFFFu) == 0) ++e;
+    *a.epoch = e;
+    clearRows = (e & 0xFFFFu) == 1;
+  }
   __syncthreads();
+  if (clearRows && a.coopRows)
+    for (size_t i = threadIdx.x; i < a.coopWords; i += blockDim.x) a.coopRows[i] = 0ull;
   for (int j = threadIdx.x; j < a.njobs; j += blockDim.x) {
     const JobDesc& jd = a.jobs[j];
     JobState s = {};
@@ -435,14 +535,10 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_eval(BatchArgs a, int parity
       uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
       se3_points<PREC, BATCH_U>(L.pts, i0 + tid, BATCH_BS, i1, L.img, L.g, x, kp, mask, acc);
     }
-    const double wsum = acc.wave_reduce_scatter();
+    const double wsum = acc.wave_slot();
     if (lane < NS) sh[wave][lane] = wsum;
     __syncthreads();
-    if (tid < NS) {
-      double v = 0.0;
-      for (int w = 0; w < BATCH_WAVES; ++w) v += sh[w][tid];
-      a.partials[((size_t)jd.slotBase + tile) * NS + tid] = v;
-    }
+    if (tid < NS) a.partials[((size_t)jd.slotBase + tile) * NS + tid] = Accum<ACC, NS>::merge_waves(&sh[0][tid], NS, BATCH_WAVES);
     __syncthreads();
   }
 }
@@ -558,14 +654,10 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
       uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
       se3_points<PREC, BATCH_U>(L.pts, tid, BATCH_JOB_BS, n, L.img, L.g, x, kp, mask, acc);
     }
-    const double wsum = acc.wave_reduce_scatter();
+    const double wsum = acc.wave_slot();
     if (lane < NS) sh[wave][lane] = wsum;
     __syncthreads();
-    if (tid < NS) {
-      double v = 0.0;
-      for (int w = 0; w < WAVES; ++w) v += sh[w][tid];
-      S[tid] = v;
-    }
+    if (tid < NS) S[tid] = Accum<ACC, NS>::merge_waves(&sh[0][tid], NS, WAVES);
     __syncthreads();
     if (tid < 64) {
       int next = 0;
@@ -582,9 +674,9 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
 // run the job's whole LM loop together, every level included.  A level of at
 // most soloMax points is evaluated whole by every block of the job.  A larger
 // level is cut into K contiguous point ranges: each block reduces its range
-// to an FP64 row, publishes it (device-scope stores, then a release add on the
-// job's arrival counter), waits until the counter shows all K rows of this
-// iteration, and sums the K rows in part order.  In both cases every block
+// to an FP64 row, publishes it as tagged words (device-scope stores), reads
+// the K rows until every word carries this iteration's tag, and sums them in
+// part order.  In both cases every block
 // of the job then holds the same sums and runs the same LM step on its own
 // LDS copy of the control block, so no result is broadcast; part 0 alone
 // counts finished jobs and writes the control block back.  Rows are double
@@ -598,7 +690,7 @@ __global__ __launch_bounds__(BATCH_JOB_BS) void k_batch_job(BatchArgs a) {
 #define LSDG_SPIN_SLEEP 2
 #endif
 
-template <int KIND, int PREC, int COOP_BS>
+template <int KIND, int PREC, int ACC, int COOP_BS>
 __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
   constexpr int NS = KIND == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3;
   constexpr int WAVES = COOP_BS / 64;
@@ -624,8 +716,8 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
   if (j >= a.njobs) return;  // padding of the last group (the whole block leaves)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int* remaining = part == 0 ? a.remaining : nullptr;
-  double* rows = a.coopRows + (size_t)j * 2 * K * NS;
-  unsigned* arrive = a.counters + j;
+  unsigned long long* rows = a.coopRows + (size_t)j * 2 * K * NS * 2;
+  const unsigned epochTag = (*a.epoch & 0xFFFFu) << 16;
   block_copy(&s, a.st + j, COOP_BS);
   block_copy(&jd, a.jobs + j, COOP_BS);
   __syncthreads();
@@ -643,7 +735,7 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
     TrackKParams kp = a.kp;
     kp.aff_a = s.affA;
     kp.aff_b = s.affB;
-    Accum<ACC_F64, NS> acc;
+    Accum<ACC, NS> acc;
     acc.zero();
     if (KIND == LSDG_JOB_SIM3) {
       kp.roll[0] = s.roll[0]; kp.roll[1] = s.roll[1]; kp.roll[2] = s.roll[2]; kp.roll[3] = s.roll[3];
@@ -652,52 +744,61 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
       uint8_t* mask = lvl == 1 ? jd.isGood : nullptr;
       se3_points<PREC, BATCH_U>(L.pts, i0 + tid, COOP_BS, i1, L.img, L.g, x, kp, mask, acc);
     }
-    const double wsum = acc.wave_reduce_scatter();
+    const double wsum = acc.wave_slot();
     if (lane < NS) sh[wave][lane] = wsum;
     __syncthreads();
     if (!split) {
-      if (tid < NS) {
-        double v = 0.0;
-        for (int w = 0; w < WAVES; ++w) v += sh[w][tid];
-        S[tid] = v;
-      }
+      if (tid < NS) S[tid] = Accum<ACC, NS>::merge_waves(&sh[0][tid], NS, WAVES);
     } else {
-      double* buf = rows + (size_t)(gen & 1) * K * NS;
-      if (tid < 64) {  // wave 0: publish this part's row, then wait for the others
+      // Tagged row exchange: each value is published as two 64-bit words
+      // (one 32-bit half of the FP64 value, one 32-bit tag: the run's epoch
+      // and this iteration's generation), so every word tells by itself
+      // whether it belongs to this iteration.  Wave 0 stores its part's row
+      // and then reads all K rows until every word carries the tag: one
+      // round of loads after the last part's stores land, with no arrival
+      // counter and no store-completion wait in between.
+      unsigned long long* buf = rows + (size_t)(gen & 1) * K * NS * 2;
+      const unsigned tag = epochTag | (gen & 0xFFFFu);
+      if (tid < 64) {
         if (tid < NS) {
-          double v = 0.0;
-          for (int w = 0; w < WAVES; ++w) v += sh[w][tid];
-          __hip_atomic_store((unsigned long long*)&buf[(size_t)part * NS + tid],
-                             (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const double v = Accum<ACC, NS>::merge_waves(&sh[0][tid], NS, WAVES);
+          const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+          unsigned long long* w = buf + ((size_t)part * NS + tid) * 2;
+          const unsigned long long t64 = (unsigned long long)tag << 32;
+          __hip_atomic_store(w, t64 | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(w + 1, t64 | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (tid == 0) {
-          wait_own_stores();  // the row stores above are agent-scope atomics
-          __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          // at most spinMax polls after the first look; arrived only if the
-          // count was reached
-          const unsigned target = (unsigned)K * (gen + 1);
-          bool ok = __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
-          for (unsigned spins = 0; !ok && spins < a.spinMax; ++spins) {
-            __builtin_amdgcn_s_sleep(LSDG_SPIN_SLEEP);
-            ok = __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+        // at most spinMax polls after the first look; the sum is taken in
+        // part order once every word of every row carries this tag
+        bool ok = false;
+        double v = 0.0;
+        for (unsigned spins = 0;; ++spins) {
+          bool mine = true;
+          double t = 0.0;
+          if (tid < NS) {
+            for (int q = 0; q < K; ++q) {
+              const unsigned long long* w = buf + ((size_t)q * NS + tid) * 2;
+              const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              mine = mine && (unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag;
+              t += __longlong_as_double((long long)((hi << 32) | (lo & 0xFFFFFFFFull)));
+            }
           }
-          arrived = ok;
+          if (__all(mine)) {
+            ok = true;
+            v = t;
+            break;
+          }
+          if (spins >= a.spinMax) break;
+          __builtin_amdgcn_s_sleep(LSDG_SPIN_SLEEP);
         }
+        if (tid < NS) S[tid] = v;
+        if (tid == 0) arrived = ok;
       }
       __syncthreads();
       if (!arrived) {
         if (tid == 0) __hip_atomic_store(a.remaining, -(1 << 20), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      if (tid < NS) {
-        double v = 0.0;
-        for (int q = 0; q < K; ++q) {
-          const unsigned long long bits = __hip_atomic_load((const unsigned long long*)&buf[(size_t)q * NS + tid],
-                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          v += __longlong_as_double((long long)bits);
-        }
-        S[tid] = v;
       }
       ++gen;
     }
@@ -716,32 +817,36 @@ __global__ __launch_bounds__(COOP_BS) void k_batch_coop(BatchArgs a) {
 }
 
 // ------------------------------------------------------------ launchers
+template <int KIND, int PREC>
+static void (*job_kernel(int acc))(BatchArgs) {
+  return acc == BATCH_ACC_KAHAN ? k_batch_job<KIND, PREC, ACC_KAHAN> : k_batch_job<KIND, PREC, ACC_F64>;
+}
+
 hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s, const KTimer* t) {
-  // FP64 per-thread sums whatever `acc` says: a thread of the whole-job block
-  // accumulates n / 512 points, unbounded in the level size
-  (void)acc;
-  auto k = kind == LSDG_JOB_SIM3 ? (prec == LSDG_FP16   ? k_batch_job<LSDG_JOB_SIM3, LSDG_FP16, ACC_F64>
-                                    : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SIM3, LSDG_BF16, ACC_F64>
-                                                        : k_batch_job<LSDG_JOB_SIM3, LSDG_FP32, ACC_F64>)
-                                 : (prec == LSDG_FP16   ? k_batch_job<LSDG_JOB_SE3, LSDG_FP16, ACC_F64>
-                                    : prec == LSDG_BF16 ? k_batch_job<LSDG_JOB_SE3, LSDG_BF16, ACC_F64>
-                                                        : k_batch_job<LSDG_JOB_SE3, LSDG_FP32, ACC_F64>);
+  auto k = kind == LSDG_JOB_SIM3 ? (prec == LSDG_FP16   ? job_kernel<LSDG_JOB_SIM3, LSDG_FP16>(acc)
+                                    : prec == LSDG_BF16 ? job_kernel<LSDG_JOB_SIM3, LSDG_BF16>(acc)
+                                                        : job_kernel<LSDG_JOB_SIM3, LSDG_FP32>(acc))
+                                 : (prec == LSDG_FP16   ? job_kernel<LSDG_JOB_SE3, LSDG_FP16>(acc)
+                                    : prec == LSDG_BF16 ? job_kernel<LSDG_JOB_SE3, LSDG_BF16>(acc)
+                                                        : job_kernel<LSDG_JOB_SE3, LSDG_FP32>(acc));
   return launch_k(k, dim3(a.njobs), dim3(BATCH_JOB_BS), s, t, a);
 }
 
-template <int BS>
+template <int ACC, int BS>
 static void (*coop_kernel_bs(int kind, int prec))(BatchArgs) {
   if (kind == LSDG_JOB_SIM3)
-    return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_FP16, BS>
-           : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_BF16, BS>
-                               : k_batch_coop<LSDG_JOB_SIM3, LSDG_FP32, BS>;
-  return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_FP16, BS>
-         : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_BF16, BS>
-                             : k_batch_coop<LSDG_JOB_SE3, LSDG_FP32, BS>;
+    return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_FP16, ACC, BS>
+           : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SIM3, LSDG_BF16, ACC, BS>
+                               : k_batch_coop<LSDG_JOB_SIM3, LSDG_FP32, ACC, BS>;
+  return prec == LSDG_FP16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_FP16, ACC, BS>
+         : prec == LSDG_BF16 ? k_batch_coop<LSDG_JOB_SE3, LSDG_BF16, ACC, BS>
+                             : k_batch_coop<LSDG_JOB_SE3, LSDG_FP32, ACC, BS>;
 }
 
-static void (*coop_kernel(int kind, int prec, int bs))(BatchArgs) {
-  return bs == 256 ? coop_kernel_bs<256>(kind, prec) : coop_kernel_bs<512>(kind, prec);
+static void (*coop_kernel(int kind, int prec, int acc, int bs))(BatchArgs) {
+  if (acc == BATCH_ACC_KAHAN)
+    return bs == 256 ? coop_kernel_bs<ACC_KAHAN, 256>(kind, prec) : coop_kernel_bs<ACC_KAHAN, 512>(kind, prec);
+  return bs == 256 ? coop_kernel_bs<ACC_F64, 256>(kind, prec) : coop_kernel_bs<ACC_F64, 512>(kind, prec);
 }
 
 bool batch_coop_block_size_ok(int bs) { return bs == 256 || bs == 512; }
@@ -751,21 +856,23 @@ int batch_coop_grid(int njobs, int K, int xcds) {
   return ((njobs + xcds - 1) / xcds) * xcds * K;
 }
 
-hipError_t batch_coop_capacity(int kind, int prec, int bs, int* blocks) {
+hipError_t batch_coop_capacity(int kind, int prec, int acc, int bs, int* blocks) {
   *blocks = 0;
   if (!batch_coop_block_size_ok(bs)) return hipErrorInvalidValue;
   int dev = 0, perCU = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coop_kernel(kind, prec, bs), bs, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coop_kernel(kind, prec, acc, bs), bs, 0);
   *blocks = e == hipSuccess ? perCU * cus : 0;
   return e;
 }
 
-hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, int bs, hipStream_t s, const KTimer* t) {
+hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, int acc, int bs, hipStream_t s,
+                             const KTimer* t) {
   if (!batch_coop_block_size_ok(bs)) return hipErrorInvalidValue;
-  return launch_k(coop_kernel(kind, prec, bs), dim3(batch_coop_grid(a.njobs, a.coopK, a.xcds)), dim3(bs), s, t, a);
+  return launch_k(coop_kernel(kind, prec, acc, bs), dim3(batch_coop_grid(a.njobs, a.coopK, a.xcds)), dim3(bs), s,
+                  t, a);
 }
 
 hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
@@ -773,23 +880,19 @@ hipError_t launch_batch_init(const BatchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int KIND, int PREC>
+static void (*eval_kernel(int acc))(BatchArgs, int) {
+  return acc == BATCH_ACC_KAHAN ? k_batch_eval<KIND, PREC, ACC_KAHAN> : k_batch_eval<KIND, PREC, ACC_F64>;
+}
+
 hipError_t launch_batch_eval(const BatchArgs& a, int kind, int prec, int acc, int parity, int grid, hipStream_t s,
                              const KTimer* t) {
-  auto k = kind == LSDG_JOB_SIM3
-               ? (acc == BATCH_ACC_F32
-                      ? (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SIM3, LSDG_FP16, ACC_F32>
-                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SIM3, LSDG_BF16, ACC_F32>
-                                             : k_batch_eval<LSDG_JOB_SIM3, LSDG_FP32, ACC_F32>)
-                      : (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SIM3, LSDG_FP16, ACC_F64>
-                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SIM3, LSDG_BF16, ACC_F64>
-                                             : k_batch_eval<LSDG_JOB_SIM3, LSDG_FP32, ACC_F64>))
-               : (acc == BATCH_ACC_F32
-                      ? (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SE3, LSDG_FP16, ACC_F32>
-                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SE3, LSDG_BF16, ACC_F32>
-                                             : k_batch_eval<LSDG_JOB_SE3, LSDG_FP32, ACC_F32>)
-                      : (prec == LSDG_FP16   ? k_batch_eval<LSDG_JOB_SE3, LSDG_FP16, ACC_F64>
-                         : prec == LSDG_BF16 ? k_batch_eval<LSDG_JOB_SE3, LSDG_BF16, ACC_F64>
-                                             : k_batch_eval<LSDG_JOB_SE3, LSDG_FP32, ACC_F64>));
+  auto k = kind == LSDG_JOB_SIM3 ? (prec == LSDG_FP16   ? eval_kernel<LSDG_JOB_SIM3, LSDG_FP16>(acc)
+                                    : prec == LSDG_BF16 ? eval_kernel<LSDG_JOB_SIM3, LSDG_BF16>(acc)
+                                                        : eval_kernel<LSDG_JOB_SIM3, LSDG_FP32>(acc))
+                                 : (prec == LSDG_FP16   ? eval_kernel<LSDG_JOB_SE3, LSDG_FP16>(acc)
+                                    : prec == LSDG_BF16 ? eval_kernel<LSDG_JOB_SE3, LSDG_BF16>(acc)
+                                                        : eval_kernel<LSDG_JOB_SE3, LSDG_FP32>(acc));
   return launch_k(k, dim3(grid), dim3(BATCH_BS), s, t, a, parity);
 }
 

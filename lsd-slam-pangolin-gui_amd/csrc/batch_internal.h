@@ -7,8 +7,8 @@
 namespace lsdg {
 
 constexpr int MAX_BATCH_JOBS = 2048;
-constexpr int BATCH_ACC_F64 = 0;
-constexpr int BATCH_ACC_F32 = 1;
+constexpr int BATCH_ACC_F64 = 0;    // LSDG_SUMS_FP64
+constexpr int BATCH_ACC_KAHAN = 1;  // LSDG_SUMS_KAHAN
 // Whole-job kernel: block size and the bound on the evaluations one job may
 // run inside it (the LM loop ends long before; the bound makes every block's
 // exit unconditional).
@@ -64,7 +64,9 @@ struct BatchArgs {
   int* remaining;     // jobs still running
   int ppt;            // points per thread per tile
   int jobMaxPoints;   // levels with at most this many points run in k_batch_job
-  double* coopRows;   // k_batch_coop: [njobs][2][coopK] rows of NS doubles
+  unsigned long long* coopRows;  // k_batch_coop: [njobs][2][coopK][NS][2] tagged words
+  size_t coopWords;              // words of coopRows (0 without the cooperative schedule)
+  unsigned* epoch;               // run counter of this chain (k_batch_init increments it)
   int coopK;          // k_batch_coop: blocks per job
   int soloMax;        // k_batch_coop: levels of at most this many points are not split
   unsigned spinMax;   // k_batch_coop: polls of the row-exchange wait before the blocks give up
@@ -84,9 +86,9 @@ bool batch_coop_block_size_ok(int bs);
 // Blocks of a cooperative launch: njobs * K, or with xcds > 1 the jobs
 // padded to a multiple of xcds (the padding blocks exit at once).
 int batch_coop_grid(int njobs, int K, int xcds);
-hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, int bs, hipStream_t s,
+hipError_t launch_batch_coop(const BatchArgs& a, int kind, int prec, int acc, int bs, hipStream_t s,
                              const KTimer* t = nullptr);
-hipError_t batch_coop_capacity(int kind, int prec, int bs, int* blocks);
+hipError_t batch_coop_capacity(int kind, int prec, int acc, int bs, int* blocks);
 // Launches with an optional kernel timer (see launch_k).
 hipError_t launch_batch_job(const BatchArgs& a, int kind, int prec, int acc, hipStream_t s,
                             const KTimer* t = nullptr);

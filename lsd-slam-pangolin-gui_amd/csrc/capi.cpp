@@ -158,6 +158,19 @@ extern "C" lsdg_status lsdg_ctx_synchronize(lsdg_ctx c) {
   return e == hipSuccess ? LSDG_OK : fail(c, LSDG_EDEVICE, "hipStreamSynchronize", e);
 }
 
+extern "C" lsdg_status lsdg_ctx_set_sums(lsdg_ctx c, int32_t mode) {
+  if (!c) return LSDG_EINVAL;
+  if (mode != LSDG_SUMS_FP64 && mode != LSDG_SUMS_KAHAN) return fail(c, LSDG_EINVAL, "unknown summation mode");
+  c->sums = mode;
+  return LSDG_OK;
+}
+
+extern "C" lsdg_status lsdg_ctx_get_sums(lsdg_ctx c, int32_t* mode) {
+  if (!c || !mode) return LSDG_EINVAL;
+  *mode = c->sums;
+  return LSDG_OK;
+}
+
 extern "C" const char* lsdg_ctx_last_error(lsdg_ctx c) { return c ? c->err.c_str() : "null context"; }
 
 // --------------------------------------------------------------- frames

@@ -50,7 +50,8 @@ struct lsdg_batch_s {
   int xcds = 1;      // dispatch groups a job's cooperative blocks are kept in (1: job order)
   int soloMax = 2048;
   unsigned spinMax = COOP_SPIN_MAX;
-  double* d_coop = nullptr;
+  unsigned long long* d_coop = nullptr;  // tagged row-exchange words of the cooperative kernel
+  unsigned* d_epoch = nullptr;           // per-chain run counters
   std::vector<lsdg_job> jobs;
   std::vector<int> depthVersion;
   std::vector<JobDesc> h_jobs;
@@ -95,6 +96,7 @@ static void batch_free(lsdg_batch b) {
   if (b->h_st) (void)hipHostFree(b->h_st);
   if (b->d_partials) (void)hipFree(b->d_partials);
   if (b->d_coop) (void)hipFree(b->d_coop);
+  if (b->d_epoch) (void)hipFree(b->d_epoch);
   if (b->d_pyr) (void)hipFree(b->d_pyr);
   if (b->h_pyr) (void)hipHostFree(b->h_pyr);
   delete b;
@@ -168,7 +170,10 @@ static BatchArgs batch_args(lsdg_batch b, int c) {
   a.remaining = b->d_remaining + c;
   a.ppt = b->ppt;
   a.jobMaxPoints = b->jobMaxPoints;
-  a.coopRows = b->d_coop + (size_t)j0 * 2 * std::max(b->coopK, 1) * (b->kind == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3);
+  const size_t rowWords = (size_t)2 * std::max(b->coopK, 1) * (b->kind == LSDG_JOB_SIM3 ? NS_SIM3 : NS_SE3) * 2;
+  a.coopRows = b->d_coop ? b->d_coop + (size_t)j0 * rowWords : nullptr;
+  a.coopWords = b->d_coop ? (size_t)(j1 - j0) * rowWords : 0;
+  a.epoch = b->d_epoch + c;
   a.coopK = b->coopK;
   a.soloMax = b->soloMax;
   a.spinMax = b->spinMax;
@@ -198,7 +203,7 @@ static lsdg_status capture(lsdg_batch b, bool first, hipGraph_t* g, hipGraphExec
     BatchArgs a = batch_args(b, k);
     if (first) {
       e = launch_batch_init(a, st);
-      if (e == hipSuccess && b->coopK > 0) e = launch_batch_coop(a, b->kind, b->prec, b->coopBS, st);
+      if (e == hipSuccess && b->coopK > 0) e = launch_batch_coop(a, b->kind, b->prec, b->acc, b->coopBS, st);
       else if (e == hipSuccess && b->jobMaxPoints > 0) e = launch_batch_job(a, b->kind, b->prec, b->acc, st);
     }
     for (int s = 0; s < steps && e == hipSuccess; ++s)
@@ -312,15 +317,10 @@ lsdg_status lsdg::batch_create_shaped(lsdg_ctx c, const lsdg_job* jobs, int32_t 
   b->n = n;
   b->kind = kind;
   b->prec = prec;
-  // Per-thread FP64 sums (the plain path's summation, so the batch takes the
-  // plain path's LM decisions).  LSDG_BATCH_ACC=1 makes the tile evaluation
-  // sum each thread's 4 points of a tile in FP32 below the FP64 trees: about
-  // 14% less time in that VALU-bound kernel, but sums that differ from the
-  // plain path's by ~1e-7 relative, enough to flip borderline LM decisions of
-  // weakly constrained problems (Sim3 scale).  The whole-job kernel always
-  // sums in FP64.
-  b->acc = BATCH_ACC_F64;
-  if (const char* e = std::getenv("LSDG_BATCH_ACC")) b->acc = std::atoi(e) ? BATCH_ACC_F32 : BATCH_ACC_F64;
+  // The context's summation mode at creation (lsdg_ctx_set_sums): FP64 sums
+  // take the plain path's LM decisions; compensated FP32 sums differ from
+  // them by ~1e-7 relative.
+  b->acc = c->sums == LSDG_SUMS_KAHAN ? BATCH_ACC_KAHAN : BATCH_ACC_F64;
   if (const char* e = std::getenv("LSDG_BATCH_STEPS")) b->steps = std::max(0, std::min(512, std::atoi(e) & ~1));
   // polls of the cooperative row-exchange wait before a job's blocks give up
   // (LSDG_EDEVICE); a small value makes the give-up path run on purpose
@@ -334,7 +334,7 @@ lsdg_status lsdg::batch_create_shaped(lsdg_ctx c, const lsdg_job* jobs, int32_t 
   // blocks per job the plan falls back to the whole-job kernel and steps.
   if (shape.schedule == LSDG_SCHEDULE_COOP) {
     int cap = 0;
-    if (batch_coop_capacity(kind, prec, shape.coop_block, &cap) == hipSuccess) {
+    if (batch_coop_capacity(kind, prec, b->acc, shape.coop_block, &cap) == hipSuccess) {
       // with dispatch groups the jobs are padded to a multiple of the groups
       const int xcds = std::max(1, std::min(64, shape.xcd_groups));
       const int nPad = xcds > 1 ? ((n + xcds - 1) / xcds) * xcds : n;
@@ -365,7 +365,9 @@ lsdg_status lsdg::batch_create_shaped(lsdg_ctx c, const lsdg_job* jobs, int32_t 
       (e = hipMalloc(&b->d_ntiles, sizeof(int) * 2 * n)) != hipSuccess ||
       (e = hipMalloc(&b->d_counters, sizeof(unsigned) * n)) != hipSuccess ||
       (e = hipMalloc(&b->d_remaining, 64)) != hipSuccess ||
-      (b->coopK > 0 && (e = hipMalloc(&b->d_coop, sizeof(double) * 2 * b->coopK * NS_SIM3 * (size_t)n)) != hipSuccess) ||
+      (b->coopK > 0 && (e = hipMalloc(&b->d_coop, sizeof(unsigned long long) * 2 * b->coopK * NS_SIM3 * 2 * (size_t)n)) !=
+                           hipSuccess) ||
+      (e = hipMalloc(&b->d_epoch, 64)) != hipSuccess || (e = hipMemset(b->d_epoch, 0, 64)) != hipSuccess ||
       (e = hipHostMalloc(&b->h_remaining, 64, hipHostMallocDefault)) != hipSuccess ||
       (e = hipHostMalloc(&b->h_st, sizeof(JobState) * n, hipHostMallocDefault)) != hipSuccess) {
     batch_free(b);
@@ -561,7 +563,7 @@ static lsdg_status batch_run_profiled(lsdg_batch b) {
     CK(c, launch_batch_init(a, c->stream));
     if (b->coopK > 0) {
       KTimer kt = prof_timer(c);
-      CK(c, launch_batch_coop(a, b->kind, b->prec, b->coopBS, c->stream, &kt));
+      CK(c, launch_batch_coop(a, b->kind, b->prec, b->acc, b->coopBS, c->stream, &kt));
       prof_timer_done(c, LSDG_KERNEL_BATCH_COOP, kt, 0.0);
     } else if (b->jobMaxPoints > 0) {
       KTimer kt = prof_timer(c);

@@ -151,6 +151,7 @@ struct lsdg_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   lsdg_params P;
+  int sums = LSDG_SUMS_FP64;  // lsdg_ctx_set_sums
   std::string err;
   // reduction scratch for the plain (one-alignment-at-a-time) path
   double* d_partials = nullptr;

@@ -6,6 +6,7 @@
 // for scaling the HBM byte counters.
 //
 //   lsdg_prof [runs] [copy_mib] [precision: 0 FP32 (default), 1 FP16, 2 BF16]
+//             [sums: 0 FP64 per thread (default), 1 Kahan-compensated FP32]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -88,6 +89,7 @@ int main(int argc, char** argv) {
   const int runs = argc > 1 ? std::atoi(argv[1]) : 5;
   const int copy_mib = argc > 2 ? std::atoi(argv[2]) : 0;
   const int prec = argc > 3 ? std::atoi(argv[3]) : LSDG_FP32;
+  const int sums = argc > 4 ? std::atoi(argv[4]) : LSDG_SUMS_FP64;
   if (prec < LSDG_FP32 || prec > LSDG_BF16) {
     std::fprintf(stderr, "precision must be 0, 1 or 2\n");
     return 2;
@@ -109,6 +111,7 @@ int main(int argc, char** argv) {
   for (int l = 0; l < LSDG_PYRAMID_LEVELS; ++l) P.max_its_per_lvl[l] = 20;
   lsdg_ctx ctx = nullptr;
   CHECK(lsdg_ctx_create(0, &P, &ctx));
+  CHECK(lsdg_ctx_set_sums(ctx, sums));
   const lsdg_camera cam{640, 480, 525.0f, 525.0f, 319.5f, 239.5f};
   const int B = 64;
   std::vector<lsdg_frame> kfs(B), frs(B);

@@ -8,10 +8,11 @@
 // need is per point and the pose is known up front, so one kernel computes
 // warp, bilinear sample, residual, isGood, depth-variance weight, Huber weight,
 // the Jacobian row and its normal-equation terms, and reduces them:
-//   per thread  FP64 accumulation of the FP32 per-point terms (or FP32 with
-//               TwoSum compensation) over a grid-stride slice of points,
-//   per wave    butterfly (__shfl_xor),
-//   per block   FP64 sum of the wave results in LDS -> one FP64 partial row,
+//   per thread  FP64 accumulation of the FP32 per-point terms over a
+//               grid-stride slice of points (or, with LSDG_SUMS_KAHAN,
+//               Kahan-compensated FP32 pairs),
+//   per wave    butterfly (__shfl_xor; pairs: TwoSum reduce-scatter),
+//   per block   sum of the wave results in LDS -> one FP64 partial row,
 //   last block  FP64 sum of all partial rows in block order (deterministic),
 //               written straight to the caller's (possibly host-mapped) buffer.
 // The per-point float expressions keep the reference's operation order so the
@@ -30,16 +31,19 @@ __device__ inline void reduce_and_merge(Accum<MODE, NS>& acc, double* __restrict
   __shared__ double sh[EVAL_WAVES][NS];
   __shared__ bool amLast;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  acc.wave_reduce();
-  if (lane == 0) {
+  if constexpr (MODE == ACC_F64) {
+    acc.wave_reduce();
+    if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) sh[wave][k] = acc.value(k);
+      for (int k = 0; k < NS; ++k) sh[wave][k] = acc.value(k);
+    }
+  } else {
+    const double cell = acc.wave_slot();
+    if (lane < NS) sh[wave][lane] = cell;
   }
   __syncthreads();
   if (threadIdx.x < NS) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < EVAL_WAVES; ++w) s += sh[w][threadIdx.x];
+    const double s = Accum<MODE, NS>::merge_waves(&sh[0][threadIdx.x], NS, EVAL_WAVES);
     __hip_atomic_store((unsigned long long*)&partials[(size_t)blockIdx.x * NS + threadIdx.x],
                        (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -134,14 +138,23 @@ static inline const void* level_image(lsdg_frame f, int level) {
   return f->precision == LSDG_FP32 ? (const void*)f->d_grad[level] : (const void*)f->d_gradh[level];
 }
 
+template <int ACC>
+static auto kern_se3_eval(int prec) {
+  return prec == LSDG_FP16 ? k_se3_eval<LSDG_FP16, ACC> : prec == LSDG_BF16 ? k_se3_eval<LSDG_BF16, ACC> : k_se3_eval<LSDG_FP32, ACC>;
+}
+template <int ACC>
+static auto kern_sim3_eval(int prec) {
+  return prec == LSDG_FP16   ? k_sim3_eval<LSDG_FP16, ACC>
+         : prec == LSDG_BF16 ? k_sim3_eval<LSDG_BF16, ACC>
+                             : k_sim3_eval<LSDG_FP32, ACC>;
+}
+
 hipError_t launch_se3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                            const TrackKParams& kp, uint8_t* d_isgood, double* d_out, const KTimer* t) {
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
   const dim3 grid(eval_blocks(c, pts.n)), block(EVAL_BS);
   const void* img = level_image(f, level);
-  auto k = f->precision == LSDG_FP16 ? k_se3_eval<LSDG_FP16, ACC_F64>
-           : f->precision == LSDG_BF16 ? k_se3_eval<LSDG_BF16, ACC_F64>
-                                       : k_se3_eval<LSDG_FP32, ACC_F64>;
+  auto k = c->sums == LSDG_SUMS_KAHAN ? kern_se3_eval<ACC_KAHAN>(f->precision) : kern_se3_eval<ACC_F64>(f->precision);
   return launch_k(k, grid, block, c->stream, t, pts, img, g, x, kp, d_isgood, c->d_partials, c->d_counter, d_out);
 }
 
@@ -150,9 +163,7 @@ hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, in
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
   const dim3 grid(eval_blocks(c, pts.n)), block(EVAL_BS);
   const void* img = level_image(f, level);
-  auto k = f->precision == LSDG_FP16 ? k_sim3_eval<LSDG_FP16, ACC_F64>
-           : f->precision == LSDG_BF16 ? k_sim3_eval<LSDG_BF16, ACC_F64>
-                                       : k_sim3_eval<LSDG_FP32, ACC_F64>;
+  auto k = c->sums == LSDG_SUMS_KAHAN ? kern_sim3_eval<ACC_KAHAN>(f->precision) : kern_sim3_eval<ACC_F64>(f->precision);
   return launch_k(k, grid, block, c->stream, t, pts, img, (const float*)f->d_idepth[level],
                   (const float*)f->d_ivar[level], g, x, kp, c->d_partials, c->d_counter, d_out);
 }

@@ -175,14 +175,17 @@ __device__ inline void jac6(float px, float py, float pz, float gx, float gy, fl
 }
 
 // Per-thread accumulators of the per-point FP32 terms.
-//  ACC_F64: each FP32 term is widened and summed in FP64 (the parity mode:
-//           the same terms as the CPU restatement, summed to ~1e-16).
-//  ACC_F32: each thread sums its few points (a tile gives it 4) in FP32;
-//           the wave, block and cross-block trees run in FP64.  Half the
-//           accumulator registers and VALU work of ACC_F64, for the
-//           reduced-precision pyramids, whose taps carry 11 or 8 bits.
+//  ACC_F64:   each FP32 term is widened and summed in FP64 (the parity mode:
+//             the same terms as the CPU restatement, summed to ~1e-16).
+//  ACC_KAHAN: compensated FP32.  Each thread keeps a (sum, correction) pair
+//             per value and adds a term with Kahan's update (4 FP32
+//             operations); the wave and block trees merge pairs with TwoSum
+//             (Knuth), so a block's total is its FP32 pair widened to FP64
+//             once, and only the cross-block merge runs in FP64.  The error
+//             of a block total is ~2 * 2^-24 * sum |term|, independent of the
+//             point count.
 constexpr int ACC_F64 = 0;
-constexpr int ACC_F32 = 1;
+constexpr int ACC_KAHAN = 1;
 
 // lo + up after swapping lo's upper half-wave (offset 32) or odd rows
 // (offset 16) with up's lower half-wave / even rows, 32 bits at a time
@@ -253,34 +256,111 @@ struct Accum<ACC_F64, NS> {
     }
   }
   __device__ inline double value(int k) const { return v[k]; }
-  __device__ inline double wave_reduce_scatter() const { return wave_reduce_scatter_f64<NS>(v); }
+  __device__ inline double wave_slot() const { return wave_reduce_scatter_f64<NS>(v); }
+  __device__ static inline double merge_waves(const double* cells, int stride, int waves) {
+    double t = 0.0;
+    for (int w = 0; w < waves; ++w) t += cells[(size_t)w * stride];
+    return t;
+  }
 };
 
+// Kahan's compensated update of the pair (s, c), whose value is s + c.
+__device__ inline void kahan_add(float& s, float& c, float x) {
+  const float y = x + c;
+  const float t = s + y;
+  c = y - (t - s);
+  s = t;
+}
+
+// (s, c) += (s2, c2): the sums' rounding error is recovered exactly with
+// TwoSum and folded into the correction.
+__device__ inline void pair_merge(float& s, float& c, float s2, float c2) {
+  const float t = s + s2;
+  const float bp = t - s;
+  const float e = (s - (t - bp)) + (s2 - bp);
+  c = (c + c2) + e;
+  s = t;
+}
+
+// A pair packed into one 64-bit LDS cell (low word sum, high word correction).
+__device__ inline double pack_pair(float s, float c) {
+  return __longlong_as_double((long long)(((unsigned long long)__float_as_uint(c) << 32) | __float_as_uint(s)));
+}
+__device__ inline void unpack_pair(double cell, float& s, float& c) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(cell);
+  s = __uint_as_float((unsigned)b);
+  c = __uint_as_float((unsigned)(b >> 32));
+}
+
+// The reduce-scatter of wave_reduce_scatter_f64 over (sum, correction) pairs:
+// afterwards lane l holds the wave's pair of value l ((0, 0) for l >= NS).
 template <int NS>
-struct Accum<ACC_F32, NS> {
-  float v[NS];
-  __device__ inline void zero() {
+__device__ inline void wave_reduce_scatter_pair(const float* vs, const float* vc, float& rs, float& rc) {
+  static_assert(NS <= 64, "reduce-scatter holds one value per lane");
+  const int lane = threadIdx.x & 63;
+  float s[32], c[32];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) v[k] = 0.f;
+  for (int i = 0; i < 32; ++i) {
+    float ls = i < NS ? vs[i] : 0.f, lc = i < NS ? vc[i] : 0.f;
+    float us = i + 32 < NS ? vs[i + 32] : 0.f, uc = i + 32 < NS ? vc[i + 32] : 0.f;
+    auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(ls), __float_as_uint(us), false, false);
+    auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(lc), __float_as_uint(uc), false, false);
+    s[i] = __uint_as_float(a[0]);
+    c[i] = __uint_as_float(b[0]);
+    pair_merge(s[i], c[i], __uint_as_float(a[1]), __uint_as_float(b[1]));
   }
-  __device__ inline void add(int k, float x) { v[k] += x; }
-  __device__ inline void wave_reduce() {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      double a = (double)v[k];
+  for (int i = 0; i < 16; ++i) {
+    auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(s[i]), __float_as_uint(s[i + 16]), false, false);
+    auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(c[i]), __float_as_uint(c[i + 16]), false, false);
+    s[i] = __uint_as_float(a[0]);
+    c[i] = __uint_as_float(b[0]);
+    pair_merge(s[i], c[i], __uint_as_float(a[1]), __uint_as_float(b[1]));
+  }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
-      v[k] = (float)a;
+  for (int h = 8; h >= 1; h >>= 1) {
+    const bool hi = (lane & h) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const float ss = hi ? s[i] : s[i + h], sc = hi ? c[i] : c[i + h];
+      float ks = hi ? s[i + h] : s[i], kc = hi ? c[i + h] : c[i];
+      pair_merge(ks, kc, __shfl_xor(ss, h), __shfl_xor(sc, h));
+      s[i] = ks;
+      c[i] = kc;
     }
   }
-  __device__ inline double value(int k) const { return (double)v[k]; }
-  __device__ inline double wave_reduce_scatter() const {
-    double d[NS];
+  rs = s[0];
+  rc = c[0];
+}
+
+template <int NS>
+struct Accum<ACC_KAHAN, NS> {
+  float s[NS], c[NS];
+  __device__ inline void zero() {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) d[k] = (double)v[k];
-    return wave_reduce_scatter_f64<NS>(d);
+    for (int k = 0; k < NS; ++k) s[k] = c[k] = 0.f;
+  }
+  __device__ inline void add(int k, float x) { kahan_add(s[k], c[k], x); }
+  // The LDS cell lane l of each wave writes for value l.
+  __device__ inline double wave_slot() const {
+    float rs, rc;
+    wave_reduce_scatter_pair<NS>(s, c, rs, rc);
+    return pack_pair(rs, rc);
+  }
+  // Block total of one value from its wave cells (cells[w * stride], wave
+  // order), widened to FP64.
+  __device__ static inline double merge_waves(const double* cells, int stride, int waves) {
+    float ts, tc;
+    unpack_pair(cells[0], ts, tc);
+    for (int w = 1; w < waves; ++w) {
+      float ws, wc;
+      unpack_pair(cells[(size_t)w * stride], ws, wc);
+      pair_merge(ts, tc, ws, wc);
+    }
+    return (double)ts + (double)tc;
   }
 };
+
 // Index of (i, j), i <= j, in the row-major upper triangle of a 7x7 matrix.
 __host__ __device__ constexpr int tri7(int i, int j) { return i * 7 - i * (i - 1) / 2 + (j - i); }
 

@@ -15,6 +15,7 @@ PYR = 5
 OK, DIVERGED, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, -1, -2, -3, -4
 NOALLOC_MAX_POINTS = 1 << 20  # LSDG_NOALLOC_MAX_POINTS
 FP32, FP16, BF16 = 0, 1, 2
+SUMS_FP64, SUMS_KAHAN = 0, 1
 WHAT_IMAGE, WHAT_GRADIENTS, WHAT_MAXGRAD, WHAT_IDEPTH, WHAT_IDEPTH_VAR, WHAT_REF_PIXEL_WAS_GOOD = range(6)
 JOB_SE3, JOB_SIM3 = 0, 1
 
@@ -257,6 +258,8 @@ def load():
         "lsdg_ctx_destroy": (i32, [vp]),
         "lsdg_ctx_set_params": (i32, [vp, C.POINTER(Params)]),
         "lsdg_ctx_synchronize": (i32, [vp]),
+        "lsdg_ctx_set_sums": (i32, [vp, i32]),
+        "lsdg_ctx_get_sums": (i32, [vp, C.POINTER(i32)]),
         "lsdg_ctx_last_error": (C.c_char_p, [vp]),
         "lsdg_frame_create": (i32, [vp, i32, C.POINTER(Camera), fp, i32, C.POINTER(vp)]),
         "lsdg_frame_create_u8": (i32, [vp, i32, C.POINTER(Camera), fp, i32, i32, C.POINTER(vp)]),
@@ -376,6 +379,16 @@ class Context:
 
     def sync(self):
         self.check(self.L.lsdg_ctx_synchronize(self.h), "lsdg_ctx_synchronize")
+
+    def set_sums(self, mode):
+        """SUMS_FP64 (default) or SUMS_KAHAN: how tracker evaluations and
+        batch plans created afterwards sum their FP32 per-point terms."""
+        self.check(self.L.lsdg_ctx_set_sums(self.h, int(mode)), "lsdg_ctx_set_sums")
+
+    def sums(self):
+        m = C.c_int32()
+        self.check(self.L.lsdg_ctx_get_sums(self.h, C.byref(m)), "lsdg_ctx_get_sums")
+        return m.value
 
     def profile(self, enable):
         self.check(self.L.lsdg_ctx_profile(self.h, int(enable)), "lsdg_ctx_profile")
