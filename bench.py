@@ -19,6 +19,7 @@ starts.
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
+import math
 import json
 import os
 import sys
@@ -34,49 +35,89 @@ import posemath  # noqa: E402
 import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Sources that define the batched kernels' code: a counter record of HBM
-# traffic (profiles/traffic.json) counts only for the sources it was measured on.
-KERNEL_SOURCES = ("batch.hip", "batch_internal.h", "track_device.h", "lsdg_internal.h", "lsdg_lie.h",
-                  "capi_batch.cpp")
 
 
-def kernel_sources_sha():
-    import hashlib
-    h = hashlib.sha256()
-    for name in KERNEL_SOURCES:
-        with open(os.path.join(ROOT, "lsd-slam-pangolin-gui_amd", "csrc", name), "rb") as f:
-            h.update(name.encode())
-            h.update(f.read())
-    return h.hexdigest()
+def _pmc_pass(counter, env, timeout_s=120):
+    """One rocprofv3 counter pass over lsdg_prof (the headline workload through
+    the C-ABI, after a float4 copy of 256 MiB): per-dispatch values of
+    `counter` for the copy kernel and the last k_batch_coop dispatch."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = os.path.join(ROOT, "lsd-slam-pangolin-gui_amd", "lsdg_prof")
+    if not shutil.which("rocprofv3") or not os.path.exists(prof):
+        return None, "rocprofv3 or lsdg_prof not available"
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "c", "--", prof, "2", "256", "0"]
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            return None, f"{counter} pass timed out"
+        path = None
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    path = os.path.join(root, f)
+        if r.returncode != 0 or path is None:
+            return None, f"{counter} pass failed (exit {r.returncode})"
+        copy, coop = [], []
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                if "copy_f4" in row["Kernel_Name"]:
+                    copy.append(float(row["Counter_Value"]))
+                elif "k_batch_coop" in row["Kernel_Name"]:
+                    coop.append(float(row["Counter_Value"]))
+        if not copy or not coop:
+            return None, f"{counter} pass: kernels not found"
+        return (float(np.median(copy)), coop[-1]), None
 
 
-def measured_traffic(record):
-    """HBM bytes per launch (FETCH_SIZE and WRITE_SIZE passes, scaled by a
-    copy kernel of known size) from the committed counter record of this
-    kernel and workload, when it was measured on the current kernel sources;
-    (None, reason) otherwise."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            data = json.load(f)
-    except (OSError, ValueError):
-        return None, "no counter record"
-    r = data.get(record)
-    if not r:
-        return None, f"no counter record for {record}"
-    if r.get("sources_sha256") != kernel_sources_sha():
-        return None, "the counter record was taken on other kernel sources"
-    return r["bytes"], r["profile"]
+def measured_traffic(shape):
+    """HBM bytes per launch of k_batch_coop, measured now: FETCH_SIZE and
+    WRITE_SIZE passes (KiB per dispatch) over the same workload in the plan's
+    shape, each scaled by what the counter reads for a copy kernel that reads
+    and writes 256 MiB.  (bytes, description) or (None, reason)."""
+    env = dict(os.environ, TMPDIR="/tmp", LSDG_BATCH_COOP="1", LSDG_BATCH_COOP_BS=str(shape["coop_block"]),
+               LSDG_BATCH_COOP_K=str(shape["coop_k"]), LSDG_BATCH_SOLO=str(shape["solo_max"]),
+               LSDG_BATCH_XCDS=str(shape["xcd_groups"]))
+    f, why = _pmc_pass("FETCH_SIZE", env)
+    if f is None:
+        return None, why
+    w, why = _pmc_pass("WRITE_SIZE", env)
+    if w is None:
+        return None, why
+    mib256 = 256.0 * (1 << 20)
+    fetch_scale = mib256 / (f[0] * 1024.0)
+    write_scale = mib256 / (w[0] * 1024.0)
+    traffic = f[1] * 1024.0 * fetch_scale + w[1] * 1024.0 * write_scale
+    return traffic, (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over lsdg_prof 2 256 0 in the same shape: "
+                     f"FETCH {f[1]:.0f} KiB x {fetch_scale:.3f}, WRITE {w[1]:.0f} KiB x {write_scale:.3f} "
+                     f"(copy kernel of 256 MiB: FETCH {f[0]:.0f} KiB, WRITE {w[0]:.0f} KiB)")
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it (what lscpu prints as the model
+    name), with the family / model numbers and the widest SIMD extension,
+    since virtualised hosts often report only a generic name."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    info.setdefault(k.strip(), v.strip())
+                if not line.strip() and info:
+                    break
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    flags = info.get("flags", "").split()
+    simd = "avx512f" if "avx512f" in flags else "avx2" if "avx2" in flags else "sse4_2" if "sse4_2" in flags else "?"
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, model "
+            f"{info.get('model', '?')}, widest SIMD {simd})")
 
 
 def cpu_threads():
@@ -528,7 +569,8 @@ def cpu_tracking_baseline(insts, params, seconds_per_run, runs=5):
     return dict(value=round(out[n_max][0], 2), unit="frames/s", cores=n_max, kind="port",
                 single_thread_value=round(out[1][0], 2), cpu_model=cpu_model(),
                 sample=f"plain SE3 alignments of the 64 640x480 instances (oracle FP32 restatement of the "
-                       f"reference CPU path, -O3 x86-64-v3, per-frame pyramid build included): median of {runs} "
+                       f"reference CPU path, -O3 -march=x86-64-v3 where the reference's Release build uses "
+                       f"-march=native -DENABLE_SSE, per-frame pyramid build included): median of {runs} "
                        f"runs of {seconds_per_run:.1f} s each on {n_max} threads ({out[n_max][1]} alignments) and "
                        f"on 1 thread ({out[1][1]} alignments)")
 
@@ -555,18 +597,41 @@ def cpu_depth_baseline(d, params, seconds):
     return out
 
 
-def oracle_poses(insts, params, prec=1):
-    """Every instance's pose from the oracle (prec 1: FP64 throughout)."""
+def oracle_poses(insts, params, prec=1, init_eps=0.0, seed=0):
+    """Every instance's pose from the oracle (prec 1: FP64 throughout),
+    tracked from the identity or, with init_eps > 0, from an initial pose
+    perturbed by init_eps in translation (m) and rotation (rad)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     o = pyoracle.Oracle(prec, params=params)
     out = []
-    for inst in insts:
+    for k, inst in enumerate(insts):
         kf = o.frame(0, inst["cam"], inst["kf_img"].astype(np.float32))
         kf.set_depth(inst["idepth"], inst["var"])
         fr = o.frame(1, inst["cam"], inst["fr_img"].astype(np.float32))
-        out.append(o.se3_track(o.ref(kf), fr, synth.IDENTITY7)[0])
+        init = synth.IDENTITY7 if init_eps <= 0 else list(synth.sample_pose(seed + k, init_eps, math.degrees(init_eps)))
+        out.append(o.se3_track(o.ref(kf), fr, init)[0])
     return out
+
+
+def fp64_self_spread(insts, params, ref64, eps=1e-9, seeds=(11, 12, 13)):
+    """How far the FP64 oracle's own result moves when its initial pose is
+    perturbed by eps (1e-9 m / 1e-9 rad): per instance, the largest twist
+    between the pose tracked from the identity and from each perturbed start.
+    The LM's accept / reject and stop decisions and the in-image / isGood
+    sets are discontinuous, so on some instances a 1e-9 perturbation of the
+    start moves the FP64 answer by 1e-5 or more: no finite-precision path can
+    be held closer to the FP64 oracle than this spread."""
+    spread = np.zeros(len(insts))
+    for sd in seeds:
+        alt = oracle_poses(insts, params, 1, eps, 1000 * sd)
+        spread = np.maximum(spread, [posemath.twist_error(p, q) for p, q in zip(alt, ref64)])
+    return spread
+
+
+def within_spread(poses, ref64, spread, floor=1e-5):
+    e = np.array([posemath.twist_error(p, q) for p, q in zip(poses, ref64)])
+    return int((e <= np.maximum(spread, floor)).sum())
 
 
 def error_stats(poses, ref):
@@ -595,6 +660,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="plans in their default shape")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the FETCH_SIZE / WRITE_SIZE counter passes")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the batched FP32 leg (for kernel profiling)")
     ap.add_argument("--precision", choices=("fp32", "fp16", "bf16"), default="fp32",
@@ -672,7 +738,10 @@ def main():
                           round(1e3 * dom["ms"] / max(dom["launches"], 1), 3), "dominant_gbs": round(dom_gbs, 2),
                           "dominant_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1), 1)}))
         return
-    traffic, traffic_src = measured_traffic(f"{dom_name}_se3_640x480_x64_fp32")
+    if dom_name == "k_batch_coop" and not args.no_traffic:
+        traffic, traffic_src = measured_traffic(shape)
+    else:
+        traffic, traffic_src = None, "not measured (--no-traffic or not the cooperative schedule)"
 
     # reduced-precision pyramids beside it
     t16, batched16_fps, _, tune16 = batched(lsdg.FP16, args.steps, args.warmup)
@@ -776,7 +845,10 @@ def main():
         "constraint_search_640x480": constraint,
         "slam_system_640x480": system,
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None if traffic is None else round(traffic, 0),
+                     "traffic_over_algorithmic": None if traffic is None else
+                     round(traffic / max(dom["bytes"] / max(dom["launches"], 1), 1.0), 3),
                      "traffic_source": traffic_src, "kernel": dom_desc,
                      "avg_launch_us": round(1e3 * dom["ms"] / max(dom["launches"], 1), 3),
                      "launches": dom["launches"],
@@ -788,6 +860,7 @@ def main():
         out["cpu_depth_updates_per_s"] = {str(k): round(v, 2) for k, v in cd.items()}
         ref64 = oracle_poses(insts, params)
         ref64c = oracle_poses(insts, converged_params())
+        spread = fp64_self_spread(insts, params, ref64)
         out["pose_error_vs_fp64_oracle"] = {
             "fp32_batched_and_plain": error_stats(poses32, ref64),
             "fp16_batched": error_stats(poses16, ref64),
@@ -795,8 +868,16 @@ def main():
             "kahan_fp32_batched": error_stats(posesk, ref64),
             "converged_fp32": error_stats(conv["fp32"], ref64c),
             "converged_fp16": error_stats(conv["fp16"], ref64c),
+            "fp64_self_spread": {"max": float(spread.max()), "median": float(np.median(spread)),
+                                 "above_1e-5": int((spread > 1e-5).sum()),
+                                 "fp32_within_max_1e-5_spread": within_spread(poses32, ref64, spread),
+                                 "fp16_within_max_1e-5_spread": within_spread(poses16, ref64, spread),
+                                 "bf16_within_max_1e-5_spread": within_spread(posesbf, ref64, spread),
+                                 "kahan_within_max_1e-5_spread": within_spread(posesk, ref64, spread)},
             "note": "twist norm of every instance's frameToRef against the FP64 oracle's; the FP32 batch equals the "
-                    "plain path to 1e-16 and the oracle's FP32 twin to 1e-9"}
+                    "plain path to 1e-16 and the oracle's FP32 twin to 1e-9; fp64_self_spread is how far the FP64 "
+                    "oracle itself moves when its start pose is perturbed by 1e-9 (3 seeds), the floor under "
+                    "any path's error against it"}
     print(json.dumps(out))
 
 

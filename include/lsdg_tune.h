@@ -52,7 +52,7 @@ typedef struct lsdg_batch_config {
 #define LSDG_TUNE_KEY_BYTES 64
 
 /* The class a batch belongs to, as the tuning table's key, e.g.
- * "se3/fp32/p19/single/n64": job kind, pyramid precision (with "+kahan" under
+ * "se3/fp32/p19/single/n64": job kind, pyramid precision (with "-kahan" under
  * LSDG_SUMS_KAHAN), ceil(log2) of the largest level-0 pixel count, whether the
  * batch mixes resolutions, and ceil(log2) of the job count. */
 lsdg_status lsdg_batch_tune_key(lsdg_ctx ctx, const lsdg_job* jobs, int32_t n, char key[LSDG_TUNE_KEY_BYTES]);

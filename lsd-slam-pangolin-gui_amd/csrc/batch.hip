@@ -62,91 +62,6 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Lane l's value of v, wave-uniform (two readlanes, no LDS round trip).
-__device__ inline double lane_value(double v, int l) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// N independent FP64 divisions in one: lane i divides num[i] by den[i] and
-// every lane reads the N quotients back.  A division is a chain of about ten
-// dependent instructions that also serialise on VCC, so N of them one after
-// another cost N chains; spread over the lanes they cost one.  Each quotient
-// is the IEEE one, as a division on one lane gives.
-template <int N>
-__device__ inline void wave_div(const double (&num)[N], const double (&den)[N], double (&out)[N]) {
-  const int lane = lm_lane();
-  double a = num[0], b = den[0];
-#pragma unroll
-  for (int i = 1; i < N; ++i) {
-    const bool m = lane == i;
-    a = m ? num[i] : a;
-    b = m ? den[i] : b;
-  }
-  const double q = a / b;
-#pragma unroll
-  for (int i = 0; i < N; ++i) out[i] = lane_value(q, i);
-}
-
-// sin and cos of |x| <= pi/4 from their Taylor series through x^17 / x^18
-// (the first omitted term is below 2e-19 of the result there, against 2^-53
-// rounding), in Horner form over x^2 with fused multiply-adds; other
-// arguments go to the library's sincos.
-__device__ inline void sincos_wave(double x, double* sv, double* cv) {
-  if (fabs(x) > 0.78539816339744831) {
-    sincos(x, sv, cv);
-    return;
-  }
-  const double y = x * x;
-  double ps = 1.0 / 355687428096000.0;       // 1/17!
-  ps = fma(ps, y, -1.0 / 1307674368000.0);   // 1/15!
-  ps = fma(ps, y, 1.0 / 6227020800.0);       // 1/13!
-  ps = fma(ps, y, -1.0 / 39916800.0);        // 1/11!
-  ps = fma(ps, y, 1.0 / 362880.0);           // 1/9!
-  ps = fma(ps, y, -1.0 / 5040.0);            // 1/7!
-  ps = fma(ps, y, 1.0 / 120.0);              // 1/5!
-  ps = fma(ps, y, -1.0 / 6.0);               // 1/3!
-  ps = fma(ps, y, 1.0);
-  double pc = -1.0 / 6402373705728000.0;     // 1/18!
-  pc = fma(pc, y, 1.0 / 20922789888000.0);   // 1/16!
-  pc = fma(pc, y, -1.0 / 87178291200.0);     // 1/14!
-  pc = fma(pc, y, 1.0 / 479001600.0);        // 1/12!
-  pc = fma(pc, y, -1.0 / 3628800.0);         // 1/10!
-  pc = fma(pc, y, 1.0 / 40320.0);            // 1/8!
-  pc = fma(pc, y, -1.0 / 720.0);             // 1/6!
-  pc = fma(pc, y, 1.0 / 24.0);               // 1/4!
-  pc = fma(pc, y, -0.5);                     // 1/2!
-  pc = fma(pc, y, 1.0);
-  *sv = x * ps;
-  *cv = pc;
-}
-
-// quat_normalize (lsdg_lie.h) with its four divisions spread over lanes 0-3.
-__device__ inline void quat_normalize_wave(double* q) {
-  const double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-  double o[4];
-  wave_div<4>({q[0], q[1], q[2], q[3]}, {n, n, n, n}, o);
-  q[0] = o[0]; q[1] = o[1]; q[2] = o[2]; q[3] = o[3];
-}
-
-// pose_mul (lsdg_lie.h), the same arithmetic, normalised with quat_normalize_wave.
-__device__ inline Pose pose_mul_wave(const Pose& a, const Pose& b) {
-  Pose r;
-  r.q[0] = a.q[3] * b.q[0] + a.q[0] * b.q[3] + a.q[1] * b.q[2] - a.q[2] * b.q[1];
-  r.q[1] = a.q[3] * b.q[1] - a.q[0] * b.q[2] + a.q[1] * b.q[3] + a.q[2] * b.q[0];
-  r.q[2] = a.q[3] * b.q[2] + a.q[0] * b.q[1] - a.q[1] * b.q[0] + a.q[2] * b.q[3];
-  r.q[3] = a.q[3] * b.q[3] - a.q[0] * b.q[0] - a.q[1] * b.q[1] - a.q[2] * b.q[2];
-  quat_normalize_wave(r.q);
-  double M[9];
-  quat_R(a.q, M);
-  for (int i = 0; i < 3; ++i)
-    r.t[i] = a.s * (M[3 * i] * b.t[0] + M[3 * i + 1] * b.t[1] + M[3 * i + 2] * b.t[2]) + a.t[i];
-  r.s = a.s * b.s;
-  return r;
-}
-
 // remaining may be null: blocks that repeat another block's LM step (the
 // cooperative kernel) leave the count to that block.
 __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* remaining) {
@@ -156,38 +71,40 @@ __device__ inline void finish_job(JobState& s, int status, int* ntNext, int* rem
 }
 
 // se3_exp (lsdg_lie.h) with sin and cos of theta / 2 (lane 0) and theta
-// (lane 1) evaluated at once, its three divisions in one (wave_div) and the
-// quaternion normalised with quat_normalize_wave.
-__device__ inline Pose se3_exp_wave(const double* __restrict__ xi) {
+// (lane 1) evaluated at once, one sincos per lane.
+__device__ inline Pose se3_exp_wave(const double* __restrict__ xi, double* __restrict__ sc) {
   const double* w = xi + 3;
   const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   const double th = sqrt(th2);
-  double k, c, bb, cc;
+  if (th >= 1e-10) {
+    const int lane = lm_lane();
+    if (lane < 2) {
+      double sv, cv;
+      sincos(lane == 0 ? 0.5 * th : th, &sv, &cv);
+      sc[lane] = sv;
+      sc[2 + lane] = cv;
+    }
+    wave_sync();
+  }
+  Pose r;
+  double k, c;
   if (th < 1e-10) {
     k = 0.5 - th2 / 48.0;
     c = 1.0 - th2 / 8.0;
+  } else {
+    k = sc[0] / th;
+    c = sc[2];
+  }
+  r.q[0] = k * w[0]; r.q[1] = k * w[1]; r.q[2] = k * w[2]; r.q[3] = c;
+  quat_normalize(r.q);
+  double bb, cc;
+  if (th < 1e-5) {
     bb = 0.5 - th2 / 24.0;
     cc = 1.0 / 6.0 - th2 / 120.0;
   } else {
-    double sv, cv;
-    sincos_wave(lm_lane() == 0 ? 0.5 * th : th, &sv, &cv);
-    const double sh = lane_value(sv, 0), ch = lane_value(cv, 0);
-    const double sf = lane_value(sv, 1), cf = lane_value(cv, 1);
-    double q[3];
-    wave_div<3>({sh, 1.0 - cf, th - sf}, {th, th2, th2 * th}, q);
-    k = q[0];
-    c = ch;
-    if (th < 1e-5) {
-      bb = 0.5 - th2 / 24.0;
-      cc = 1.0 / 6.0 - th2 / 120.0;
-    } else {
-      bb = q[1];
-      cc = q[2];
-    }
+    bb = (1.0 - sc[3]) / th2;
+    cc = (th - sc[1]) / (th2 * th);
   }
-  Pose r;
-  r.q[0] = k * w[0]; r.q[1] = k * w[1]; r.q[2] = k * w[2]; r.q[3] = c;
-  quat_normalize_wave(r.q);
   apply_abc(1.0, bb, cc, w, xi, r.t);
   r.s = 1.0;
   return r;
@@ -230,7 +147,7 @@ __device__ inline bool solve_and_propose(JobState& __restrict__ s, double* __res
     if (!(dd >= 0 && dd < 1)) return false;
     s.Tn = pose_mul(sim3_exp(inc), s.T);
   } else {
-    s.Tn = pose_mul_wave(se3_exp_wave(inc), s.T);
+    s.Tn = pose_mul(se3_exp_wave(inc, sc), s.T);
   }
   s.phase = 1;
   set_eval_pose(s, s.Tn, KIND == LSDG_JOB_SIM3);
@@ -289,15 +206,12 @@ __device__ void lm_step_se3(JobState& __restrict__ s, const JobDesc& __restrict_
   s.evalsAt[lvl]++;
   s.good = S[S3_GOOD];
   s.bad = S[S3_BAD];
-  const double sxx = S[S3_SXX], syy = S[S3_SYY], sx = S[S3_SX], sy = S[S3_SY], sw = S[S3_SW];
-  double q[5];  // the step's independent divisions, one per lane
-  wave_div<5>({S[S3_SIGNED], S[S3_USAGE], S[S3_ERR], sy * sy, sx * sx},
-              {S[S3_GOOD], (double)L.pts.n, S[S3_WARPED], sw, sw}, q);
-  s.meanRes = q[0];
-  s.usage = q[1];
+  s.meanRes = S[S3_SIGNED] / S[S3_GOOD];
+  s.usage = S[S3_USAGE] / (double)L.pts.n;
   // the affine iterate is read only when the affine brightness is estimated
   if (P.use_affine_lightning_estimation) {
-    s.affAIt = (float)sqrt((syy - q[3]) / (sxx - q[4]));
+    const double sxx = S[S3_SXX], syy = S[S3_SYY], sx = S[S3_SX], sy = S[S3_SY], sw = S[S3_SW];
+    s.affAIt = (float)sqrt((syy - sy * sy / sw) / (sxx - sx * sx / sw));
     s.affBIt = (float)((sy - (double)s.affAIt * sx) / sw);
   }
   const float lim = P.min_goodperall_pixel_absmin * (float)(L.g.w * L.g.h);
@@ -305,7 +219,7 @@ __device__ void lm_step_se3(JobState& __restrict__ s, const JobDesc& __restrict_
     finish_job(s, 2, ntNext, remaining);
     return;
   }
-  const double err = q[2];
+  const double err = S[S3_ERR] / S[S3_WARPED];
   s.numRes[lvl]++;
   if (s.phase == 0) {
     if (P.use_affine_lightning_estimation) { s.affA = s.affAIt; s.affB = s.affBIt; }
@@ -538,7 +452,8 @@ __global__ __launch_bounds__(BATCH_BS) void k_batch_eval(BatchArgs a, int parity
     const double wsum = acc.wave_slot();
     if (lane < NS) sh[wave][lane] = wsum;
     __syncthreads();
-    if (tid < NS) a.partials[((size_t)jd.slotBase + tile) * NS + tid] = Accum<ACC, NS>::merge_waves(&sh[0][tid], NS, BATCH_WAVES);
+    if (tid < NS)
+      a.partials[((size_t)jd.slotBase + tile) * NS + tid] = Accum<ACC, NS>::merge_waves(&sh[0][tid], NS, BATCH_WAVES);
     __syncthreads();
   }
 }

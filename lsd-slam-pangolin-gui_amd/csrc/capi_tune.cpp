@@ -287,7 +287,7 @@ extern "C" lsdg_status lsdg_batch_tune_key(lsdg_ctx c, const lsdg_job* jobs, int
   const int prec = jobs[0].frame->precision;
   // compensated sums run different kernels: their own class
   std::snprintf(key, LSDG_TUNE_KEY_BYTES, "%s/%s%s/p%d/%s/n%d", jobs[0].kind == LSDG_JOB_SIM3 ? "sim3" : "se3",
-                kPrec[prec >= 0 && prec <= 2 ? prec : 0], c->sums == LSDG_SUMS_KAHAN ? "+kahan" : "",
+                kPrec[prec >= 0 && prec <= 2 ? prec : 0], c->sums == LSDG_SUMS_KAHAN ? "-kahan" : "",
                 ceil_log2(maxP), mixed ? "mixed" : "single", ceil_log2(n));
   return LSDG_OK;
 }

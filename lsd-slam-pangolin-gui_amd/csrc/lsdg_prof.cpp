@@ -7,6 +7,7 @@
 //
 //   lsdg_prof [runs] [copy_mib] [precision: 0 FP32 (default), 1 FP16, 2 BF16]
 //             [sums: 0 FP64 per thread (default), 1 Kahan-compensated FP32]
+//             [tune: 1 = run the plan tuner first and use its shape, as bench.py does]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "../../include/lsdg.h"
+#include "../../include/lsdg_tune.h"
 
 extern "C" {
 size_t synth_scene_size(void);
@@ -90,6 +92,7 @@ int main(int argc, char** argv) {
   const int copy_mib = argc > 2 ? std::atoi(argv[2]) : 0;
   const int prec = argc > 3 ? std::atoi(argv[3]) : LSDG_FP32;
   const int sums = argc > 4 ? std::atoi(argv[4]) : LSDG_SUMS_FP64;
+  const int tune = argc > 5 ? std::atoi(argv[5]) : 0;
   if (prec < LSDG_FP32 || prec > LSDG_BF16) {
     std::fprintf(stderr, "precision must be 0, 1 or 2\n");
     return 2;
@@ -132,6 +135,13 @@ int main(int argc, char** argv) {
     J.start_level = 4;
     J.final_level = 1;
     J.rebuild_pyramid = 1;
+  }
+  if (tune) {  // as the bench does: the tuner's shape, recorded in the table, is the plan's
+    lsdg_batch_config best;
+    double t_default = 0, t_best = 0;
+    CHECK(lsdg_batch_tune(ctx, jobs.data(), B, 5, &best, &t_default, &t_best));
+    std::printf("tuned shape: block %d, K %d, solo %d, xcd groups %d (%.4f -> %.4f ms)\n", best.coop_block,
+                best.coop_k, best.solo_max, best.xcd_groups, t_default, t_best);
   }
   lsdg_batch b = nullptr;
   CHECK(lsdg_batch_create(ctx, jobs.data(), B, &b));

@@ -140,7 +140,9 @@ static inline const void* level_image(lsdg_frame f, int level) {
 
 template <int ACC>
 static auto kern_se3_eval(int prec) {
-  return prec == LSDG_FP16 ? k_se3_eval<LSDG_FP16, ACC> : prec == LSDG_BF16 ? k_se3_eval<LSDG_BF16, ACC> : k_se3_eval<LSDG_FP32, ACC>;
+  return prec == LSDG_FP16   ? k_se3_eval<LSDG_FP16, ACC>
+         : prec == LSDG_BF16 ? k_se3_eval<LSDG_BF16, ACC>
+                             : k_se3_eval<LSDG_FP32, ACC>;
 }
 template <int ACC>
 static auto kern_sim3_eval(int prec) {

@@ -169,7 +169,12 @@ def test_kahan_batch(kahan):
         items.append((kf, fr, kahan.ref(kf)))
     jobs = [lsdg.Job(ref.h, fr.h, lsdg.pose8(list(synth.IDENTITY7) + [1.0]), lsdg.JOB_SE3, 4, 1, 1)
             for kf, fr, ref in items]
-    assert "+kahan/" in kahan.batch_key(jobs)
+    key = kahan.batch_key(jobs)
+    assert "-kahan/" in key
+    b = kahan.batch(jobs)
+    lsdg.tune_table_put(key, b.config())  # the key is one the table accepts
+    b.destroy()
+    assert lsdg.tune_table_get(key) is not None
     o = pyoracle.Oracle(2, params=p)
     twin = []
     for inst in insts:
