@@ -545,6 +545,30 @@ typedef struct lsdg_constraint_result {
 lsdg_status lsdg_constraint_search(lsdg_ctx ctx, const lsdg_constraint_candidate* candidates, int32_t n,
                                    float strictness, lsdg_constraint_result* results);
 
+/* The keyframe candidates of a constraint search (upstream's
+ * TrackableKeyFrameSearch::findCandidates and findEuclideanOverlapFrames,
+ * without FAB-MAP's appearance candidates): of the keyframes `others`, those
+ * whose camera position lies near the keyframe's and whose optical axis
+ * points the same way.  With t the camToWorld translation, s its scale, d
+ * the mean inverse depth and z the optical axis (third column of the
+ * rotation), a keyframe k is a candidate when
+ *   |(t_k - t) * min(d_k / s_k, d / s)|^2 <= closeness_th * 15 / kf_dist_weight^2
+ *   z_k . z >= cos((1 - 0.25 closeness_th) * (fov_x + fov_y) / 2),
+ * fov = 2 atan(size / (2 f)) of `cam`.  Host computation, no device work.
+ * out_index[i] indexes `others`; dist2 / cos_angle receive the two measures
+ * (may be null); *count receives the number of candidates (at most cap are
+ * written). */
+typedef struct lsdg_keyframe_pose {
+  int32_t id;
+  float mean_idepth;        /* Frame::meanIdepth */
+  double cam_to_world[8];   /* Sim3 (qx, qy, qz, qw, tx, ty, tz, s) */
+} lsdg_keyframe_pose;
+
+lsdg_status lsdg_keyframe_candidates(const lsdg_camera* cam, const lsdg_keyframe_pose* keyframe,
+                                     const lsdg_keyframe_pose* others, int32_t n, float closeness_th,
+                                     float kf_dist_weight, int32_t* out_index, double* dist2, double* cos_angle,
+                                     int32_t cap, int32_t* count);
+
 /* ----------------------------------------------------------- the system */
 
 /* lsd-slam's SlamSystem front end, the direct caller of the path
@@ -553,28 +577,44 @@ lsdg_status lsdg_constraint_search(lsdg_ctx ctx, const lsdg_constraint_candidate
  * loss detection, keyframe selection) followed, as with blockUntilMapped, by
  * the mapping iterations it waits for (updateKeyframe with the queued
  * frames, or finalizeKeyFrame + createKeyFrame on the latest tracked frame).
- * One thread, deterministic.  The pose graph, loop closures and
- * relocalisation are not part of it: once tracking is lost, later images
- * are reported as not tracked. */
+ * One thread, deterministic.  The pose graph and loop closures are not part
+ * of it.  After a loss every image goes to the relocaliser (upstream's
+ * Relocalizer, which SlamSystem::trackFrame feeds while tracking is lost):
+ * one batched SE3 plan tracks the image against every keyframe from the
+ * identity at reloc_level only (trackFrameOnPermaref's quick check); a
+ * keyframe is a candidate when the check does not diverge and good / (good
+ * + bad) of its last evaluation exceeds 1 - 0.75 (1 - MIN_GOODPERGOODBAD_PIXEL).
+ * The candidate with the lowest residual is confirmed when at least half of
+ * its neighbours in the keyframe chain (the keyframes before and after it)
+ * also pass the check from the pose it implies; the lowest-residual one of
+ * the candidate and its passing neighbours becomes the current keyframe (its
+ * stored depth reloaded into the map, loadNewCurrentKeyframe), and the image
+ * is then tracked on it as usual (LSDG_SYSTEM_RELOCALISED).  Upstream runs
+ * the checks on several threads and takes the first confirmed candidate;
+ * here the choice is deterministic. */
 typedef struct lsdg_system_s* lsdg_system;
 
-/* SlamSystem's keyframe-selection constants (lsd-slam settings). */
+/* SlamSystem's keyframe-selection constants (lsd-slam settings) and the
+ * relocaliser's. */
 typedef struct lsdg_system_params {
   float kf_dist_weight;                /* KFDistWeight = 4 */
   float kf_usage_weight;               /* KFUsageWeight = 3 */
   int32_t min_num_mapped;              /* MIN_NUM_MAPPED = 5 */
   int32_t initialization_phase_count;  /* INITIALIZATION_PHASE_COUNT = 5 */
+  int32_t relocalise;                  /* 1: relocalise after a loss (default); 0: stay lost */
+  int32_t reloc_level;                 /* QUICK_KF_CHECK_LVL = 4: the level of the candidate checks */
 } lsdg_system_params;
 void lsdg_system_params_default(lsdg_system_params* p);
 
 #define LSDG_SYSTEM_TRACKED 0
 #define LSDG_SYSTEM_LOST 1        /* tracking lost on this image */
-#define LSDG_SYSTEM_NOT_TRACKED 2 /* tracking was lost before */
+#define LSDG_SYSTEM_NOT_TRACKED 2 /* tracking was lost before and this image did not relocalise */
+#define LSDG_SYSTEM_RELOCALISED 3 /* tracking was lost before; this image relocalised and was tracked */
 
 typedef struct lsdg_system_frame {
   int32_t id;
   int32_t status;             /* LSDG_SYSTEM_* */
-  int32_t keyframe_id;        /* the keyframe it was tracked on */
+  int32_t keyframe_id;        /* the keyframe it was tracked on; -1 when not tracked */
   int32_t new_keyframe;       /* 1 when the mapping made it the new keyframe */
   float closeness_score;      /* lastTrackingClosenessScore; -1 when not evaluated */
   float keyframe_mean_idepth; /* the keyframe's meanIdepth the score used */
@@ -602,6 +642,17 @@ lsdg_status lsdg_system_track(lsdg_system s, int32_t id, const uint8_t* image, i
 lsdg_status lsdg_system_keyframes(lsdg_system s, int32_t* ids, double* cam_to_world, int32_t cap, int32_t* count);
 /* The current keyframe's depth map hypotheses (lsdg_depth_read). */
 lsdg_status lsdg_system_depth_read(lsdg_system s, lsdg_hypotheses* out);
+/* SlamSystem::findConstraintsForNewKeyframes for the system's keyframe
+ * `index` (0 = the first): lsdg_keyframe_candidates over the other
+ * keyframes (meanIdepth of each from its level-0 depth), minus its
+ * neighbours (its tracking parent and the keyframes tracked on it), then
+ * lsdg_constraint_search of every candidate against it (frame = the
+ * keyframe, initial candidate-to-frame from the two camToWorld) at
+ * `strictness`.  Writes up to cap results with their candidates' keyframe
+ * ids; *count receives the number of candidates. */
+lsdg_status lsdg_system_find_constraints(lsdg_system s, int32_t index, float closeness_th, float strictness,
+                                         lsdg_constraint_result* results, int32_t* candidate_ids, int32_t cap,
+                                         int32_t* count);
 
 /* ----------------------------------------------------------------- ingest */
 

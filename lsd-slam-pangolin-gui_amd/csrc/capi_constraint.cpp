@@ -243,3 +243,45 @@ extern "C" lsdg_status lsdg_constraint_search(lsdg_ctx c, const lsdg_constraint_
   }
   return LSDG_OK;
 }
+
+// ------------------------------------------------------------- candidates
+extern "C" lsdg_status lsdg_keyframe_candidates(const lsdg_camera* cam, const lsdg_keyframe_pose* kf,
+                                                const lsdg_keyframe_pose* others, int32_t n, float closeness_th,
+                                                float kf_dist_weight, int32_t* out_index, double* dist2,
+                                                double* cos_angle, int32_t cap, int32_t* count) {
+  if (!cam || !kf || (!others && n > 0) || n < 0 || !count || cap < 0 || (cap > 0 && !out_index) ||
+      cam->fx <= 0.0f || cam->fy <= 0.0f || !(kf_dist_weight > 0.0f))
+    return LSDG_EINVAL;
+  // the thresholds findCandidates passes to findEuclideanOverlapFrames
+  const double distTh = (double)closeness_th * 15.0 / ((double)kf_dist_weight * kf_dist_weight);
+  const double angleTh = 1.0 - 0.25 * (double)closeness_th;
+  const double fovX = 2.0 * std::atan((double)cam->w / cam->fx / 2.0);
+  const double fovY = 2.0 * std::atan((double)cam->h / cam->fy / 2.0);
+  const double cosTh = std::cos(angleTh * 0.5 * (fovX + fovY));
+  const Pose P = pose_from8(kf->cam_to_world);
+  double R[9];
+  quat_R(P.q, R);
+  const double zx = R[2], zy = R[5], zz = R[8];  // optical axis in the world
+  const double facSelf = (double)kf->mean_idepth / P.s;
+  int found = 0;
+  for (int i = 0; i < n; ++i) {
+    const Pose Q = pose_from8(others[i].cam_to_world);
+    double fac = (double)others[i].mean_idepth / Q.s;
+    if (facSelf < fac) fac = facSelf;
+    const double dx = (P.t[0] - Q.t[0]) * fac, dy = (P.t[1] - Q.t[1]) * fac, dz = (P.t[2] - Q.t[2]) * fac;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 <= distTh)) continue;
+    double S[9];
+    quat_R(Q.q, S);
+    const double c = S[2] * zx + S[5] * zy + S[8] * zz;
+    if (c < cosTh) continue;
+    if (found < cap) {
+      out_index[found] = i;
+      if (dist2) dist2[found] = d2;
+      if (cos_angle) cos_angle[found] = c;
+    }
+    ++found;
+  }
+  *count = found;
+  return LSDG_OK;
+}

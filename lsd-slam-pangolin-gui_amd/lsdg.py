@@ -175,9 +175,13 @@ class ConstraintResult(C.Structure):
                 ("candidate_on_frame", ConstraintEdge), ("frame_on_candidate", ConstraintEdge)]
 
 
+class KeyframePose(C.Structure):
+    _fields_ = [("id", C.c_int32), ("mean_idepth", C.c_float), ("cam_to_world", C.c_double * 8)]
+
+
 class SystemParams(C.Structure):
     _fields_ = [("kf_dist_weight", C.c_float), ("kf_usage_weight", C.c_float), ("min_num_mapped", C.c_int32),
-                ("initialization_phase_count", C.c_int32)]
+                ("initialization_phase_count", C.c_int32), ("relocalise", C.c_int32), ("reloc_level", C.c_int32)]
 
 
 class SystemFrame(C.Structure):
@@ -186,7 +190,7 @@ class SystemFrame(C.Structure):
                 ("frame_to_keyframe", C.c_double * 7), ("cam_to_world", C.c_double * 8), ("stats", TrackStats)]
 
 
-SYSTEM_TRACKED, SYSTEM_LOST, SYSTEM_NOT_TRACKED = 0, 1, 2
+SYSTEM_TRACKED, SYSTEM_LOST, SYSTEM_NOT_TRACKED, SYSTEM_RELOCALISED = 0, 1, 2, 3
 
 
 class PointDense(C.Structure):
@@ -333,6 +337,11 @@ def load():
         "lsdg_system_track": (i32, [vp, i32, fp, i32, C.POINTER(SystemFrame)]),
         "lsdg_system_keyframes": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_double), i32, C.POINTER(i32)]),
         "lsdg_system_depth_read": (i32, [vp, C.POINTER(Hypotheses)]),
+        "lsdg_system_find_constraints": (i32, [vp, i32, f32, f32, C.POINTER(ConstraintResult), C.POINTER(i32), i32,
+                                               C.POINTER(i32)]),
+        "lsdg_keyframe_candidates": (i32, [C.POINTER(Camera), C.POINTER(KeyframePose), C.POINTER(KeyframePose), i32,
+                                           f32, f32, C.POINTER(i32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           i32, C.POINTER(i32)]),
         "lsdg_constraint_search": (i32, [vp, C.POINTER(ConstraintCandidate), i32, f32,
                                          C.POINTER(ConstraintResult)]),
     }
@@ -600,6 +609,16 @@ class System:
         self.ctx.check(self.ctx.L.lsdg_system_depth_read(self.h, C.byref(h)), "lsdg_system_depth_read")
         return arrs
 
+    def find_constraints(self, index, closeness_th=1.0, strictness=1.5):
+        """findConstraintsForNewKeyframes for keyframe `index`: [(candidate id, ConstraintResult), ...]."""
+        cap = 64
+        res = (ConstraintResult * cap)()
+        ids = (C.c_int32 * cap)()
+        n = C.c_int32()
+        self.ctx.check(self.ctx.L.lsdg_system_find_constraints(self.h, index, closeness_th, strictness, res, ids, cap,
+                                                               C.byref(n)), "lsdg_system_find_constraints")
+        return [(ids[i], res[i]) for i in range(min(n.value, cap))]
+
     def destroy(self):
         if self.h:
             self.ctx.L.lsdg_system_destroy(self.h)
@@ -853,3 +872,22 @@ def tune_table_load(path):
     st = load().lsdg_tune_table_load(os.fsencode(path))
     if st != OK:
         raise LsdgError(f"lsdg_tune_table_load({path}) failed with status {st}")
+
+
+def keyframe_candidates(cam, keyframe, others, closeness_th=1.0, kf_dist_weight=4.0):
+    """lsdg_keyframe_candidates (host computation): keyframe and others are
+    (id, mean_idepth, cam_to_world8) tuples; returns [(index, dist2, cos), ...]."""
+    L = load()
+    n = len(others)
+    arr = (KeyframePose * max(n, 1))(*[KeyframePose(i, m, (C.c_double * 8)(*p)) for i, m, p in others])
+    kf = KeyframePose(keyframe[0], keyframe[1], (C.c_double * 8)(*keyframe[2]))
+    idx = (C.c_int32 * max(n, 1))()
+    d2 = (C.c_double * max(n, 1))()
+    cs = (C.c_double * max(n, 1))()
+    cnt = C.c_int32()
+    cc = Camera.of(cam)
+    st = L.lsdg_keyframe_candidates(C.byref(cc), C.byref(kf), arr, n, closeness_th, kf_dist_weight, idx, d2, cs, n,
+                                    C.byref(cnt))
+    if st != OK:
+        raise LsdgError(f"lsdg_keyframe_candidates failed with status {st}")
+    return [(idx[i], d2[i], cs[i]) for i in range(cnt.value)]

@@ -422,6 +422,39 @@ def ldlt(A, b):
     return np.array(x[:])
 
 
+def _rot(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def keyframe_candidates(cam, keyframe, others, closeness_th=1.0, kf_dist_weight=4.0):
+    """TrackableKeyFrameSearch::findEuclideanOverlapFrames as findCandidates
+    calls it (no FAB-MAP), restated: keyframe / others are (id, meanIdepth,
+    camToWorld8); returns [(index into others, dist2, cos), ...]."""
+    dist_th = closeness_th * 15.0 / (kf_dist_weight * kf_dist_weight)
+    angle_th = 1.0 - 0.25 * closeness_th
+    fov = 2.0 * np.arctan(cam["w"] / cam["fx"] / 2.0) + 2.0 * np.arctan(cam["h"] / cam["fy"] / 2.0)
+    cos_th = np.cos(angle_th * 0.5 * fov)
+    p = np.asarray(keyframe[2], np.float64)
+    axis = _rot(p[:4] / np.linalg.norm(p[:4]))[:, 2]
+    own = keyframe[1] / p[7]
+    out = []
+    for i, (_, mean, c2w) in enumerate(others):
+        q = np.asarray(c2w, np.float64)
+        fac = min(mean / q[7], own)
+        d = (p[4:7] - q[4:7]) * fac
+        d2 = float(d @ d)
+        if not d2 <= dist_th:
+            continue
+        c = float(_rot(q[:4] / np.linalg.norm(q[:4]))[:, 2] @ axis)
+        if c < cos_th:
+            continue
+        out.append((i, d2, c))
+    return out
+
+
 class OSystem:
     """lsd-slam's SlamSystem front end restated on the oracle (trackFrame's
     tracking, loss test and keyframe selection, then the mapping iterations
@@ -430,6 +463,11 @@ class OSystem:
     8-vectors (qx, qy, qz, qw, tx, ty, tz, s)."""
 
     KF_DIST_WEIGHT, KF_USAGE_WEIGHT, MIN_NUM_MAPPED, INIT_PHASE = 4.0, 3.0, 5, 5
+    # relocalisation: the candidate check runs SE3 at this level only, and a
+    # candidate counts when good / (good + bad) of its last evaluation exceeds
+    # 1 - 0.75 (1 - MIN_GOODPERGOODBAD_PIXEL)
+    RELOC_LEVEL = 4
+    RELOCALISE = True
 
     def __init__(self, o, cam):
         self.o, self.cam = o, cam
@@ -491,12 +529,93 @@ class OSystem:
             self.dm.update(list(self.unmapped))
             self.unmapped.pop(0)
 
+    def _reloc_track(self, kf, f, init7):
+        """SE3 from init7 at RELOC_LEVEL only; (frameToKf, good?, goodVal)."""
+        p = self.o.params
+        q = lsdg.Params.from_buffer_copy(p)
+        q.se3_max_level = self.RELOC_LEVEL + 1
+        q.se3_min_level = self.RELOC_LEVEL
+        self.o.params = q
+        try:
+            pose, st, status = self.o.se3_track(self.o.ref(kf), f, init7)
+        finally:
+            self.o.params = p
+        n = st.last_good_count + st.last_bad_count
+        val = st.last_good_count / n if n > 0 else 0.0
+        thr = 1.0 - 0.75 * (1.0 - p.min_goodpergoodbad_pixel)
+        return pose, (not st.diverged) and val > thr, float(st.last_residual)
+
+    def _relocalise(self, f):
+        """Relocalizer: the image against every keyframe from the identity;
+        of the candidates that track good, the one with the lowest residual
+        is confirmed when at least half of its chain neighbours (the
+        keyframes before and after it) also track good from the pose it
+        implies; the lowest-residual one of the candidate and its good
+        neighbours becomes the keyframe.  Returns (index, frameToKf) or None."""
+        res = [self._reloc_track(k, f, [0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0]) for k, _ in self.keyframes]
+        good = [i for i, r in enumerate(res) if r[1]]
+        if not good:
+            return None
+        c = min(good, key=lambda i: (res[i][2], i))
+        c2w_f = sim3_mul(self.keyframes[c][1], np.r_[res[c][0], 1.0])
+        best, best_val, best_pose = c, res[c][2], res[c][0]
+        n_good = n_bad = 0
+        for n in (c - 1, c + 1):
+            if n < 0 or n >= len(self.keyframes):
+                continue
+            init = sim3_mul(sim3_inv(self.keyframes[n][1]), c2w_f)
+            pose, ok, val = self._reloc_track(self.keyframes[n][0], f, list(init[:7]))
+            if ok:
+                n_good += 1
+                if val < best_val:
+                    best, best_val, best_pose = n, val, pose
+            else:
+                n_bad += 1
+        if n_good < n_bad:
+            return None
+        return best, best_pose
+
+    def find_constraints(self, index, closeness_th=1.0, strictness=1.5):
+        """findConstraintsForNewKeyframes for keyframe `index` (see
+        lsdg_system_find_constraints): [(candidate id, ConstraintResult), ...]."""
+        def mean_of(f):
+            d = f.read(lsdg.WHAT_IDEPTH, 0).ravel()
+            v = d[d != np.float32(-1.0)]
+            return float(np.cumsum(v, dtype=np.float32)[-1] / np.float32(len(v)))
+        kfs = self.keyframes
+        me, me_c2w = kfs[index]
+        parent = me.get_tracking()[0]
+        others = [i for i, (f, _) in enumerate(kfs)
+                  if i != index and f.id != parent and f.get_tracking()[0] != me.id]
+        poses = [(f.id, mean_of(f), c2w) for f, c2w in kfs]
+        cands = keyframe_candidates(self.cam, poses[index], [poses[i] for i in others], closeness_th,
+                                    self.KF_DIST_WEIGHT)
+        out = []
+        for j, _, _ in cands:
+            f, c2w = kfs[others[j]]
+            init = sim3_mul(sim3_inv(me_c2w), c2w)
+            out.append((f.id, self.o.constraint_test(self.o.ref(me), self.o.ref(f), init, strictness)))
+        return out
+
     def track(self, fid, image):
         info = dict(id=fid, status=0, keyframe_id=-1, new_keyframe=0, closeness_score=-1.0)
-        if not self.good:
-            info["status"] = 2
-            return info
         f = self.o.frame(fid, self.cam, np.asarray(image, np.float32))
+        relocalised = False
+        if not self.good:
+            r = self._relocalise(f) if self.RELOCALISE else None
+            if r is None:
+                info["status"] = 2
+                return info
+            # loadNewCurrentKeyframe: the keyframe's depth becomes the map again
+            k, pose = r
+            kf, c2w = self.keyframes[k]
+            self.dm.set_prior(kf, kf.read(lsdg.WHAT_IDEPTH, 0), kf.read(lsdg.WHAT_IDEPTH_VAR, 0), 20)
+            self.kf, self.kf_c2w = kf, c2w
+            self.last_c2w = sim3_mul(c2w, np.r_[pose, 1.0])
+            self.unmapped = []
+            self.create = False
+            self.good = True
+            relocalised = True
         my_create = self.create
         ref = self.o.ref(self.kf)
         rel = sim3_mul(sim3_inv(self.kf_c2w), self.last_c2w)
@@ -508,6 +627,8 @@ class OSystem:
             self.good = False
             info["status"] = 1
             return info
+        if relocalised:
+            info["status"] = 3
         c2w = sim3_mul(self.kf_c2w, np.r_[pose, 1.0])
         f.set_cam_to_world(c2w)
         self.last_c2w = c2w

@@ -109,3 +109,63 @@ def test_unrelated_scenes_are_rejected(o):
     r = o.constraint_test(ra, rb, list(synth.IDENTITY7) + [1.0], 1.5)
     assert r.accepted == 0 and r.rejected_at >= 1
     assert r.error[r.rejected_at - 1] > [3000, 4000, 6000][r.rejected_at - 1] * 1.5
+
+
+# ----------------------------------------------------- keyframe candidates
+def _yaw(deg):
+    a = np.deg2rad(deg) / 2.0
+    return [0.0, np.sin(a), 0.0, np.cos(a)]
+
+
+def _cam():
+    return {"w": 320, "h": 240, "fx": 262.5, "fy": 262.5, "cx": 159.5, "cy": 119.5}
+
+
+def test_keyframe_candidates_closed_forms():
+    """findEuclideanOverlapFrames' two tests on their boundaries: squared
+    distance closeness * 15 / KFDistWeight^2 after scaling by meanIdepth /
+    scale (the smaller factor of the two keyframes), and the optical axes
+    within (1 - 0.25 closeness) of the mean field of view."""
+    import lsdg
+    cam = _cam()
+    me = (0, 0.5, [0, 0, 0, 1, 0, 0, 0, 1.0])
+    dist_th = 15.0 / 16.0
+    t_edge = np.sqrt(dist_th) / 0.5            # |t * 0.5|^2 = dist_th
+    fov = 2 * np.arctan(320 / 262.5 / 2) + 2 * np.arctan(240 / 262.5 / 2)  # fov_x + fov_y
+    ang_edge = np.rad2deg(0.75 * 0.5 * fov)    # closeness 1: angle_th = 0.75
+    others = [
+        (1, 0.5, [0, 0, 0, 1, 0, 0, 0, 1.0]),                        # same pose
+        (2, 0.5, [0, 0, 0, 1, 0.99 * t_edge, 0, 0, 1.0]),             # just inside
+        (3, 0.5, [0, 0, 0, 1, 1.01 * t_edge, 0, 0, 1.0]),             # just outside
+        (4, 0.5, _yaw(0.98 * ang_edge) + [0, 0, 0, 1.0]),             # axis just inside
+        (5, 0.5, _yaw(1.02 * ang_edge) + [0, 0, 0, 1.0]),             # axis just outside
+        (6, 0.5, [0, 0, 0, 1, 1.5 * t_edge, 0, 0, 2.0]),              # scale 2: factor 0.25 -> inside
+        (7, 2.0, [0, 0, 0, 1, 1.5 * t_edge, 0, 0, 1.0]),              # larger own factor: min keeps 0.5 -> outside
+    ]
+    got = lsdg.keyframe_candidates(cam, me, others)
+    assert [i for i, _, _ in got] == [0, 1, 3, 5]
+    d2 = {i: d for i, d, _ in got}
+    assert d2[0] == 0.0 and abs(d2[1] - 0.99 ** 2 * dist_th) < 1e-12 and abs(d2[5] - (1.5 * t_edge * 0.25) ** 2) < 1e-12
+    assert abs(dict((i, c) for i, _, c in got)[3] - np.cos(np.deg2rad(0.98 * ang_edge))) < 1e-12
+    assert [i for i, _, _ in pyoracle.keyframe_candidates(cam, me, others)] == [0, 1, 3, 5]
+
+
+def test_keyframe_candidates_match_the_restatement():
+    import lsdg
+    rng = np.random.default_rng(5)
+    cam = _cam()
+    for trial in range(20):
+        def pose():
+            q = rng.normal(size=4) * [0.1, 0.3, 0.1, 1.0]
+            q /= np.linalg.norm(q)
+            return list(q) + list(rng.normal(size=3) * 1.5) + [float(rng.uniform(0.5, 2.0))]
+        def mean():  # the C-ABI carries meanIdepth as a float
+            return float(np.float32(rng.uniform(0.2, 1.0)))
+        me = (0, mean(), pose())
+        others = [(k, mean(), pose()) for k in range(1, 30)]
+        closeness = float(rng.uniform(0.5, 2.0))
+        a = lsdg.keyframe_candidates(cam, me, others, closeness)
+        b = pyoracle.keyframe_candidates(cam, me, others, closeness)
+        assert [i for i, _, _ in a] == [i for i, _, _ in b], trial
+        for (_, d1, c1), (_, d2, c2) in zip(a, b):
+            assert abs(d1 - d2) <= 1e-12 * max(1.0, d2) and abs(c1 - c2) <= 1e-12

@@ -10,6 +10,7 @@ lsd-slam, which is not in the tree: parity unpinned for them."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 import lsdg
 import pyoracle
@@ -46,3 +47,48 @@ def test_oracle_system_follows_a_synthetic_sequence():
     # keyframes are rescaled to unit mean inverse depth: the scale of their
     # camToWorld is the scene's mean depth (about 2-4 m here)
     assert all(1.0 < c2w[7] < 5.0 for _, c2w in s.keyframes[1:])
+
+
+# ------------------------------------------------------------ relocalisation
+def _run_twin(kind, relocalise=True):
+    import relocalise_scenarios as rs
+    seq = rs.sequence()
+    o = pyoracle.OSystem(pyoracle.Oracle(2), seq["cam"])
+    o.INIT_PHASE = 1
+    o.RELOCALISE = relocalise
+    o.init_gt(0, seq["images"][0], seq["idepth"], seq["var"])
+    out = []
+    for k, idx in enumerate(rs.order(kind)):
+        out.append((idx, o.track(k + 1, rs.image(seq, idx))))
+    return seq, o, out
+
+
+@pytest.mark.parametrize("kind", ["resume", "kidnap"])
+def test_twin_relocalises_onto_the_overlapping_keyframe(kind):
+    """After two black images the restatement reports the loss, then the
+    next image relocalises: on the latest keyframe when the camera resumes,
+    on the first keyframe when it reappears 0.9 m back; the relocalised
+    camToWorld is within millimetres of the ground truth."""
+    seq, o, out = _run_twin(kind)
+    statuses = [e["status"] for _, e in out]
+    k_lost = statuses.index(1)
+    assert out[k_lost][0] is None and statuses[k_lost + 1] == 2
+    k_rel = k_lost + 2
+    idx, e = out[k_rel]
+    assert e["status"] == 3
+    ids = [f.id for f, _ in o.keyframes]
+    if kind == "kidnap":
+        assert e["keyframe_id"] == ids[0]
+    else:
+        assert e["keyframe_id"] == out[k_lost - 1][1]["keyframe_id"]
+    assert np.linalg.norm(np.asarray(e["cam_to_world"][4:7]) - seq["poses"][idx][4:7]) < 5e-3
+    for idx, e in out[k_rel + 1:]:
+        assert e["status"] == 0
+        assert np.linalg.norm(np.asarray(e["cam_to_world"][4:7]) - seq["poses"][idx][4:7]) < 1e-2
+
+
+def test_twin_without_relocalisation_stays_lost():
+    _, _, out = _run_twin("resume", relocalise=False)
+    statuses = [e["status"] for _, e in out]
+    k_lost = statuses.index(1)
+    assert all(s == 2 for s in statuses[k_lost + 1:])
