@@ -391,9 +391,13 @@ extern "C" lsdg_status lsdg_frame_invalidate(lsdg_frame f) {
 lsdg_status lsdg::frame_depth_from_level0(lsdg_frame f) {
   lsdg_ctx c = f->ctx;
   CK(c, launch_pyr_depth(f, c->stream));
+  frame_depth_written(f);
+  return LSDG_OK;
+}
+
+void lsdg::frame_depth_written(lsdg_frame f) {
   f->hasDepth = true;
   f->depthVersion++;
-  return LSDG_OK;
 }
 
 extern "C" lsdg_status lsdg_frame_set_depth(lsdg_frame f, const float* idepth, const float* var) {

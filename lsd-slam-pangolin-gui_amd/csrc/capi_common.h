@@ -7,6 +7,9 @@ namespace lsdg {
 // Records `what` (and the HIP error) as the context's last error; returns st.
 lsdg_status fail(lsdg_ctx c, lsdg_status st, const char* what, hipError_t e = hipSuccess);
 lsdg_status frame_depth_from_level0(lsdg_frame f);
+// Marks a frame's depth pyramid as written (every level) by a kernel already
+// queued, e.g. launch_fill_regularize with the frame as setDepth.
+void frame_depth_written(lsdg_frame f);
 // A frame of camera cam with every array allocated and nothing valid yet.
 lsdg_status frame_alloc(lsdg_ctx c, int32_t id, const lsdg_camera* cam, int32_t precision, lsdg_frame* out);
 // Builds the pyramid data in `need` (fd_bit flags) that the frame does not

@@ -279,8 +279,7 @@ extern "C" lsdg_status lsdg_depth_update(lsdg_depthmap dm, const lsdg_frame* ref
   CK(c, launch_observe(dm, P, dm->h_refs, n, dm->h_refById, (int)byId.size(), offset, 0, n - 1,
                        (float)kf->numFramesTrackedOnThis, (float)kf->numMappedOnThis, c->stream));
   CK(c, launch_fill_regularize(dm, P, P.val_sum_min_for_keep, kf, c->stream));
-  st = frame_depth_from_level0(kf);
-  if (st != LSDG_OK) return st;
+  frame_depth_written(kf);
   // algorithmic bytes of one update (181 B per pixel: observe 70, fill holes 66,
   // regularize 37, setDepth 8)
   prof_end(c, pm, 181.0 * dm->g.w * dm->g.h);
@@ -331,8 +330,7 @@ extern "C" lsdg_status lsdg_depth_finalize_keyframe(lsdg_depthmap dm) {
   lsdg_status st0 = frame_require(dm->kf, KF_DATA);
   if (st0 != LSDG_OK) return st0;
   CK(c, launch_fill_regularize(dm, c->P, c->P.val_sum_min_for_keep, dm->kf, c->stream));
-  lsdg_status st = frame_depth_from_level0(dm->kf);
-  if (st != LSDG_OK) return st;
+  frame_depth_written(dm->kf);
   CK(c, hipStreamSynchronize(c->stream));
   return LSDG_OK;
 }

@@ -66,13 +66,16 @@ struct Quad {
 };
 __device__ inline Quad fetch_quad(const float* __restrict__ m, float x, float y, int w, int h) {
   // the linear index, clamped to [0, w (h - 1) - 2]: unchanged for
-  // 0 <= ix <= w - 2, 0 <= iy <= h - 2 (every blended position)
-  const unsigned lin = (unsigned)(int)x + (unsigned)(int)y * (unsigned)w;
+  // 0 <= ix <= w - 2, 0 <= iy <= h - 2 (every blended position; sizes are
+  // at most 16384, so the 24-bit product is exact there), and byte offsets
+  // in 32 bits (an image is far below 4 GiB)
+  const unsigned lin = (unsigned)(int)x + __umul24((unsigned)(int)y, (unsigned)w);
   const unsigned off = (unsigned)min(max((int)lin, 0), w * (h - 1) - 2);
-  const float* bp = m + off;
+  const char* base = (const char*)m;
+  const unsigned top = off * 4u, bottom = top + (unsigned)w * 4u;
   Quad q;
-  q.t = *(GlobalTexel2)bp;
-  q.b = *(GlobalTexel2)(bp + w);
+  q.t = *(GlobalTexel2)(base + top);
+  q.b = *(GlobalTexel2)(base + bottom);
   return q;
 }
 // dinterp's bilinear blend of fetched texels (same expression and order).
@@ -135,17 +138,15 @@ struct ObsCtx {
   const float4* kfGrad;
 };
 
-__device__ inline bool make_and_check_epl(const ObsCtx& o, const lsdg_params& P, int x, int y, const StereoRef& ref,
-                                          float* pepx, float* pepy) {
-  int idx = x + y * o.g.w;
-  float epx = -o.g.fx * ref.t_t2o[0] + ref.t_t2o[2] * ((float)x - o.g.cx);
-  float epy = -o.g.fy * ref.t_t2o[1] + ref.t_t2o[2] * ((float)y - o.g.cy);
+// gx, gy: the keyframe image's central differences at (x, y),
+// I(x+1, y) - I(x-1, y) and I(x, y+1) - I(x, y-1).
+__device__ inline bool make_and_check_epl(const DepthGeom& g, const lsdg_params& P, int x, int y, float gx, float gy,
+                                          const StereoRef& ref, float* pepx, float* pepy) {
+  float epx = -g.fx * ref.t_t2o[0] + ref.t_t2o[2] * ((float)x - g.cx);
+  float epy = -g.fy * ref.t_t2o[1] + ref.t_t2o[2] * ((float)y - g.cy);
   if (isnan(epx + epy)) return false;
   float eplLengthSquared = epx * epx + epy * epy;
   if (eplLengthSquared < P.min_epl_length_squared) return false;
-  const float* I = o.kfImg;
-  float gx = I[idx + 1] - I[idx - 1];
-  float gy = I[idx + o.g.w] - I[idx - o.g.w];
   float eplGradSquared = gx * epx + gy * epy;
   eplGradSquared = eplGradSquared * eplGradSquared / eplLengthSquared;
   if (eplGradSquared < P.min_epl_grad_squared) return false;
@@ -386,7 +387,7 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
       walk(wx, wy, wj, j0, incx, incy);
       float cpx = wx, cpy = wy;
       int loop = j0;
-      float val_m2 = W[1], val_m1 = W[2], val = W[3], val_p1 = W[4], val_p2;
+      float val_m2 = W[1], val_m1 = W[2], val = W[3], val_p1 = W[4];
       float e1A = NAN, e2A = NAN, e3A = NAN, e4A = NAN, e5A = NAN;
       float e1B = NAN, e2B = NAN, e3B = NAN, e4B = NAN, e5B = NAN;
       float eeLast = -1.0f;
@@ -417,13 +418,18 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
       // the wave's outstanding-load count would differ between paths and the
       // compiler would wait for all fetches at every step.
       bool alive = true;
+      // the step bound, held in a vector register: as a scalar the
+      // compiler reloads it from the kernel arguments at every step (a
+      // scalar-memory round trip per step)
+      int stepLimit;
+      asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
       for (;;) {
 #pragma unroll
         for (int k = 0; k < STEREO_AHEAD; ++k) {
           // the reference's while condition, without short-circuit branches
           const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
-          const bool act = alive & go & (loop < maxSteps);
-          val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
+          const bool act = alive & go & (loop < stepLimit);
+          const float val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
           ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
           lx += incx;
           ly += incy;
@@ -444,8 +450,10 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
             e4B = val_m1 - rv_m1; ee += e4B * e4B;
             e5B = val_m2 - rv_m2; ee += e5B * e5B;
           }
-          const bool look = act & (loop == j1);  // the next lane's first step: only the error after a best
-          const bool step = act & (loop != j1);
+          // the next lane's first step: only the error after a best (a
+          // single lane's chunk ends at maxSteps, which act excludes)
+          const bool look = G > 1 && (act & (loop == j1));
+          const bool step = G > 1 ? act & (loop != j1) : act;
           const bool nb = step & (ee < sc.best);                 // new best
           const bool post = act & !nb & bestWasLast;             // the step after the best
           const bool ns = step & !nb & (ee < sc.second);         // new second
@@ -472,9 +480,11 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
           val_m1 = val;
           val = val_p1;
           val_p1 = val_p2;
-          cpx = step ? cpx + incx : cpx;
-          cpy = step ? cpy + incy : cpy;
-          loop += step ? 1 : 0;
+          // (one lane per line: the position and count of a lane that has
+          // left it are never read again, so they advance regardless)
+          cpx = (G == 1 || step) ? cpx + incx : cpx;
+          cpy = (G == 1 || step) ? cpy + incy : cpy;
+          loop += (G == 1 || step) ? 1 : 0;
         }
         if (!__any(alive)) break;
       }
@@ -646,10 +656,16 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
     a.obs[x + y * w] = LSDG_OBS_NONE;
   } else if (x < w && y < h) {
     idx = x + y * w;
-    ObsCtx o{a.g, a.kfImg, a.kfGrad};
+    // every per-pixel input is read up front (all in bounds: the pixel is
+    // interior), so the tests below wait on memory once, and once more for
+    // the reference's tracking mask
     const bool has = a.cur.valid[idx] != 0;
     const int bl = a.cur.bl[idx];
     const float mg = a.kfMaxg[idx];
+    const float nsid = a.cur.nsid[idx];
+    const float* I = a.kfImg;
+    const float gx = I[idx + 1] - I[idx - 1];
+    const float gy = I[idx + w] - I[idx - w];
     uint8_t code = LSDG_OBS_NONE;
     if (has && mg < P.min_abs_grad_decrease) {
       a.cur.valid[idx] = 0;
@@ -661,7 +677,7 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
       if (!has) {
         ri = a.kfReactivated ? a.newestIdx : a.oldestIdx;
       } else if (!a.kfReactivated) {
-        const int k = (int)a.cur.nsid[idx] - a.refByIdOffset;
+        const int k = (int)nsid - a.refByIdOffset;
         if (k >= a.nRefById) {
           code = LSDG_OBS_SKIP_ALREADY_GOOD;
           go = false;
@@ -674,7 +690,7 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
       if (go) {
         const StereoRef& ref = obs_ref<INL>(a, ri);
         if (!was_good_gate(ref, x, y, w)) code = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
-        else if (!make_and_check_epl(o, P, x, y, ref, &epx, &epy)) code = LSDG_OBS_EPL_FAIL;
+        else if (!make_and_check_epl(a.g, P, x, y, gx, gy, ref, &epx, &epy)) code = LSDG_OBS_EPL_FAIL;
         else kind = has ? 2 : 1;
       }
     }
@@ -706,22 +722,27 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
 }
 
 #ifndef LSDG_STEREO_GROUP
-#define LSDG_STEREO_GROUP 2
+#define LSDG_STEREO_GROUP 1
 #endif
-// Lanes per observeDepthCreate search (a power of two up to 64).
+// Lanes per observeDepthCreate search (a power of two up to 64).  One lane
+// per search is the fastest measured: lanes splitting a line repeat its
+// setup and each chunk's warm-up, and the stereo pass is bound by how long
+// its longest waves run, not by how many there are.
 constexpr int STEREO_GROUP = LSDG_STEREO_GROUP;
 static_assert(STEREO_GROUP >= 1 && STEREO_GROUP <= 64 && (STEREO_GROUP & (STEREO_GROUP - 1)) == 0,
               "lanes per search");
+#ifndef LSDG_STEREO_MIN_BLOCKS
+#define LSDG_STEREO_MIN_BLOCKS 3
+#endif
+// Stereo blocks per CU the register budget is held to (3: 168 VGPRs, no spills).
+constexpr int STEREO_MIN_BLOCKS = LSDG_STEREO_MIN_BLOCKS;
 
-// observeDepthCreate item: G lanes (creates search the whole inverse-depth
-// range, so their lines are long); `blk` indexes segcap G / 256 blocks.
+// observeDepthCreate item `i` of segment `seg`, searched by the G lanes of a
+// group (lane g); creates search the whole inverse-depth range, so their
+// lines are long.
 template <bool INL, int G>
 __device__ inline void observe_create(const ObsArgs& a, const lsdg_params& P, const int4* __restrict__ work,
-                                      const int* __restrict__ count, int segcap, int blk) {
-  const int seg = blockIdx.y;
-  const int i = blk * (256 / G) + threadIdx.x / G, g = threadIdx.x % G;
-  const int nc = count[(2 * seg) * OBS_COUNTER_STRIDE];
-  if (i >= nc) return;  // the whole group
+                                      int segcap, int seg, int i, int g) {
   const int4 item = work[(size_t)seg * segcap + i];
   const int w = a.g.w;
   const int idx = item.x, x = idx % w, y = idx / w;
@@ -748,15 +769,11 @@ __device__ inline void observe_create(const ObsArgs& a, const lsdg_params& P, co
   a.obs[idx] = code;
 }
 
-// observeDepthUpdate item, one thread (short lines around the prior); `blk`
-// indexes segcap / 256 blocks; item i of a segment is slot segcap - 1 - i.
+// observeDepthUpdate item `i` of segment `seg` (slot segcap - 1 - i), one
+// lane (short lines around the prior).
 template <bool INL>
 __device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, const int4* __restrict__ work,
-                                      const int* __restrict__ count, int segcap, int blk) {
-  const int seg = blockIdx.y;
-  const int i = blk * 256 + threadIdx.x;
-  const int nu = count[(2 * seg + 1) * OBS_COUNTER_STRIDE];
-  if (i >= nu) return;
+                                      int segcap, int seg, int i) {
   const int4 item = work[(size_t)seg * segcap + segcap - 1 - i];
   const int w = a.g.w;
   const int idx = item.x, x = idx % w, y = idx / w;
@@ -820,15 +837,51 @@ __device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, co
   a.obs[idx] = code;
 }
 
-// Both searches in one launch, so that the short update searches fill the
-// gaps the long create searches leave.  Grid: OBS_SEGS x (segcap G / 256 +
-// segcap / 256) blocks, blockIdx.y the segment; create blocks first.
+// Both searches in one persistent launch.  The work is cut into wave-sized
+// chunks: a create chunk is 64 / G consecutive creates of one segment, an
+// update chunk 64 consecutive updates; chunks are numbered creates first
+// (segment by segment), then updates, and wave v of the grid's V waves takes
+// chunks v, v + V, v + 2V, ...  The grid is what the device holds at once,
+// so every wave starts at launch: the long create searches spread over all
+// SIMDs, the short update searches fill in beside them, and no wave is
+// dispatched only to find its part of the list empty.
 template <bool INL, int G>
-__global__ __launch_bounds__(256) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
+__global__ __launch_bounds__(256, STEREO_MIN_BLOCKS) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
                                                         const int* __restrict__ count, int segcap) {
-  const int createBlocks = segcap / 256 * G;
-  if ((int)blockIdx.x < createBlocks) observe_create<INL, G>(a, P, work, count, segcap, blockIdx.x);
-  else observe_update<INL>(a, P, work, count, segcap, blockIdx.x - createBlocks);
+  constexpr int PER = 64 / G;  // creates per chunk
+  const int lane = (int)(threadIdx.x & 63);
+  const int v = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), V = (int)(gridDim.x * 4);
+  // Chunk counts, one per lane: lane l < OBS_SEGS holds segment l's create
+  // chunks, lane OBS_SEGS + l its update chunks (one vector load for the
+  // whole table), and their inclusive prefix over lanes 0 .. 2 OBS_SEGS - 1.
+  int nItems = 0, pre = 0;
+  if (lane < 2 * OBS_SEGS) {
+    const int s = lane % OBS_SEGS, k = lane / OBS_SEGS;
+    nItems = count[(2 * s + k) * OBS_COUNTER_STRIDE];
+    pre = k == 0 ? (nItems + PER - 1) / PER : (nItems + 63) / 64;
+  }
+#pragma unroll
+  for (int d = 1; d < 2 * OBS_SEGS; d <<= 1) {
+    const int o = __shfl_up(pre, d, 64);
+    if (lane >= d) pre += o;
+  }
+  const int nChunk = __shfl(pre, 2 * OBS_SEGS - 1, 64);
+  for (int c = v; c < nChunk; c += V) {
+    // the table lane of chunk c: the first whose prefix exceeds c (prefixes
+    // do not decrease, so the lanes at or below c are the ones before it)
+    const int tl = (int)__popcll(__ballot(lane < 2 * OBS_SEGS && pre <= c));
+    const int q = c - (tl > 0 ? __shfl(pre, tl - 1, 64) : 0);
+    const int n = __shfl(nItems, tl, 64);
+    const bool create = tl < OBS_SEGS;
+    const int seg = tl % OBS_SEGS;
+    if (create) {
+      const int i = q * PER + lane / G;
+      if (i < n) observe_create<INL, G>(a, P, work, segcap, seg, i, lane % G);  // whole groups
+    } else {
+      const int i = q * 64 + lane;
+      if (i < n) observe_update<INL>(a, P, work, segcap, seg, i);
+    }
+  }
 }
 
 // ---------------------------------------------------------- fill holes
@@ -993,15 +1046,13 @@ __device__ __forceinline__ Hyp regularize_pixel(const RegTile<N>& t, int cx, int
 }
 
 // Frame::setDepth for one pixel: the smoothed idepth / variance of a valid
-// hypothesis not behind the camera, else -1 for both.
-__device__ inline void set_depth_pixel(const Hyp& d, int i, float* __restrict__ idepth, float* __restrict__ var) {
-  if (d.valid && (double)d.dv.z >= -0.05) {
-    idepth[i] = unzero(d.dv.z);
-    var[i] = unzero(d.dv.w);
-  } else {
-    idepth[i] = -1.0f;
-    var[i] = -1.0f;
-  }
+// hypothesis not behind the camera, else -1 for both; returns what it wrote.
+__device__ inline float2 set_depth_pixel(const Hyp& d, int i, float* __restrict__ idepth, float* __restrict__ var) {
+  float2 r = make_float2(-1.0f, -1.0f);
+  if (d.valid && (double)d.dv.z >= -0.05) r = make_float2(unzero(d.dv.z), unzero(d.dv.w));
+  idepth[i] = r.x;
+  var[i] = r.y;
+  return r;
 }
 
 // ---------------------------------------------------------- regularize
@@ -1050,10 +1101,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_regularize(DepthGeom g, HypBuf
 // stores the filled map (`filled`: the reference's other map afterwards) and
 // the regularized one (`out`), and, when kfIdepth is set, Frame::setDepth's
 // level-0 idepth / variance of the regularized map.
+static_assert(PYR == 5, "a 16x16 tile holds whole 2^(PYR-1) blocks of level-0 pixels");
 __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, HypBuf in, HypBuf filled, HypBuf out,
                                                                   const float* __restrict__ maxg, lsdg_params P,
-                                                                  int validityTH, float* __restrict__ kfIdepth,
-                                                                  float* __restrict__ kfVar) {
+                                                                  int validityTH, PyrDepthArgs kf) {
   __shared__ FillTile<24> a;  // input, pixels and a 4-pixel halo
   __shared__ RegTile<20> f;   // filled, pixels and a 2-pixel halo
   constexpr int AITER = (24 * 24 + TILE_THREADS - 1) / TILE_THREADS;
@@ -1119,18 +1170,52 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, H
     f.put(ty, tx, v, c, e, rdv);
   }
   __syncthreads();
-  if (!inside) return;
-  const int cx = lx + 2, cy = ly + 2;
-  // the filled hypothesis: DepthMapPixelHypothesis(obs, VAR_RANDOM_INIT_INITIAL, 0) where created
-  Hyp d = as_hyp(self);
-  if (f.valid[cy][cx] && !d.valid) {
-    const float2 e = f.xy[cy][cx];
-    d = make_hyp(e.x, e.y, 0);
+  float2 d0 = make_float2(-1.0f, -1.0f);
+  if (inside) {
+    const int cx = lx + 2, cy = ly + 2;
+    // the filled hypothesis: DepthMapPixelHypothesis(obs, VAR_RANDOM_INIT_INITIAL, 0) where created
+    Hyp d = as_hyp(self);
+    if (f.valid[cy][cx] && !d.valid) {
+      const float2 e = f.xy[cy][cx];
+      d = make_hyp(e.x, e.y, 0);
+    }
+    store_hyp(filled, idx, d);
+    const Hyp r = regularize_pixel(f, cx, cy, x, y, w, h, d, P, 0, validityTH);
+    store_hyp(out, idx, r);
+    if (kf.id[0]) d0 = set_depth_pixel(r, idx, kf.id[0], kf.var[0]);
   }
-  store_hyp(filled, idx, d);
-  const Hyp r = regularize_pixel(f, cx, cy, x, y, w, h, d, P, 0, validityTH);
-  store_hyp(out, idx, r);
-  if (kfIdepth) set_depth_pixel(r, idx, kfIdepth, kfVar);
+  if (!kf.id[0]) return;  // (uniform)
+  // The keyframe's depth pyramid: each level-l pixel fuses the 2^l x 2^l
+  // level-0 pixels below it, which lie in this tile, so levels 1-4 follow
+  // from the tile's setDepth values in shared memory (level by level, in
+  // place: a level's reads end at a barrier before its writes).
+  __shared__ float pid[16][17], pvar[16][17];
+  pid[ly][lx] = d0.x;
+  pvar[ly][lx] = d0.y;
+  __syncthreads();
+#pragma unroll
+  for (int l = 1, n = 8; l < PYR; ++l, n >>= 1) {
+    const int tx = threadIdx.x % n, ty = threadIdx.x / n;
+    float oi = -1.0f, ov = -1.0f;
+    if ((int)threadIdx.x < n * n) {
+      const int px = blockIdx.x * n + tx, py = blockIdx.y * n + ty;
+      if (px < kf.w[l] && py < kf.h[l]) {
+        const float ia[4] = {pid[2 * ty][2 * tx], pid[2 * ty][2 * tx + 1], pid[2 * ty + 1][2 * tx],
+                             pid[2 * ty + 1][2 * tx + 1]};
+        const float va[4] = {pvar[2 * ty][2 * tx], pvar[2 * ty][2 * tx + 1], pvar[2 * ty + 1][2 * tx],
+                             pvar[2 * ty + 1][2 * tx + 1]};
+        fuse4(ia, va, oi, ov);
+        kf.id[l][px + py * kf.w[l]] = oi;
+        kf.var[l][px + py * kf.w[l]] = ov;
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < n * n) {
+      pid[ty][tx] = oi;
+      pvar[ty][tx] = ov;
+    }
+    __syncthreads();
+  }
 }
 
 // -------------------------------------------------------- Frame::setDepth
@@ -1465,7 +1550,18 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
   int* count = dm->d_work_count + dm->obsParity * set;
   int* other = dm->d_work_count + (1 - dm->obsParity) * set;
   dm->obsParity ^= 1;
-  const dim3 gs(segBlocks * (STEREO_GROUP + 1), OBS_SEGS);
+  // stereo grid: the blocks the device holds at once, at most one wave per
+  // chunk the lists can hold
+  int& resident = dm->stereoBlocks[inl ? 0 : 1];
+  if (resident == 0) {
+    int perCU = 0;
+    e = inl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_observe_stereo<true, STEREO_GROUP>, 256, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_observe_stereo<false, STEREO_GROUP>, 256, 0);
+    if (e != hipSuccess) return e;
+    resident = perCU > 0 ? perCU * dm->ctx->num_cus : 1;
+  }
+  const long long maxChunks = (long long)OBS_SEGS * ((segcap + 64 / STEREO_GROUP - 1) / (64 / STEREO_GROUP) + (segcap + 63) / 64);
+  const dim3 gs((unsigned)std::min<long long>(resident, (maxChunks + 3) / 4));
   if (inl) {
     k_observe_gate<true><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
     k_observe_stereo<true, STEREO_GROUP><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
@@ -1486,9 +1582,17 @@ hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool remove
 hipError_t launch_fill_regularize(lsdg_depthmap dm, const lsdg_params& P, int th, lsdg_frame setDepth,
                                   hipStream_t s) {
   const int in = dm->cur, fill = dm->oth, reg = 3 - dm->cur - dm->oth;
+  PyrDepthArgs kf{};
+  if (setDepth) {
+    for (int l = 0; l < PYR; ++l) {
+      kf.id[l] = setDepth->d_idepth[l];
+      kf.var[l] = setDepth->d_ivar[l];
+      kf.w[l] = setDepth->w[l];
+      kf.h[l] = setDepth->h[l];
+    }
+  }
   k_fill_regularize<<<grid16(dm->g.w, dm->g.h), dim3(TILE_THREADS), 0, s>>>(
-      dm->g, dm->buf[in], dm->buf[fill], dm->buf[reg], dm->kf->d_maxg[0], P, th,
-      setDepth ? setDepth->d_idepth[0] : nullptr, setDepth ? setDepth->d_ivar[0] : nullptr);
+      dm->g, dm->buf[in], dm->buf[fill], dm->buf[reg], dm->kf->d_maxg[0], P, th, kf);
   dm->oth = fill;
   dm->cur = reg;
   return hipGetLastError();

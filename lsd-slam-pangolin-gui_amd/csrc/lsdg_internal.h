@@ -100,6 +100,40 @@ struct DepthGeom {
 };
 
 // Device tables of a batched pyramid rebuild.
+// A frame's inverse-depth pyramid (levels 0 .. PYR - 1, each half the one
+// above, rounded down).
+struct PyrDepthArgs {
+  float* id[PYR];
+  float* var[PYR];
+  int w[PYR], h[PYR];
+};
+
+// Frame::buildIDepthAndIDepthVar for one pixel: inverse-variance fusion of
+// its 4 children (order: (0,0), (1,0), (0,1), (1,1)); (-1, -1) when none has
+// a positive variance.
+__device__ inline void fuse4(const float* ida, const float* va, float& oid, float& ov) {
+  float isum = 0.f, ivsum = 0.f;
+  int num = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float v = va[k];
+    if (v > 0.f) {
+      float iv = 1.0f / v;
+      ivsum += iv;
+      isum += iv * ida[k];
+      num++;
+    }
+  }
+  if (num > 0) {
+    float depth = ivsum / isum;
+    oid = 1.0f / depth;
+    ov = (float)num / ivsum;
+  } else {
+    oid = -1.f;
+    ov = -1.f;
+  }
+}
+
 struct PyrBatch {
   const void* img_args;
   const void* grad_args;
@@ -244,6 +278,7 @@ struct lsdg_depthmap_s {
   int4* d_work = nullptr;    // epipolar-search work items, one slot per pixel
   int* d_work_count = nullptr;  // two sets of per-segment (create, update) item counters
   int obsParity = 0;            // counter set of the next observe pass
+  int stereoBlocks[2] = {0, 0};  // resident blocks of the stereo pass (refs inline, in memory), 0 = not yet queried
   lsdg::StereoRef* h_refs = nullptr;  // pinned staging
   int* h_refById = nullptr;
   // propagation scratch
@@ -327,7 +362,8 @@ hipError_t launch_regularize(lsdg_depthmap dm, const lsdg_params& P, bool remove
                              hipStream_t s);  // swaps buffers
 // regularizeDepthMapFillHoles then regularizeDepthMap(false, th) in one pass,
 // leaving the current and other maps as the two passes would; with setDepth
-// non-null, also launch_set_depth(setDepth) (level 0 only)
+// non-null, also launch_set_depth(setDepth) and the frame's inverse-depth
+// levels 1-4 (what launch_pyr_depth would write after it)
 hipError_t launch_fill_regularize(lsdg_depthmap dm, const lsdg_params& P, int th, lsdg_frame setDepth,
                                   hipStream_t s);
 hipError_t launch_set_depth(lsdg_depthmap dm, lsdg_frame f, hipStream_t s);

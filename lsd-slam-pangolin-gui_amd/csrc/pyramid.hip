@@ -343,36 +343,6 @@ __global__ __launch_bounds__(256) void k_pyr_grads_batch(const PyrGradArgs* __re
   pyr_grads_block(args[f], blockIdx.x - off[f]);
 }
 
-struct PyrDepthArgs {
-  float* id[PYR];
-  float* var[PYR];
-  int w[PYR], h[PYR];
-};
-
-// Inverse-variance fusion of 4 children (order: (0,0), (1,0), (0,1), (1,1)).
-__device__ inline void fuse4(const float* ida, const float* va, float& oid, float& ov) {
-  float isum = 0.f, ivsum = 0.f;
-  int num = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    float v = va[k];
-    if (v > 0.f) {
-      float iv = 1.0f / v;
-      ivsum += iv;
-      isum += iv * ida[k];
-      num++;
-    }
-  }
-  if (num > 0) {
-    float depth = ivsum / isum;
-    oid = 1.0f / depth;
-    ov = (float)num / ivsum;
-  } else {
-    oid = -1.f;
-    ov = -1.f;
-  }
-}
-
 __global__ __launch_bounds__(256) void k_pyr_depth(PyrDepthArgs p) {
   __shared__ float I1[16][17], V1[16][17], I2[8][9], V2[8][9], I3[4][5], V3[4][5];
   const int t = threadIdx.x, bx = blockIdx.x, by = blockIdx.y;
