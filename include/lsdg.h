@@ -340,7 +340,7 @@ lsdg_status lsdg_energy_grad_sim3(lsdg_ctx ctx, lsdg_ref ref, lsdg_frame frame, 
 
 /* lsdg_energy_grad with no device allocation, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops).  No
- * counterpart in the reference.  Its reduction rows, counter, merged row and
+ * counterpart in the reference.  Its reduction rows, merged row and
  * per-point dE/dd staging live in a fixed device array of about 4.1 MB per
  * GPU (held until the process ends), so:
  *   - the reference's point cloud of `level` and the frame's pyramid level

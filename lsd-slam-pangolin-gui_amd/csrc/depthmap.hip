@@ -1427,15 +1427,7 @@ __global__ __launch_bounds__(256) void k_mean_idepth(HypBuf cur, int n, double* 
   __syncthreads();
   if (!amLast) return;
   __threadfence();
-  if (threadIdx.x < 2) {
-    double a = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
-      unsigned long long bits = __hip_atomic_load((const unsigned long long*)&partials[2 * b + threadIdx.x],
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      a += __longlong_as_double((long long)bits);
-    }
-    out[threadIdx.x] = a;
-  }
+  merge_rows<2, 256>(partials, gridDim.x, out);
   if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 

@@ -134,6 +134,55 @@ __device__ inline void fuse4(const float* ida, const float* va, float& oid, floa
   }
 }
 
+// The merge by the last block of a launch: out[k] = the sum over b of
+// rows[b * NS + k], in block order (k < NS; BS threads).  The rows are staged
+// through LDS a chunk at a time, every thread issuing its agent-scope atomic
+// loads back to back (the rows were written with agent-scope stores; no L2
+// invalidation needed), so a chunk costs one memory round trip rather than
+// one per row; NS lanes then add their column in block order, as a
+// sequential loop over the rows would.
+constexpr int MERGE_CHUNK = 2048;  // doubles staged per round (16 KB of LDS)
+template <int NS, int BS>
+__device__ inline void merge_rows(const double* __restrict__ rows, unsigned nrows, double* __restrict__ out) {
+  static_assert(MERGE_CHUNK % BS == 0 && NS <= BS, "merge layout");
+  constexpr int LOADS = MERGE_CHUNK / BS;
+  __shared__ double stage[MERGE_CHUNK];
+  const unsigned total = nrows * NS;
+  double s = 0.0;
+  for (unsigned base = 0; base < total; base += MERGE_CHUNK) {
+    const unsigned m = min((unsigned)MERGE_CHUNK, total - base);
+    unsigned long long v[LOADS];
+#pragma unroll
+    for (int u = 0; u < LOADS; ++u) {
+      const unsigned i = threadIdx.x + u * BS;
+      v[u] = i < m ? __hip_atomic_load((const unsigned long long*)&rows[base + i], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)
+                   : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < LOADS; ++u) {
+      const unsigned i = threadIdx.x + u * BS;
+      if (i < m) stage[i] = __longlong_as_double((long long)v[u]);
+    }
+    __syncthreads();
+    if (threadIdx.x < NS) {
+      // this column's entries of the chunk, added in row order; eight LDS
+      // reads are in flight ahead of the additions
+      unsigned i = (threadIdx.x + NS - base % NS) % NS;
+      for (; i + 7 * NS < m; i += 8 * NS) {
+        double r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r[u] = stage[i + u * NS];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += r[u];
+      }
+      for (; i < m; i += NS) s += stage[i];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < NS) out[threadIdx.x] = s;
+}
+
 struct PyrBatch {
   const void* img_args;
   const void* grad_args;

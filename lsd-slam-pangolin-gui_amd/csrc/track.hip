@@ -55,17 +55,65 @@ __device__ inline void reduce_and_merge(Accum<MODE, NS>& acc, double* __restrict
   }
   __syncthreads();
   if (!amLast) return;
-  // the rows are read with agent-scope atomic loads: no L2 invalidation needed
-  if (threadIdx.x < NS) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
-      unsigned long long bits = __hip_atomic_load((const unsigned long long*)&partials[(size_t)b * NS + threadIdx.x],
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s += __longlong_as_double((long long)bits);
-    }
-    out[threadIdx.x] = s;
-  }
+  merge_rows<NS, EVAL_BS>(partials, gridDim.x, out);
   if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide reduction of the per-thread accumulators into this block's FP64
+// row (the merge is a second launch, k_merge_rows: no completion counter, so
+// no same-address atomics, which serialise once the grid has more than a
+// few hundred blocks).
+template <int MODE, int NS>
+__device__ inline void reduce_to_row(Accum<MODE, NS>& acc, double* __restrict__ rows) {
+  __shared__ double sh[EVAL_WAVES][NS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if constexpr (MODE == ACC_F64) {
+    acc.wave_reduce();
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) sh[wave][k] = acc.value(k);
+    }
+  } else {
+    const double cell = acc.wave_slot();
+    if (lane < NS) sh[wave][lane] = cell;
+  }
+  __syncthreads();
+  if (threadIdx.x < NS)
+    rows[(size_t)blockIdx.x * NS + threadIdx.x] = Accum<MODE, NS>::merge_waves(&sh[0][threadIdx.x], NS, EVAL_WAVES);
+}
+
+// out[k] = the sum of rows[b * NS + k] over the nrows blocks, in a fixed
+// order: EVAL_BS / NS lanes per column each add the rows b = lane, lane + L,
+// ... in order, then the lanes' sums are added pairwise in a fixed tree.  The
+// result does not depend on timing (the adjoint's sums are held to the FP64
+// oracle by tolerance, not to a particular order), and the column sums run in
+// parallel instead of one dependent addition per row.
+template <int NS>
+__global__ __launch_bounds__(EVAL_BS) void k_merge_rows(const double* __restrict__ rows, unsigned nrows,
+                                                        double* __restrict__ out) {
+  constexpr int L = EVAL_BS / NS;  // lanes per column
+  __shared__ double part[NS][L];
+  const int col = threadIdx.x % NS, lane = threadIdx.x / NS;
+  if (lane < L) {
+    double s = 0.0;
+    unsigned b = lane;
+    for (; b + 15 * L < nrows; b += 16 * L) {  // 16 loads in flight, added in row order
+      double r[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) r[u] = rows[(size_t)(b + u * L) * NS + col];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s += r[u];
+    }
+    for (; b < nrows; b += L) s += rows[(size_t)b * NS + col];
+    part[col][lane] = s;
+  }
+  __syncthreads();
+  // pairwise tree over the lanes (a lane without a partner passes its sum up)
+  for (int w = 1; w < L; w <<= 1) {
+    if (lane < L && lane % (2 * w) == 0 && lane + w < L) part[col][lane] += part[col][lane + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < NS) out[threadIdx.x] = part[threadIdx.x][0];
 }
 
 // ------------------------------------------------------------------- SE3
@@ -101,13 +149,12 @@ __global__ __launch_bounds__(EVAL_BS) void k_sim3_eval(PointsLevel pts, const vo
 template <int PREC, int ACC>
 __global__ __launch_bounds__(EVAL_BS) void k_energy_grad(PointsLevel pts, const void* __restrict__ img, LevelGeom g,
                                                          Xf x, TrackKParams kp, float* __restrict__ grad_d,
-                                                         double* __restrict__ partials, unsigned* counter,
-                                                         double* __restrict__ out) {
+                                                         double* __restrict__ rows) {
   Accum<ACC, NS_ADJ> acc;
   acc.zero();
   for (int i = blockIdx.x * EVAL_BS + threadIdx.x; i < pts.n; i += gridDim.x * EVAL_BS)
     adjoint_point<PREC>(pts, i, img, g, x, kp, grad_d, acc);
-  reduce_and_merge(acc, partials, counter, out);
+  reduce_to_row(acc, rows);
 }
 
 // Sim3 adjoint: dE/dxi (7), 2E split into its photometric and depth parts,
@@ -117,13 +164,12 @@ __global__ __launch_bounds__(EVAL_BS) void k_energy_grad_sim3(PointsLevel pts, c
                                                               const float* __restrict__ fid,
                                                               const float* __restrict__ fvar, LevelGeom g, Xf x,
                                                               TrackKParams kp, float* __restrict__ grad_d,
-                                                              double* __restrict__ partials, unsigned* counter,
-                                                              double* __restrict__ out) {
+                                                              double* __restrict__ rows) {
   Accum<ACC, NS_ADJ_SIM3> acc;
   acc.zero();
   for (int i = blockIdx.x * EVAL_BS + threadIdx.x; i < pts.n; i += gridDim.x * EVAL_BS)
     adjoint_point_sim3<PREC>(pts, i, img, fid, fvar, g, x, kp, grad_d, acc);
-  reduce_and_merge(acc, partials, counter, out);
+  reduce_to_row(acc, rows);
 }
 
 // ------------------------------------------------------------ launchers
@@ -170,45 +216,73 @@ hipError_t launch_sim3_eval(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, in
                   (const float*)f->d_ivar[level], g, x, kp, c->d_partials, c->d_counter, d_out);
 }
 
+// Adjoint grid: two points per thread (one, two and four measure within 10%
+// for the kernel; fewer blocks leave fewer rows to merge), at most
+// NOALLOC_MAX_BLOCKS blocks: lsdg_energy_grad and its allocation-free variant
+// cut the points alike, so their sums agree bit for bit.
+static inline int adjoint_blocks(int n) {
+  static_assert(NOALLOC_MAX_BLOCKS <= 4096, "the context's partial rows hold the adjoint's blocks");
+  int nb = (n + EVAL_BS * 2 - 1) / (EVAL_BS * 2);
+  if (nb < 1) nb = 1;
+  if (nb > NOALLOC_MAX_BLOCKS) nb = NOALLOC_MAX_BLOCKS;
+  return nb;
+}
+
+// The adjoint kernel, then the merge of its rows; a timer spans both (the
+// first launch's start to the second's end).
+template <typename... KArgs, typename... Args>
+static hipError_t launch_adjoint(void (*k)(KArgs...), void (*merge)(const double*, unsigned, double*), int nb,
+                                 double* rows, double* out, hipStream_t s, const KTimer* t, Args... args) {
+  if (t && t->a) hipExtLaunchKernelGGL(k, dim3(nb), dim3(EVAL_BS), 0, s, t->a, nullptr, 0, static_cast<KArgs>(args)...);
+  else k<<<dim3(nb), dim3(EVAL_BS), 0, s>>>(static_cast<KArgs>(args)...);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (t && t->a) hipExtLaunchKernelGGL(merge, dim3(1), dim3(EVAL_BS), 0, s, nullptr, t->b, 0, (const double*)rows,
+                                       (unsigned)nb, out);
+  else merge<<<dim3(1), dim3(EVAL_BS), 0, s>>>(rows, (unsigned)nb, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_energy_grad(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                               const TrackKParams& kp, float* d_grad_d, double* d_out, const KTimer* t) {
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
-  const int nb = eval_blocks(c, pts.n);
+  const int nb = adjoint_blocks(pts.n);
   const void* img = level_image(f, level);
   auto k = f->precision == LSDG_FP16 ? k_energy_grad<LSDG_FP16, ACC_F64>
            : f->precision == LSDG_BF16 ? k_energy_grad<LSDG_BF16, ACC_F64>
                                        : k_energy_grad<LSDG_FP32, ACC_F64>;
-  return launch_k(k, dim3(nb), dim3(EVAL_BS), c->stream, t, pts, img, g, x, kp, d_grad_d, c->d_partials, c->d_counter,
-                  d_out);
+  return launch_adjoint(k, k_merge_rows<NS_ADJ>, nb, c->d_partials, d_out, c->stream, t, pts, img, g, x, kp, d_grad_d,
+                        c->d_partials);
 }
 
 hipError_t launch_energy_grad_sim3(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                                    const TrackKParams& kp, float* d_grad_d, double* d_out, const KTimer* t) {
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
+  const int nb = adjoint_blocks(pts.n);
   const void* img = level_image(f, level);
   auto k = f->precision == LSDG_FP16 ? k_energy_grad_sim3<LSDG_FP16, ACC_F64>
            : f->precision == LSDG_BF16 ? k_energy_grad_sim3<LSDG_BF16, ACC_F64>
                                        : k_energy_grad_sim3<LSDG_FP32, ACC_F64>;
-  return launch_k(k, dim3(eval_blocks(c, pts.n)), dim3(EVAL_BS), c->stream, t, pts, img,
-                  (const float*)f->d_idepth[level], (const float*)f->d_ivar[level], g, x, kp, d_grad_d,
-                  c->d_partials, c->d_counter, d_out);
+  return launch_adjoint(k, k_merge_rows<NS_ADJ_SIM3>, nb, c->d_partials, d_out, c->stream, t, pts, img,
+                        (const float*)f->d_idepth[level], (const float*)f->d_ivar[level], g, x, kp, d_grad_d,
+                        c->d_partials);
 }
 
 // ------------------------------------------------ allocation-free adjoint
 // lsdg_energy_grad_noalloc's device memory: one file-scope array per GPU,
 // loaded with the code object and held until the process ends.  Layout:
 //   [0, PARTIALS)              FP64 partial rows, NOALLOC_MAX_BLOCKS x NS_ADJ
-//   [COUNTER, +256)            the last-block counter (unsigned)
 //   [RESULT, +256)             the merged row (NS_ADJ doubles)
 //   [GRAD, +4 * max points)    per-point dE/dd staging
+// Every byte a call reads is written earlier in the same call (rows by the
+// adjoint kernel, the merged row by the merge), so nothing needs clearing.
 // Host code uses the address hipGetSymbolAddress returns for the current
 // device, never the array's name (which is a host-side stand-in).  Every
 // call on a device shares that device's array: calls that overlap on one
 // GPU race.
 namespace {
 constexpr size_t NOALLOC_PARTIALS = (size_t)NOALLOC_MAX_BLOCKS * NS_ADJ * sizeof(double);
-constexpr size_t NOALLOC_COUNTER = NOALLOC_PARTIALS;
-constexpr size_t NOALLOC_RESULT = NOALLOC_COUNTER + 256;
+constexpr size_t NOALLOC_RESULT = NOALLOC_PARTIALS;
 constexpr size_t NOALLOC_GRAD = NOALLOC_RESULT + 256;
 constexpr size_t NOALLOC_BYTES = NOALLOC_GRAD + (size_t)NOALLOC_MAX_POINTS * sizeof(float);
 static_assert(NOALLOC_PARTIALS % 256 == 0, "sections stay 256-byte aligned");
@@ -232,8 +306,6 @@ hipError_t noalloc_scratch_init(int device) {
   return hipSuccess;
 }
 
-__global__ void k_noalloc_clear(unsigned* counter) { *counter = 0u; }
-
 hipError_t launch_energy_grad_noalloc(lsdg_ctx c, const PointsLevel& pts, lsdg_frame f, int level, const Xf& x,
                                       const TrackKParams& kp, double* h_row, float* h_grad) {
   unsigned char* base = nullptr;
@@ -244,21 +316,15 @@ hipError_t launch_energy_grad_noalloc(lsdg_ctx c, const PointsLevel& pts, lsdg_f
   if (!base) return hipErrorNotInitialized;
   if (pts.n > NOALLOC_MAX_POINTS) return hipErrorInvalidValue;
   double* partials = (double*)base;
-  unsigned* counter = (unsigned*)(base + NOALLOC_COUNTER);
   double* row = (double*)(base + NOALLOC_RESULT);
   float* grad = (float*)(base + NOALLOC_GRAD);
   LevelGeom g{f->w[level], f->h[level], f->fx[level], f->fy[level], f->cx[level], f->cy[level]};
-  int nb = eval_blocks(c, pts.n);
-  if (nb > NOALLOC_MAX_BLOCKS) nb = NOALLOC_MAX_BLOCKS;
-  // the counter must start at zero whatever the last call on this device left
-  k_noalloc_clear<<<dim3(1), dim3(1), 0, c->stream>>>(counter);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  const int nb = adjoint_blocks(pts.n);
   auto k = f->precision == LSDG_FP16 ? k_energy_grad<LSDG_FP16, ACC_F64>
            : f->precision == LSDG_BF16 ? k_energy_grad<LSDG_BF16, ACC_F64>
                                        : k_energy_grad<LSDG_FP32, ACC_F64>;
-  e = launch_k(k, dim3(nb), dim3(EVAL_BS), c->stream, nullptr, pts, level_image(f, level), g, x, kp, grad, partials,
-               counter, row);
+  hipError_t e = launch_adjoint(k, k_merge_rows<NS_ADJ>, nb, partials, row, c->stream, nullptr, pts,
+                                level_image(f, level), g, x, kp, grad, partials);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(h_row, row, sizeof(double) * NS_ADJ, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && h_grad)
