@@ -193,43 +193,44 @@ __device__ inline uint8_t undistort_pixel(const uint8_t* __restrict__ src, int i
 
 // InputThread's ingest of one camera image (lib/App/InputThread.cpp:59-71):
 // undistortion, the 8-bit to float conversion of Frame's constructor and
-// Frame::buildImage for levels 1-4, in one launch.  A block owns a 32x32
-// level-0 tile: it remaps the tile into LDS (writing level 0 and, when
-// `undist` is set, the 8-bit undistorted image) and reduces it through the
-// levels as k_pyr_images does.
+// Frame::buildImage for levels 1-4, in one launch.  A block owns a 16x16
+// level-0 tile (one pixel per thread: the table entry, then the source taps,
+// two waits on memory), writes level 0 (and, when `undist` is set, the 8-bit
+// undistorted image), and reduces the tile level by level in shared memory:
+// a 16x16 tile holds whole 2^4 blocks of level 0, so levels 1-4 follow from
+// it alone.
 __global__ __launch_bounds__(256) void k_ingest(PyrImgArgs p, const uint8_t* __restrict__ src, int inW,
                                                 const float2* __restrict__ map, uint8_t* __restrict__ undist) {
-  __shared__ float L0[32][33];
-  __shared__ float L1[16][17];
-  const int t = threadIdx.x;
-  const int bx = blockIdx.x, by = blockIdx.y;
-  const int w0 = p.w[0], h0 = p.h[0];
-  for (int k = t; k < 1024; k += 256) {
-    const int lx = k & 31, ly = k >> 5;
-    const int x = bx * 32 + lx, y = by * 32 + ly;
-    float v = 0.0f;
-    if (x < w0 && y < h0) {
-      const int i = x + y * w0;
-      const uint8_t u = undistort_pixel(src, inW, map[i]);
-      v = (float)u;
-      p.img[0][i] = v;
-      if (undist) undist[i] = u;
-    }
-    L0[ly][lx] = v;
+  static_assert(PYR == 5, "a 16x16 tile holds whole 2^(PYR-1) blocks");
+  __shared__ float T[16][17];
+  const int t = threadIdx.x, lx = t & 15, ly = t >> 4;
+  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
+  const bool inside = x < p.w[0] && y < p.h[0];
+  const int i = inside ? x + y * p.w[0] : 0;
+  const uint8_t u = undistort_pixel(src, inW, inside ? map[i] : make_float2(-1.0f, -1.0f));
+  float v = 0.0f;
+  if (inside) {
+    v = (float)u;
+    p.img[0][i] = v;
+    if (undist) undist[i] = u;
   }
+  T[ly][lx] = v;
   __syncthreads();
-  {
-    const int lx = t & 15, ly = t >> 4;
-    const int x = bx * 16 + lx, y = by * 16 + ly;
-    float v = 0.f;
-    if (x < p.w[1] && y < p.h[1]) {
-      v = (L0[2 * ly][2 * lx] + L0[2 * ly][2 * lx + 1] + L0[2 * ly + 1][2 * lx] + L0[2 * ly + 1][2 * lx + 1]) * 0.25f;
-      p.img[1][x + y * p.w[1]] = v;
+#pragma unroll
+  for (int l = 1, n = 8; l < PYR; ++l, n >>= 1) {
+    const int tx = t % n, ty = t / n;
+    float c = 0.0f;
+    if (t < n * n) {
+      const int cx = blockIdx.x * n + tx, cy = blockIdx.y * n + ty;
+      if (cx < p.w[l] && cy < p.h[l]) {
+        c = (T[2 * ty][2 * tx] + T[2 * ty][2 * tx + 1] + T[2 * ty + 1][2 * tx] + T[2 * ty + 1][2 * tx + 1]) * 0.25f;
+        p.img[l][cx + cy * p.w[l]] = c;
+      }
     }
-    L1[ly][lx] = v;
+    __syncthreads();  // the level's reads end before its values replace them
+    if (t < n * n) T[ty][tx] = c;
+    __syncthreads();
   }
-  __syncthreads();
-  pyr_levels_2to4(p, L1, bx, by);
 }
 
 struct PyrGradArgs {
@@ -517,7 +518,7 @@ hipError_t launch_ingest(lsdg_frame f, const uint8_t* d_src, int inW, const floa
     p.w[l] = f->w[l];
     p.h[l] = f->h[l];
   }
-  dim3 grid((f->w[0] + 31) / 32, (f->h[0] + 31) / 32);
+  dim3 grid((f->w[0] + 15) / 16, (f->h[0] + 15) / 16);
   k_ingest<<<grid, dim3(256), 0, s>>>(p, d_src, inW, d_map, d_undist);
   return hipGetLastError();
 }
