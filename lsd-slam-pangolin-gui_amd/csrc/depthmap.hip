@@ -229,6 +229,29 @@ __device__ inline void walk(float& x, float& y, int& j, int to, float ix, float 
   }
 }
 
+// A step's matching error: the sum of the squared differences of its five
+// samples (newest first) and the keyframe's, in the reference's order.
+__device__ inline float stereo_error(float v_p2, float v_p1, float v, float v_m1, float v_m2, float r_p2, float r_p1,
+                                     float r, float r_m1, float r_m2) {
+  float e, ee = 0.0f;
+  e = v_p2 - r_p2; ee += e * e;
+  e = v_p1 - r_p1; ee += e * e;
+  e = v - r; ee += e * e;
+  e = v_m1 - r_m1; ee += e * e;
+  e = v_m2 - r_m2; ee += e * e;
+  return ee;
+}
+__device__ inline float stereo_sq(const float* e) {
+  float ee = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) ee += e[i] * e[i];
+  return ee;
+}
+// The dot product of two steps' error vectors, as the reference forms it.
+__device__ inline float stereo_dot(const float* a, const float* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4];
+}
+
 // doLineStereo: the matching error (>= 0) or a code -1..-4.  Called by the G
 // lanes of a group (lane g) with the same arguments; all return the same.
 template <int G>
@@ -363,138 +386,240 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
   sc.diffPost = NAN;
   sc.bx = -1.0f;
   sc.by = -1.0f;
-  float wx = cp0x, wy = cp0y;  // walker at cp_{wj}
-  int wj = 0;
-  for (int base = 0;; base += G * C) {
-    const int j0 = base + g * C, j1 = j0 + C;
-    bool lookValid = false;  // step j1 is on the line
-    if (j0 < maxSteps) {
-      // window V_{j0-3} .. V_{j0+1}; V_m for m >= 2 is sampled at cp_{m-2} + 2 inc
-      float W[5];
-      walk(wx, wy, wj, j0 - 5, incx, incy);
+  if constexpr (G == 1) {
+    // One lane per line, in two passes.  The search keeps only what decides
+    // the match: each step's error and the first and second minima (the
+    // reference's selects, in step order).  The best step's other records
+    // (the errors and error-vector dot products of its neighbours, its
+    // position) are then recomputed from the samples around it, which the
+    // same chain of float additions gives again (walk), so they equal what
+    // the reference's loop records when it meets the best.
+    float val_m2 = init_m2, val_m1 = init_m1, val = init_0, val_p1 = init_p1;
+    float cpx = cp0x, cpy = cp0y, lx = cp0x, ly = cp0y;
+    int loop = 0;
+    // The texels of the next STEREO_AHEAD steps' samples are in flight
+    // while a step is evaluated: a lead copy of the position chain fetches
+    // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
+    // end are clamped to the image and never used.
+    Quad ring[STEREO_AHEAD];
 #pragma unroll
-      for (int t = 0; t < 5; ++t) {
-        const int m = j0 - 3 + t;
-        if (m <= 1) {
-          W[t] = m == 1 ? init_p1 : m == 0 ? init_0 : m == -1 ? init_m1 : init_m2;
-        } else {
-          walk(wx, wy, wj, m - 2, incx, incy);
-          // clamped fetch: a chunk past the line's end may start off the image
-          W[t] = blend_quad(fetch_quad(refImg, wx + 2.0f * incx, wy + 2.0f * incy, width, height),
-                            wx + 2.0f * incx, wy + 2.0f * incy);
-        }
-      }
-      walk(wx, wy, wj, j0, incx, incy);
-      float cpx = wx, cpy = wy;
-      int loop = j0;
-      float val_m2 = W[1], val_m1 = W[2], val = W[3], val_p1 = W[4];
-      float e1A = NAN, e2A = NAN, e3A = NAN, e4A = NAN, e5A = NAN;
-      float e1B = NAN, e2B = NAN, e3B = NAN, e4B = NAN, e5B = NAN;
-      float eeLast = -1.0f;
-      if (j0 > 0) {  // the step before the chunk (odd: the B registers)
-        eeLast = 0.0f;
-        e1B = W[4] - rv_p2; eeLast += e1B * e1B;
-        e2B = W[3] - rv_p1; eeLast += e2B * e2B;
-        e3B = W[2] - rv; eeLast += e3B * e3B;
-        e4B = W[1] - rv_m1; eeLast += e4B * e4B;
-        e5B = W[0] - rv_m2; eeLast += e5B * e5B;
-      }
-      bool bestWasLast = false;
-      // The texels of the next STEREO_AHEAD steps' samples are in flight
-      // while a step is evaluated: a lead copy of the position chain fetches
-      // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
-      // end are clamped to the image and never used.
-      float lx = cpx, ly = cpy;
-      Quad ring[STEREO_AHEAD];
+    for (int k = 0; k < STEREO_AHEAD; ++k) {
+      ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+      lx += incx;
+      ly += incy;
+    }
+    // Lanes leave the loop together (a wave-uniform exit every
+    // STEREO_AHEAD steps; finished lanes idle), and every step issues its
+    // fetch whether or not the lane is still searching: with per-lane exits
+    // the wave's outstanding-load count would differ between paths and the
+    // compiler would wait for all fetches at every step.
+    bool alive = true;
+    // the step bound, held in a vector register: as a scalar the compiler
+    // reloads it from the kernel arguments at every step (a scalar-memory
+    // round trip per step)
+    int stepLimit;
+    asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
+    for (;;) {
 #pragma unroll
       for (int k = 0; k < STEREO_AHEAD; ++k) {
+        // the reference's while condition, without short-circuit branches
+        const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
+        const bool act = alive & go & (loop < stepLimit);
+        const float val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
         ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
         lx += incx;
         ly += incy;
+        const float ee = stereo_error(val_p2, val_p1, val, val_m1, val_m2, rv_p2, rv_p1, rv, rv_m1, rv_m2);
+        const bool nb = act & (ee < sc.best);          // new best
+        const bool ns = act & !nb & (ee < sc.second);  // new second
+        sc.second = nb ? sc.best : (ns ? ee : sc.second);
+        sc.isecond = nb ? sc.ibest : (ns ? loop : sc.isecond);
+        sc.best = nb ? ee : sc.best;
+        sc.ibest = nb ? loop : sc.ibest;
+        alive = act;
+        val_m2 = val_m1;
+        val_m1 = val;
+        val = val_p1;
+        val_p1 = val_p2;
+        // the position and count of a lane that has left the line are never
+        // read again, so they advance regardless
+        cpx += incx;
+        cpy += incy;
+        ++loop;
       }
-      // Lanes leave the loop together (a wave-uniform exit every
-      // STEREO_AHEAD steps; finished lanes idle), and every step issues its
-      // fetch whether or not the lane is still searching: with per-lane exits
-      // the wave's outstanding-load count would differ between paths and the
-      // compiler would wait for all fetches at every step.
-      bool alive = true;
-      // the step bound, held in a vector register: as a scalar the
-      // compiler reloads it from the kernel arguments at every step (a
-      // scalar-memory round trip per step)
-      int stepLimit;
-      asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
-      for (;;) {
+      if (!__any(alive)) break;
+    }
+    if (sc.best < INFINITY) {
+      // Samples V_{b-3} .. V_{b+3} around the best step b (V_m, m >= 2, at
+      // cp_{m-2} + 2 inc; V_{-2} .. V_1 the initial samples): the error
+      // vectors of steps b - 1, b and b + 1 (step k's window is V_{k-2} ..
+      // V_{k+2}), and the positions cp_b and cp_{b+1}.
+      const int b = sc.ibest;
+      float V[7];
+      float px = cp0x, py = cp0y, bpx = cp0x, bpy = cp0y;
+      int pj = 0;
 #pragma unroll
+      for (int t = 0; t < 7; ++t) {
+        const int m = b - 3 + t;
+        if (m <= 1) {
+          V[t] = m == 1 ? init_p1 : m == 0 ? init_0 : m == -1 ? init_m1 : init_m2;
+        } else {
+          walk(px, py, pj, m - 2, incx, incy);
+          V[t] = blend_quad(fetch_quad(refImg, px + 2.0f * incx, py + 2.0f * incy, width, height), px + 2.0f * incx,
+                            py + 2.0f * incy);
+        }
+        if (t == 5) {  // m = b + 2: the walker is at cp_b
+          bpx = px;
+          bpy = py;
+        }
+      }
+      // (the walker is now at cp_{b+1})
+      const float ePrev[5] = {V[4] - rv_p2, V[3] - rv_p1, V[2] - rv, V[1] - rv_m1, V[0] - rv_m2};
+      const float eBest[5] = {V[5] - rv_p2, V[4] - rv_p1, V[3] - rv, V[2] - rv_m1, V[1] - rv_m2};
+      const float eNext[5] = {V[6] - rv_p2, V[5] - rv_p1, V[4] - rv, V[3] - rv_m1, V[2] - rv_m2};
+      // the step after the best, if the line went on (the lane was on it at b)
+      const bool post = (b + 1 < maxSteps) & (negx == (px > closeX)) & (negy == (py > closeY));
+      sc.errPre = b == 0 ? -1.0f : stereo_sq(ePrev);
+      sc.diffPre = b == 0 ? NAN : stereo_dot(eBest, ePrev);
+      sc.errPost = post ? stereo_sq(eNext) : -1.0f;
+      sc.diffPost = post ? stereo_dot(eNext, eBest) : -1.0f;
+      sc.bx = bpx;
+      sc.by = bpy;
+    }
+  } else {
+    float wx = cp0x, wy = cp0y;  // walker at cp_{wj}
+    int wj = 0;
+    for (int base = 0;; base += G * C) {
+      const int j0 = base + g * C, j1 = j0 + C;
+      bool lookValid = false;  // step j1 is on the line
+      if (j0 < maxSteps) {
+        // window V_{j0-3} .. V_{j0+1}; V_m for m >= 2 is sampled at cp_{m-2} + 2 inc
+        float W[5];
+        walk(wx, wy, wj, j0 - 5, incx, incy);
+  #pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          const int m = j0 - 3 + t;
+          if (m <= 1) {
+            W[t] = m == 1 ? init_p1 : m == 0 ? init_0 : m == -1 ? init_m1 : init_m2;
+          } else {
+            walk(wx, wy, wj, m - 2, incx, incy);
+            // clamped fetch: a chunk past the line's end may start off the image
+            W[t] = blend_quad(fetch_quad(refImg, wx + 2.0f * incx, wy + 2.0f * incy, width, height),
+                              wx + 2.0f * incx, wy + 2.0f * incy);
+          }
+        }
+        walk(wx, wy, wj, j0, incx, incy);
+        float cpx = wx, cpy = wy;
+        int loop = j0;
+        float val_m2 = W[1], val_m1 = W[2], val = W[3], val_p1 = W[4];
+        float e1A = NAN, e2A = NAN, e3A = NAN, e4A = NAN, e5A = NAN;
+        float e1B = NAN, e2B = NAN, e3B = NAN, e4B = NAN, e5B = NAN;
+        float eeLast = -1.0f;
+        if (j0 > 0) {  // the step before the chunk (odd: the B registers)
+          eeLast = 0.0f;
+          e1B = W[4] - rv_p2; eeLast += e1B * e1B;
+          e2B = W[3] - rv_p1; eeLast += e2B * e2B;
+          e3B = W[2] - rv; eeLast += e3B * e3B;
+          e4B = W[1] - rv_m1; eeLast += e4B * e4B;
+          e5B = W[0] - rv_m2; eeLast += e5B * e5B;
+        }
+        bool bestWasLast = false;
+        // The texels of the next STEREO_AHEAD steps' samples are in flight
+        // while a step is evaluated: a lead copy of the position chain fetches
+        // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
+        // end are clamped to the image and never used.
+        float lx = cpx, ly = cpy;
+        Quad ring[STEREO_AHEAD];
+  #pragma unroll
         for (int k = 0; k < STEREO_AHEAD; ++k) {
-          // the reference's while condition, without short-circuit branches
-          const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
-          const bool act = alive & go & (loop < stepLimit);
-          const float val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
           ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
           lx += incx;
           ly += incy;
-          // The step in straight-line code: every lane computes it, and the
-          // bookkeeping applies the reference's branches as selects (a lane
-          // that has left the line keeps its state).
-          float ee = 0.0f;
-          if ((k & 1) == 0) {
-            e1A = val_p2 - rv_p2; ee += e1A * e1A;
-            e2A = val_p1 - rv_p1; ee += e2A * e2A;
-            e3A = val - rv; ee += e3A * e3A;
-            e4A = val_m1 - rv_m1; ee += e4A * e4A;
-            e5A = val_m2 - rv_m2; ee += e5A * e5A;
-          } else {
-            e1B = val_p2 - rv_p2; ee += e1B * e1B;
-            e2B = val_p1 - rv_p1; ee += e2B * e2B;
-            e3B = val - rv; ee += e3B * e3B;
-            e4B = val_m1 - rv_m1; ee += e4B * e4B;
-            e5B = val_m2 - rv_m2; ee += e5B * e5B;
-          }
-          // the next lane's first step: only the error after a best (a
-          // single lane's chunk ends at maxSteps, which act excludes)
-          const bool look = G > 1 && (act & (loop == j1));
-          const bool step = G > 1 ? act & (loop != j1) : act;
-          const bool nb = step & (ee < sc.best);                 // new best
-          const bool post = act & !nb & bestWasLast;             // the step after the best
-          const bool ns = step & !nb & (ee < sc.second);         // new second
-          sc.second = nb ? sc.best : (ns ? ee : sc.second);
-          sc.isecond = nb ? sc.ibest : (ns ? loop : sc.isecond);
-          // a best or the step after one is rare once a line is under way:
-          // its records are written under a wave-uniform branch
-          if (__any(nb | post)) {
-            const float dot = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-            sc.best = nb ? ee : sc.best;
-            sc.ibest = nb ? loop : sc.ibest;
-            sc.errPre = nb ? eeLast : sc.errPre;
-            sc.diffPre = nb ? dot : sc.diffPre;
-            sc.errPost = nb ? -1.0f : (post ? ee : sc.errPost);
-            sc.diffPost = nb ? -1.0f : (post ? dot : sc.diffPost);
-            sc.bx = nb ? cpx : sc.bx;
-            sc.by = nb ? cpy : sc.by;
-          }
-          bestWasLast = nb | (bestWasLast & !post);
-          lookValid |= look;
-          alive = step;
-          eeLast = ee;
-          val_m2 = val_m1;
-          val_m1 = val;
-          val = val_p1;
-          val_p1 = val_p2;
-          // (one lane per line: the position and count of a lane that has
-          // left it are never read again, so they advance regardless)
-          cpx = (G == 1 || step) ? cpx + incx : cpx;
-          cpy = (G == 1 || step) ? cpy + incy : cpy;
-          loop += (G == 1 || step) ? 1 : 0;
         }
-        if (!__any(alive)) break;
+        // Lanes leave the loop together (a wave-uniform exit every
+        // STEREO_AHEAD steps; finished lanes idle), and every step issues its
+        // fetch whether or not the lane is still searching: with per-lane exits
+        // the wave's outstanding-load count would differ between paths and the
+        // compiler would wait for all fetches at every step.
+        bool alive = true;
+        // the step bound, held in a vector register: as a scalar the
+        // compiler reloads it from the kernel arguments at every step (a
+        // scalar-memory round trip per step)
+        int stepLimit;
+        asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
+        for (;;) {
+  #pragma unroll
+          for (int k = 0; k < STEREO_AHEAD; ++k) {
+            // the reference's while condition, without short-circuit branches
+            const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
+            const bool act = alive & go & (loop < stepLimit);
+            const float val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
+            ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+            lx += incx;
+            ly += incy;
+            // The step in straight-line code: every lane computes it, and the
+            // bookkeeping applies the reference's branches as selects (a lane
+            // that has left the line keeps its state).
+            float ee = 0.0f;
+            if ((k & 1) == 0) {
+              e1A = val_p2 - rv_p2; ee += e1A * e1A;
+              e2A = val_p1 - rv_p1; ee += e2A * e2A;
+              e3A = val - rv; ee += e3A * e3A;
+              e4A = val_m1 - rv_m1; ee += e4A * e4A;
+              e5A = val_m2 - rv_m2; ee += e5A * e5A;
+            } else {
+              e1B = val_p2 - rv_p2; ee += e1B * e1B;
+              e2B = val_p1 - rv_p1; ee += e2B * e2B;
+              e3B = val - rv; ee += e3B * e3B;
+              e4B = val_m1 - rv_m1; ee += e4B * e4B;
+              e5B = val_m2 - rv_m2; ee += e5B * e5B;
+            }
+            // the next lane's first step: only the error after a best (a
+            // single lane's chunk ends at maxSteps, which act excludes)
+            const bool look = G > 1 && (act & (loop == j1));
+            const bool step = G > 1 ? act & (loop != j1) : act;
+            const bool nb = step & (ee < sc.best);                 // new best
+            const bool post = act & !nb & bestWasLast;             // the step after the best
+            const bool ns = step & !nb & (ee < sc.second);         // new second
+            sc.second = nb ? sc.best : (ns ? ee : sc.second);
+            sc.isecond = nb ? sc.ibest : (ns ? loop : sc.isecond);
+            // a best or the step after one is rare once a line is under way:
+            // its records are written under a wave-uniform branch
+            if (__any(nb | post)) {
+              const float dot = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
+              sc.best = nb ? ee : sc.best;
+              sc.ibest = nb ? loop : sc.ibest;
+              sc.errPre = nb ? eeLast : sc.errPre;
+              sc.diffPre = nb ? dot : sc.diffPre;
+              sc.errPost = nb ? -1.0f : (post ? ee : sc.errPost);
+              sc.diffPost = nb ? -1.0f : (post ? dot : sc.diffPost);
+              sc.bx = nb ? cpx : sc.bx;
+              sc.by = nb ? cpy : sc.by;
+            }
+            bestWasLast = nb | (bestWasLast & !post);
+            lookValid |= look;
+            alive = step;
+            eeLast = ee;
+            val_m2 = val_m1;
+            val_m1 = val;
+            val = val_p1;
+            val_p1 = val_p2;
+            // (one lane per line: the position and count of a lane that has
+            // left it are never read again, so they advance regardless)
+            cpx = (G == 1 || step) ? cpx + incx : cpx;
+            cpy = (G == 1 || step) ? cpy + incy : cpy;
+            loop += (G == 1 || step) ? 1 : 0;
+          }
+          if (!__any(alive)) break;
+        }
+        wx = cpx;
+        wy = cpy;
+        wj = loop;
       }
-      wx = cpx;
-      wy = cpy;
-      wj = loop;
+      if (G == 1) break;
+      // another round while lane G-1's look-ahead step (base + G C) is on the line
+      if (!__shfl(lookValid, G - 1, G)) break;
     }
-    if (G == 1) break;
-    // another round while lane G-1's look-ahead step (base + G C) is on the line
-    if (!__shfl(lookValid, G - 1, G)) break;
   }
   if (G > 1) sc = scan_group<G>(sc);
   float best_err = sc.best;
