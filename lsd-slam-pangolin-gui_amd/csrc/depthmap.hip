@@ -157,56 +157,17 @@ __device__ inline bool make_and_check_epl(const DepthGeom& g, const lsdg_params&
   return true;
 }
 
-// Outcome of the epipolar search over a set of steps: the smallest error
-// (first step on ties) with its neighbours' errors and error-vector dot
-// products and its position, and the smallest error of the other steps
-// (first step on ties).  Empty entries hold (INFINITY, INT_MAX): the
-// reference only records errors below INFINITY.
+// Outcome of the epipolar search: the smallest error (first step on ties)
+// with its neighbours' errors and error-vector dot products and its
+// position, and the smallest error of the other steps (first step on ties).
+// Empty entries hold (INFINITY, INT_MAX): the reference only records errors
+// below INFINITY.
 struct Scan {
   float best, second;
   int ibest, isecond;
   float errPre, diffPre, errPost, diffPost;
   float bx, by;
 };
-
-__device__ inline bool scan_before(float v, int i, float u, int k) { return v < u || (v == u && i < k); }
-
-// Union of two disjoint step sets.  Over a sequence in step order the
-// reference keeps the first minimum as best and, as second, the first minimum
-// of the remaining steps; both are minima in the (error, step) order, so the
-// union's best is the better of the two bests and its second the better of
-// the winner's second and the loser's best.
-__device__ inline Scan scan_merge(const Scan& a, const Scan& b) {
-  const bool tb = scan_before(b.best, b.ibest, a.best, a.ibest);
-  Scan r = tb ? b : a;
-  const Scan& l = tb ? a : b;
-  if (scan_before(l.best, l.ibest, r.second, r.isecond)) {
-    r.second = l.best;
-    r.isecond = l.ibest;
-  }
-  return r;
-}
-
-// Butterfly over the G lanes of a group; every lane ends with the union.
-template <int G>
-__device__ inline Scan scan_group(Scan s) {
-#pragma unroll
-  for (int m = 1; m < G; m <<= 1) {
-    Scan o;
-    o.best = __shfl_xor(s.best, m, G);
-    o.second = __shfl_xor(s.second, m, G);
-    o.ibest = __shfl_xor(s.ibest, m, G);
-    o.isecond = __shfl_xor(s.isecond, m, G);
-    o.errPre = __shfl_xor(s.errPre, m, G);
-    o.diffPre = __shfl_xor(s.diffPre, m, G);
-    o.errPost = __shfl_xor(s.errPost, m, G);
-    o.diffPost = __shfl_xor(s.diffPost, m, G);
-    o.bx = __shfl_xor(s.bx, m, G);
-    o.by = __shfl_xor(s.by, m, G);
-    s = scan_merge(s, o);
-  }
-  return s;
-}
 
 // Advances the position chain cp_{j+1} = cp_j + inc (float additions, one
 // at a time, as the reference's loop does) from step j to step `to`.
@@ -226,6 +187,43 @@ __device__ inline void walk(float& x, float& y, int& j, int to, float ix, float 
     x += ix;
     y += iy;
     ++j;
+  }
+}
+
+// Samples V_{m0} .. V_{m0+N-1} of the epipolar search (V_m, m >= 2, is
+// sampled at cp_{m-2} + 2 inc; V_{-2} .. V_1 are the initial samples), with
+// the walker (x, y) at cp_j, j <= max(m0 - 2, 0), on entry and at
+// cp_{m0+N-3} on return (when m0 + N - 3 >= j).  The positions come first and
+// every texel fetch is issued before the first blend, so the N samples cost
+// one round trip to memory, not N.  With bx set, (*bx, *by) receives
+// cp_{m0+N-4}: the position of the step whose newest sample is V_{m0+N-2}.
+// Fetches for m <= 1 are clamped to the image and not used.
+template <int N>
+__device__ inline void window_samples(const float* __restrict__ img, int w, int h, float ix, float iy, int m0,
+                                      float i_m2, float i_m1, float i_0, float i_p1, float* out, float& x, float& y,
+                                      int& j, float* bx = nullptr, float* by = nullptr) {
+  float sx[N], sy[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    walk(x, y, j, m0 + t - 2, ix, iy);  // at most one step after the first
+    sx[t] = x + 2.0f * ix;
+    sy[t] = y + 2.0f * iy;
+    if (bx && t == N - 2) {
+      *bx = x;
+      *by = y;
+    }
+  }
+  Quad q[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) q[t] = fetch_quad(img, sx[t], sy[t], w, h);
+  // every fetch's value is taken here, so none is deferred into the
+  // per-sample selects below (one wait for all N, not one per branch)
+#pragma unroll
+  for (int t = 0; t < N; ++t) asm volatile("" ::"v"(q[t].t.x), "v"(q[t].t.y), "v"(q[t].b.x), "v"(q[t].b.y));
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    const int m = m0 + t;
+    out[t] = m >= 2 ? blend_quad(q[t], sx[t], sy[t]) : m == 1 ? i_p1 : m == 0 ? i_0 : m == -1 ? i_m1 : i_m2;
   }
 }
 
@@ -252,12 +250,10 @@ __device__ inline float stereo_dot(const float* a, const float* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4];
 }
 
-// doLineStereo: the matching error (>= 0) or a code -1..-4.  Called by the G
-// lanes of a group (lane g) with the same arguments; all return the same.
-template <int G>
+// doLineStereo: the matching error (>= 0) or a code -1..-4.
 __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, float v, float epxn, float epyn,
                              float min_idepth, float prior_idepth, float max_idepth, const StereoRef& ref,
-                             float& r_idepth, float& r_var, float& r_epl, int g) {
+                             float& r_idepth, float& r_var, float& r_epl) {
   const int width = o.g.w, height = o.g.h;
   const float* kfImg = o.kfImg;
   const float* refImg = ref.img;
@@ -352,12 +348,13 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
     if (pClose[0] <= B || pClose[0] >= Wb || pClose[1] <= B || pClose[1] >= Hb || newLen < 8.0f) return -1.0f;
   }
 
-  // Epipolar search.  With G > 1 the G lanes of a group split the steps into
-  // chunks (lane g takes [base + g C, base + (g + 1) C) of every round);
-  // each lane runs the reference loop body on its chunk from the state the
-  // sequential loop would have there, and the group merges the lanes'
-  // results (scan_merge).  The positions come from the same chain of float
-  // additions as the sequential loop (walk), so every sample is the same.
+  // Epipolar search, in two passes.  The search keeps only what decides the
+  // match: each step's error and the first and second minima (the
+  // reference's selects, in step order).  The best step's other records (the
+  // errors and error-vector dot products of its neighbours, its position) are
+  // then recomputed from the samples around it, which the same chain of float
+  // additions gives again (walk), so they equal what the reference's loop
+  // records when it meets the best.
   const float cp0x = pFar[0], cp0y = pFar[1];
   const float init_m2 = dinterp(refImg, cp0x - 2.0f * incx, cp0y - 2.0f * incy, width);
   const float init_m1 = dinterp(refImg, cp0x - incx, cp0y - incy, width);
@@ -366,15 +363,6 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
   const int maxSteps = P.stereo_max_steps;
   const bool negx = incx < 0.0f, negy = incy < 0.0f;
   const float closeX = pClose[0], closeY = pClose[1];
-  int C = maxSteps;
-  if (G > 1) {
-    // chunk length from the step count to pClose (rounds continue if short)
-    const float nx = incx != 0.0f ? (closeX - cp0x) / incx : INFINITY;
-    const float ny = incy != 0.0f ? (closeY - cp0y) / incy : INFINITY;
-    const float n = fminf(fmaxf(fminf(nx, ny), 0.0f), (float)maxSteps);
-    C = max(((int)n + 3 + G - 1) / G, 6);  // >= 6: a lane's next chunk starts past its warm-up
-    C += C & 1;                            // chunks start on even steps: the e-vector parity is k's
-  }
   Scan sc;
   sc.best = INFINITY;
   sc.second = INFINITY;
@@ -386,17 +374,16 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
   sc.diffPost = NAN;
   sc.bx = -1.0f;
   sc.by = -1.0f;
-  if constexpr (G == 1) {
-    // One lane per line, in two passes.  The search keeps only what decides
-    // the match: each step's error and the first and second minima (the
-    // reference's selects, in step order).  The best step's other records
-    // (the errors and error-vector dot products of its neighbours, its
-    // position) are then recomputed from the samples around it, which the
-    // same chain of float additions gives again (walk), so they equal what
-    // the reference's loop records when it meets the best.
-    float val_m2 = init_m2, val_m1 = init_m1, val = init_0, val_p1 = init_p1;
-    float cpx = cp0x, cpy = cp0y, lx = cp0x, ly = cp0y;
-    int loop = 0;
+  float val_m2 = init_m2, val_m1 = init_m1, val = init_0, val_p1 = init_p1;
+  float cpx = cp0x, cpy = cp0y;
+  int loop = 0;
+  // the step bound, held in a vector register: as a scalar the compiler
+  // reloads it from the kernel arguments at every step (a scalar-memory
+  // round trip per step)
+  int stepLimit;
+  asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
+  {
+    float lx = cpx, ly = cpy;
     // The texels of the next STEREO_AHEAD steps' samples are in flight
     // while a step is evaluated: a lead copy of the position chain fetches
     // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
@@ -414,11 +401,6 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
     // the wave's outstanding-load count would differ between paths and the
     // compiler would wait for all fetches at every step.
     bool alive = true;
-    // the step bound, held in a vector register: as a scalar the compiler
-    // reloads it from the kernel arguments at every step (a scalar-memory
-    // round trip per step)
-    int stepLimit;
-    asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
     for (;;) {
 #pragma unroll
       for (int k = 0; k < STEREO_AHEAD; ++k) {
@@ -446,182 +428,42 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
         cpx += incx;
         cpy += incy;
         ++loop;
+        // Keeps each step's fetch in its step: the next step's position
+        // comes out of a barrier that no memory access crosses.  Without it
+        // the compiler either merges a ring slot's loop-carried loaded value
+        // into one load of a loop-carried address, issued where the value is
+        // used, or gathers the group's fetches after its last step: either
+        // way every group of STEREO_AHEAD steps starts with a round trip to
+        // memory.
+        asm volatile("" : "+v"(cpx), "+v"(cpy)::"memory");
       }
       if (!__any(alive)) break;
     }
-    if (sc.best < INFINITY) {
-      // Samples V_{b-3} .. V_{b+3} around the best step b (V_m, m >= 2, at
-      // cp_{m-2} + 2 inc; V_{-2} .. V_1 the initial samples): the error
-      // vectors of steps b - 1, b and b + 1 (step k's window is V_{k-2} ..
-      // V_{k+2}), and the positions cp_b and cp_{b+1}.
-      const int b = sc.ibest;
-      float V[7];
-      float px = cp0x, py = cp0y, bpx = cp0x, bpy = cp0y;
-      int pj = 0;
-#pragma unroll
-      for (int t = 0; t < 7; ++t) {
-        const int m = b - 3 + t;
-        if (m <= 1) {
-          V[t] = m == 1 ? init_p1 : m == 0 ? init_0 : m == -1 ? init_m1 : init_m2;
-        } else {
-          walk(px, py, pj, m - 2, incx, incy);
-          V[t] = blend_quad(fetch_quad(refImg, px + 2.0f * incx, py + 2.0f * incy, width, height), px + 2.0f * incx,
-                            py + 2.0f * incy);
-        }
-        if (t == 5) {  // m = b + 2: the walker is at cp_b
-          bpx = px;
-          bpy = py;
-        }
-      }
-      // (the walker is now at cp_{b+1})
-      const float ePrev[5] = {V[4] - rv_p2, V[3] - rv_p1, V[2] - rv, V[1] - rv_m1, V[0] - rv_m2};
-      const float eBest[5] = {V[5] - rv_p2, V[4] - rv_p1, V[3] - rv, V[2] - rv_m1, V[1] - rv_m2};
-      const float eNext[5] = {V[6] - rv_p2, V[5] - rv_p1, V[4] - rv, V[3] - rv_m1, V[2] - rv_m2};
-      // the step after the best, if the line went on (the lane was on it at b)
-      const bool post = (b + 1 < maxSteps) & (negx == (px > closeX)) & (negy == (py > closeY));
-      sc.errPre = b == 0 ? -1.0f : stereo_sq(ePrev);
-      sc.diffPre = b == 0 ? NAN : stereo_dot(eBest, ePrev);
-      sc.errPost = post ? stereo_sq(eNext) : -1.0f;
-      sc.diffPost = post ? stereo_dot(eNext, eBest) : -1.0f;
-      sc.bx = bpx;
-      sc.by = bpy;
-    }
-  } else {
-    float wx = cp0x, wy = cp0y;  // walker at cp_{wj}
-    int wj = 0;
-    for (int base = 0;; base += G * C) {
-      const int j0 = base + g * C, j1 = j0 + C;
-      bool lookValid = false;  // step j1 is on the line
-      if (j0 < maxSteps) {
-        // window V_{j0-3} .. V_{j0+1}; V_m for m >= 2 is sampled at cp_{m-2} + 2 inc
-        float W[5];
-        walk(wx, wy, wj, j0 - 5, incx, incy);
-  #pragma unroll
-        for (int t = 0; t < 5; ++t) {
-          const int m = j0 - 3 + t;
-          if (m <= 1) {
-            W[t] = m == 1 ? init_p1 : m == 0 ? init_0 : m == -1 ? init_m1 : init_m2;
-          } else {
-            walk(wx, wy, wj, m - 2, incx, incy);
-            // clamped fetch: a chunk past the line's end may start off the image
-            W[t] = blend_quad(fetch_quad(refImg, wx + 2.0f * incx, wy + 2.0f * incy, width, height),
-                              wx + 2.0f * incx, wy + 2.0f * incy);
-          }
-        }
-        walk(wx, wy, wj, j0, incx, incy);
-        float cpx = wx, cpy = wy;
-        int loop = j0;
-        float val_m2 = W[1], val_m1 = W[2], val = W[3], val_p1 = W[4];
-        float e1A = NAN, e2A = NAN, e3A = NAN, e4A = NAN, e5A = NAN;
-        float e1B = NAN, e2B = NAN, e3B = NAN, e4B = NAN, e5B = NAN;
-        float eeLast = -1.0f;
-        if (j0 > 0) {  // the step before the chunk (odd: the B registers)
-          eeLast = 0.0f;
-          e1B = W[4] - rv_p2; eeLast += e1B * e1B;
-          e2B = W[3] - rv_p1; eeLast += e2B * e2B;
-          e3B = W[2] - rv; eeLast += e3B * e3B;
-          e4B = W[1] - rv_m1; eeLast += e4B * e4B;
-          e5B = W[0] - rv_m2; eeLast += e5B * e5B;
-        }
-        bool bestWasLast = false;
-        // The texels of the next STEREO_AHEAD steps' samples are in flight
-        // while a step is evaluated: a lead copy of the position chain fetches
-        // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
-        // end are clamped to the image and never used.
-        float lx = cpx, ly = cpy;
-        Quad ring[STEREO_AHEAD];
-  #pragma unroll
-        for (int k = 0; k < STEREO_AHEAD; ++k) {
-          ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
-          lx += incx;
-          ly += incy;
-        }
-        // Lanes leave the loop together (a wave-uniform exit every
-        // STEREO_AHEAD steps; finished lanes idle), and every step issues its
-        // fetch whether or not the lane is still searching: with per-lane exits
-        // the wave's outstanding-load count would differ between paths and the
-        // compiler would wait for all fetches at every step.
-        bool alive = true;
-        // the step bound, held in a vector register: as a scalar the
-        // compiler reloads it from the kernel arguments at every step (a
-        // scalar-memory round trip per step)
-        int stepLimit;
-        asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
-        for (;;) {
-  #pragma unroll
-          for (int k = 0; k < STEREO_AHEAD; ++k) {
-            // the reference's while condition, without short-circuit branches
-            const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
-            const bool act = alive & go & (loop < stepLimit);
-            const float val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
-            ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
-            lx += incx;
-            ly += incy;
-            // The step in straight-line code: every lane computes it, and the
-            // bookkeeping applies the reference's branches as selects (a lane
-            // that has left the line keeps its state).
-            float ee = 0.0f;
-            if ((k & 1) == 0) {
-              e1A = val_p2 - rv_p2; ee += e1A * e1A;
-              e2A = val_p1 - rv_p1; ee += e2A * e2A;
-              e3A = val - rv; ee += e3A * e3A;
-              e4A = val_m1 - rv_m1; ee += e4A * e4A;
-              e5A = val_m2 - rv_m2; ee += e5A * e5A;
-            } else {
-              e1B = val_p2 - rv_p2; ee += e1B * e1B;
-              e2B = val_p1 - rv_p1; ee += e2B * e2B;
-              e3B = val - rv; ee += e3B * e3B;
-              e4B = val_m1 - rv_m1; ee += e4B * e4B;
-              e5B = val_m2 - rv_m2; ee += e5B * e5B;
-            }
-            // the next lane's first step: only the error after a best (a
-            // single lane's chunk ends at maxSteps, which act excludes)
-            const bool look = G > 1 && (act & (loop == j1));
-            const bool step = G > 1 ? act & (loop != j1) : act;
-            const bool nb = step & (ee < sc.best);                 // new best
-            const bool post = act & !nb & bestWasLast;             // the step after the best
-            const bool ns = step & !nb & (ee < sc.second);         // new second
-            sc.second = nb ? sc.best : (ns ? ee : sc.second);
-            sc.isecond = nb ? sc.ibest : (ns ? loop : sc.isecond);
-            // a best or the step after one is rare once a line is under way:
-            // its records are written under a wave-uniform branch
-            if (__any(nb | post)) {
-              const float dot = e1A * e1B + e2A * e2B + e3A * e3B + e4A * e4B + e5A * e5B;
-              sc.best = nb ? ee : sc.best;
-              sc.ibest = nb ? loop : sc.ibest;
-              sc.errPre = nb ? eeLast : sc.errPre;
-              sc.diffPre = nb ? dot : sc.diffPre;
-              sc.errPost = nb ? -1.0f : (post ? ee : sc.errPost);
-              sc.diffPost = nb ? -1.0f : (post ? dot : sc.diffPost);
-              sc.bx = nb ? cpx : sc.bx;
-              sc.by = nb ? cpy : sc.by;
-            }
-            bestWasLast = nb | (bestWasLast & !post);
-            lookValid |= look;
-            alive = step;
-            eeLast = ee;
-            val_m2 = val_m1;
-            val_m1 = val;
-            val = val_p1;
-            val_p1 = val_p2;
-            // (one lane per line: the position and count of a lane that has
-            // left it are never read again, so they advance regardless)
-            cpx = (G == 1 || step) ? cpx + incx : cpx;
-            cpy = (G == 1 || step) ? cpy + incy : cpy;
-            loop += (G == 1 || step) ? 1 : 0;
-          }
-          if (!__any(alive)) break;
-        }
-        wx = cpx;
-        wy = cpy;
-        wj = loop;
-      }
-      if (G == 1) break;
-      // another round while lane G-1's look-ahead step (base + G C) is on the line
-      if (!__shfl(lookValid, G - 1, G)) break;
-    }
   }
-  if (G > 1) sc = scan_group<G>(sc);
+  if (sc.best < INFINITY) {
+    // Samples V_{b-3} .. V_{b+3} around the best step b (V_m, m >= 2, at
+    // cp_{m-2} + 2 inc; V_{-2} .. V_1 the initial samples): the error
+    // vectors of steps b - 1, b and b + 1 (step k's window is V_{k-2} ..
+    // V_{k+2}), and the positions cp_b and cp_{b+1}.
+    const int b = sc.ibest;
+    float V[7];
+    float px = cp0x, py = cp0y, bpx, bpy;
+    int pj = 0;
+    window_samples<7>(refImg, width, height, incx, incy, b - 3, init_m2, init_m1, init_0, init_p1, V, px,
+                      py, pj, &bpx, &bpy);
+    // (the walker is now at cp_{b+1}, and (bpx, bpy) = cp_b)
+    const float ePrev[5] = {V[4] - rv_p2, V[3] - rv_p1, V[2] - rv, V[1] - rv_m1, V[0] - rv_m2};
+    const float eBest[5] = {V[5] - rv_p2, V[4] - rv_p1, V[3] - rv, V[2] - rv_m1, V[1] - rv_m2};
+    const float eNext[5] = {V[6] - rv_p2, V[5] - rv_p1, V[4] - rv, V[3] - rv_m1, V[2] - rv_m2};
+    // the step after the best, if the line went on (the search was on it at b)
+    const bool post = (b + 1 < maxSteps) & (negx == (px > closeX)) & (negy == (py > closeY));
+    sc.errPre = b == 0 ? -1.0f : stereo_sq(ePrev);
+    sc.diffPre = b == 0 ? NAN : stereo_dot(eBest, ePrev);
+    sc.errPost = post ? stereo_sq(eNext) : -1.0f;
+    sc.diffPost = post ? stereo_dot(eNext, eBest) : -1.0f;
+    sc.bx = bpx;
+    sc.by = bpy;
+  }
   float best_err = sc.best;
   const float second_err = sc.second;
   const int loopBest = sc.best < INFINITY ? sc.ibest : -1;
@@ -846,28 +688,17 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
   else segw[segcap - 1 - (base[1] + r)] = item;
 }
 
-#ifndef LSDG_STEREO_GROUP
-#define LSDG_STEREO_GROUP 1
-#endif
-// Lanes per observeDepthCreate search (a power of two up to 64).  One lane
-// per search is the fastest measured: lanes splitting a line repeat its
-// setup and each chunk's warm-up, and the stereo pass is bound by how long
-// its longest waves run, not by how many there are.
-constexpr int STEREO_GROUP = LSDG_STEREO_GROUP;
-static_assert(STEREO_GROUP >= 1 && STEREO_GROUP <= 64 && (STEREO_GROUP & (STEREO_GROUP - 1)) == 0,
-              "lanes per search");
 #ifndef LSDG_STEREO_MIN_BLOCKS
 #define LSDG_STEREO_MIN_BLOCKS 3
 #endif
 // Stereo blocks per CU the register budget is held to (3: 168 VGPRs, no spills).
 constexpr int STEREO_MIN_BLOCKS = LSDG_STEREO_MIN_BLOCKS;
 
-// observeDepthCreate item `i` of segment `seg`, searched by the G lanes of a
-// group (lane g); creates search the whole inverse-depth range, so their
-// lines are long.
-template <bool INL, int G>
+// observeDepthCreate item `i` of segment `seg`; creates search the whole
+// inverse-depth range, so their lines are long.
+template <bool INL>
 __device__ inline void observe_create(const ObsArgs& a, const lsdg_params& P, const int4* __restrict__ work,
-                                      int segcap, int seg, int i, int g) {
+                                      int segcap, int seg, int i) {
   const int4 item = work[(size_t)seg * segcap + i];
   const int w = a.g.w;
   const int idx = item.x, x = idx % w, y = idx / w;
@@ -878,8 +709,7 @@ __device__ inline void observe_create(const ObsArgs& a, const lsdg_params& P, co
   ObsCtx o{a.g, a.kfImg, a.kfGrad};
   float rid, rvar, repl;
   float err =
-      line_stereo<G>(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl, g);
-  if (g != 0) return;
+      line_stereo(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
   uint8_t code;
   if (err == -3.0f || err == -2.0f) a.cur.bl[idx] = a.cur.bl[idx] - 1;
   if (err < 0.0f) {
@@ -914,7 +744,7 @@ __device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, co
   if (min_id < 0.0f) min_id = 0.0f;
   if (max_id > 1.0f / P.min_depth) max_id = 1.0f / P.min_depth;
   float rid = 0.0f, rvar = 0.0f, repl = 0.0f;
-  float err = line_stereo<1>(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl, 0);
+  float err = line_stereo(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl);
   float diff = rid - t.dv.z;
   if (err == -1.0f) {
     code = LSDG_OBS_STEREO_FAIL_BASE + 1;
@@ -963,46 +793,73 @@ __device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, co
 }
 
 // Both searches in one persistent launch.  The work is cut into wave-sized
-// chunks: a create chunk is 64 / G consecutive creates of one segment, an
-// update chunk 64 consecutive updates; chunks are numbered creates first
-// (segment by segment), then updates, and wave v of the grid's V waves takes
-// chunks v, v + V, v + 2V, ...  The grid is what the device holds at once,
-// so every wave starts at launch: the long create searches spread over all
-// SIMDs, the short update searches fill in beside them, and no wave is
-// dispatched only to find its part of the list empty.
-template <bool INL, int G>
+// chunks, numbered creates first, then updates.  The creates of all segments,
+// in segment order, form one list cut into chunks of 64; an update chunk is
+// 64 consecutive updates of one segment.  One lane searches one line.
+//
+// The grid is what the device holds at once, so every wave starts at launch.
+// Blocks are dealt to the CUs in turn (block b to the same CU as block
+// b mod (simds / 4)), so the waves fall into rounds of `simds` waves, one per
+// SIMD.  A create wave runs about three times as long as an update wave and
+// the pass ends with its busiest SIMD: round 0 takes the first create chunks,
+// round 1 the rest of the creates and then updates, and round 2 updates only
+// on the CUs whose round-1 block took none of the creates.  Later chunks go
+// to every wave in turn.
+template <bool INL>
 __global__ __launch_bounds__(256, STEREO_MIN_BLOCKS) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
-                                                        const int* __restrict__ count, int segcap) {
-  constexpr int PER = 64 / G;  // creates per chunk
+                                                        const int* __restrict__ count, int segcap, int simds) {
   const int lane = (int)(threadIdx.x & 63);
   const int v = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), V = (int)(gridDim.x * 4);
-  // Chunk counts, one per lane: lane l < OBS_SEGS holds segment l's create
-  // chunks, lane OBS_SEGS + l its update chunks (one vector load for the
-  // whole table), and their inclusive prefix over lanes 0 .. 2 OBS_SEGS - 1.
+  // The counts, one per lane (one vector load for the whole table): lane
+  // l < OBS_SEGS holds segment l's creates and lane OBS_SEGS + l its update
+  // chunks, each with its inclusive prefix over its half of the table.
   int nItems = 0, pre = 0;
   if (lane < 2 * OBS_SEGS) {
     const int s = lane % OBS_SEGS, k = lane / OBS_SEGS;
     nItems = count[(2 * s + k) * OBS_COUNTER_STRIDE];
-    pre = k == 0 ? (nItems + PER - 1) / PER : (nItems + 63) / 64;
+    pre = k == 0 ? nItems : (nItems + 63) / 64;
   }
 #pragma unroll
-  for (int d = 1; d < 2 * OBS_SEGS; d <<= 1) {
+  for (int d = 1; d < OBS_SEGS; d <<= 1) {
     const int o = __shfl_up(pre, d, 64);
-    if (lane >= d) pre += o;
+    if (lane % OBS_SEGS >= d) pre += o;
   }
-  const int nChunk = __shfl(pre, 2 * OBS_SEGS - 1, 64);
-  for (int c = v; c < nChunk; c += V) {
-    // the table lane of chunk c: the first whose prefix exceeds c (prefixes
-    // do not decrease, so the lanes at or below c are the ones before it)
-    const int tl = (int)__popcll(__ballot(lane < 2 * OBS_SEGS && pre <= c));
-    const int q = c - (tl > 0 ? __shfl(pre, tl - 1, 64) : 0);
-    const int n = __shfl(nItems, tl, 64);
-    const bool create = tl < OBS_SEGS;
-    const int seg = tl % OBS_SEGS;
-    if (create) {
-      const int i = q * PER + lane / G;
-      if (i < n) observe_create<INL, G>(a, P, work, segcap, seg, i, lane % G);  // whole groups
+  // the create prefixes as wave-uniform values
+  int cpre[OBS_SEGS];
+#pragma unroll
+  for (int k = 0; k < OBS_SEGS; ++k) cpre[k] = __builtin_amdgcn_readlane(pre, k);
+  const int nCreate = cpre[OBS_SEGS - 1];
+  const int nUpd = __builtin_amdgcn_readlane(pre, 2 * OBS_SEGS - 1);
+  const int nCreateChunks = (nCreate + 63) / 64;
+  const int nChunk = nCreateChunks + nUpd;
+  // the round-2 waves left idle: those of the CUs with creates in round 1
+  const int skip = V >= 3 * simds ? min(simds, (max(nCreateChunks - simds, 0) + 3) & ~3) : 0;
+  const int r = v / simds, rs = v - r * simds;
+  int c = r < 2 || skip == 0 ? v : r == 2 ? (rs < skip ? -1 : v - skip) : v - skip;
+  for (int k = 0; c < nChunk; ++k, c = V - skip + (k - 1) * V + v) {
+    if (c < 0) continue;
+    if (c < nCreateChunks) {
+      const int vi = 64 * c + lane;
+      if (vi < nCreate) {
+        // its segment: the number of segments whose prefix is at or below
+        // it; its index there: vi less the largest such prefix
+        int seg = 0, base = 0;
+#pragma unroll
+        for (int t = 0; t < OBS_SEGS - 1; ++t) {
+          const bool below = cpre[t] <= vi;
+          seg += below ? 1 : 0;
+          base = below ? cpre[t] : base;
+        }
+        observe_create<INL>(a, P, work, segcap, seg, vi - base);
+      }
     } else {
+      const int u = c - nCreateChunks;
+      // the update table lane of chunk u: the first whose prefix exceeds u
+      // (prefixes do not decrease, so the lanes at or below u are the ones
+      // before it)
+      const int seg = (int)__popcll(__ballot(lane >= OBS_SEGS && lane < 2 * OBS_SEGS && pre <= u));
+      const int q = u - (seg > 0 ? __shfl(pre, OBS_SEGS + seg - 1, 64) : 0);
+      const int n = __shfl(nItems, OBS_SEGS + seg, 64);
       const int i = q * 64 + lane;
       if (i < n) observe_update<INL>(a, P, work, segcap, seg, i);
     }
@@ -1672,19 +1529,20 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
   int& resident = dm->stereoBlocks[inl ? 0 : 1];
   if (resident == 0) {
     int perCU = 0;
-    e = inl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_observe_stereo<true, STEREO_GROUP>, 256, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_observe_stereo<false, STEREO_GROUP>, 256, 0);
+    e = inl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_observe_stereo<true>, 256, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_observe_stereo<false>, 256, 0);
     if (e != hipSuccess) return e;
     resident = perCU > 0 ? perCU * dm->ctx->num_cus : 1;
   }
-  const long long maxChunks = (long long)OBS_SEGS * ((segcap + 64 / STEREO_GROUP - 1) / (64 / STEREO_GROUP) + (segcap + 63) / 64);
+  const long long maxChunks = (long long)OBS_SEGS * 2 * ((segcap + 63) / 64);
+  const int simds = 4 * dm->ctx->num_cus;  // 4 SIMDs per CU
   const dim3 gs((unsigned)std::min<long long>(resident, (maxChunks + 3) / 4));
   if (inl) {
     k_observe_gate<true><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
-    k_observe_stereo<true, STEREO_GROUP><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
+    k_observe_stereo<true><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap, simds);
   } else {
     k_observe_gate<false><<<gg, dim3(256), 0, s>>>(a, P, dm->d_work, count, other, segcap);
-    k_observe_stereo<false, STEREO_GROUP><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap);
+    k_observe_stereo<false><<<gs, dim3(256), 0, s>>>(a, P, dm->d_work, count, segcap, simds);
   }
   return hipGetLastError();
 }
