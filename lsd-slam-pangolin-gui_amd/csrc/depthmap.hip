@@ -86,6 +86,27 @@ __device__ inline float blend_quad(const Quad& q, float x, float y) {
   return dxdy * q.b.y + (dy - dxdy) * q.b.x + (dx - dxdy) * q.t.y + (1.0f - dx - dy + dxdy) * q.t.x;
 }
 
+// The quad fetch through a buffer resource over an image that is the same
+// for the whole wave.  The hardware's range check returns zeros for byte
+// offsets at or past the image's end (negative offsets wrap there too), so
+// no clamp is needed: only lead fetches past a line's end, which are never
+// blended, leave [0, w (h - 1) - 2].
+struct TexBuf {
+  __amdgpu_buffer_rsrc_t r;
+  unsigned rowBytes;
+};
+__device__ inline TexBuf tex_buf(const float* img, int w, int h) {
+  // word 3 of the descriptor: 32-bit data format, as for any raw access
+  return TexBuf{__builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, w * h * 4, 0x00020000), (unsigned)w * 4u};
+}
+__device__ inline Quad fetch_quad(const TexBuf& b, float x, float y, int w) {
+  const unsigned top = ((unsigned)(int)x + __umul24((unsigned)(int)y, (unsigned)w)) * 4u;
+  Quad q;
+  q.t = __builtin_bit_cast(Texel2, __builtin_amdgcn_raw_buffer_load_b64(b.r, top, 0, 0));
+  q.b = __builtin_bit_cast(Texel2, __builtin_amdgcn_raw_buffer_load_b64(b.r, top + b.rowBytes, 0, 0));
+  return q;
+}
+
 #ifndef LSDG_STEREO_AHEAD
 #define LSDG_STEREO_AHEAD 8
 #endif
@@ -250,7 +271,10 @@ __device__ inline float stereo_dot(const float* a, const float* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4];
 }
 
-// doLineStereo: the matching error (>= 0) or a code -1..-4.
+// doLineStereo: the matching error (>= 0) or a code -1..-4.  With UNI the
+// reference image is the same for the whole wave and the search loop fetches
+// through a buffer resource.
+template <bool UNI>
 __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, float v, float epxn, float epyn,
                              float min_idepth, float prior_idepth, float max_idepth, const StereoRef& ref,
                              float& r_idepth, float& r_var, float& r_epl) {
@@ -384,6 +408,11 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
   asm("v_mov_b32 %0, %1" : "=v"(stepLimit) : "s"(maxSteps));
   {
     float lx = cpx, ly = cpy;
+    const TexBuf tb = tex_buf(refImg, width, height);
+    const auto ring_fetch = [&](float fx, float fy) {
+      if constexpr (UNI) return fetch_quad(tb, fx, fy, width);
+      else return fetch_quad(refImg, fx, fy, width, height);
+    };
     // The texels of the next STEREO_AHEAD steps' samples are in flight
     // while a step is evaluated: a lead copy of the position chain fetches
     // each sample STEREO_AHEAD steps early.  Lead fetches past the line's
@@ -391,7 +420,7 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
     Quad ring[STEREO_AHEAD];
 #pragma unroll
     for (int k = 0; k < STEREO_AHEAD; ++k) {
-      ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+      ring[k] = ring_fetch(lx + 2.0f * incx, ly + 2.0f * incy);
       lx += incx;
       ly += incy;
     }
@@ -408,7 +437,7 @@ __device__ float line_stereo(const ObsCtx& o, const lsdg_params& P, float u, flo
         const bool go = ((negx == (cpx > closeX)) & (negy == (cpy > closeY))) | (loop == 0);
         const bool act = alive & go & (loop < stepLimit);
         const float val_p2 = blend_quad(ring[k], cpx + 2.0f * incx, cpy + 2.0f * incy);
-        ring[k] = fetch_quad(refImg, lx + 2.0f * incx, ly + 2.0f * incy, width, height);
+        ring[k] = ring_fetch(lx + 2.0f * incx, ly + 2.0f * incy);
         lx += incx;
         ly += incy;
         const float ee = stereo_error(val_p2, val_p1, val, val_m1, val_m2, rv_p2, rv_p1, rv, rv_m1, rv_m2);
@@ -709,7 +738,7 @@ __device__ inline void observe_create(const ObsArgs& a, const lsdg_params& P, co
   ObsCtx o{a.g, a.kfImg, a.kfGrad};
   float rid, rvar, repl;
   float err =
-      line_stereo(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
+      line_stereo<true>(o, P, (float)x, (float)y, epx, epy, 0.0f, 1.0f, 1.0f / P.min_depth, ref, rid, rvar, repl);
   uint8_t code;
   if (err == -3.0f || err == -2.0f) a.cur.bl[idx] = a.cur.bl[idx] - 1;
   if (err < 0.0f) {
@@ -744,7 +773,7 @@ __device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, co
   if (min_id < 0.0f) min_id = 0.0f;
   if (max_id > 1.0f / P.min_depth) max_id = 1.0f / P.min_depth;
   float rid = 0.0f, rvar = 0.0f, repl = 0.0f;
-  float err = line_stereo(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl);
+  float err = line_stereo<false>(o, P, (float)x, (float)y, epx, epy, min_id, t.dv.z, max_id, ref, rid, rvar, repl);
   float diff = rid - t.dv.z;
   if (err == -1.0f) {
     code = LSDG_OBS_STEREO_FAIL_BASE + 1;
