@@ -625,7 +625,7 @@ __device__ inline bool was_good_gate(const StereoRef& r, int x, int y, int w) {
 // b % OBS_SEGS): same-address device atomics serialise, so one counter for
 // the whole grid would cost more than the search itself.  Segment s holds
 // creates from its front and updates from its back; its capacity is the
-// pixel count of its blocks, so it cannot overflow.
+// pixel count of its blocks (GATE_PX x 256 each), so it cannot overflow.
 constexpr int OBS_SEGS = 16;
 constexpr int OBS_COUNTER_STRIDE = 32;  // ints between counters: one 128-byte line each
 
@@ -633,72 +633,120 @@ __device__ inline int lane_rank(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// One thread per pixel of the whole map (pixels outside [3, w-3) x [3, h-3)
-// get LSDG_OBS_NONE).  Appends go to the counter set `count`; the other set,
-// `other`, is cleared for the next update (the two alternate, so no clearing
-// pass runs between updates).
+// A gate block covers GATE_PX x 16 by 16 pixels: thread (tx, ty), tx < 16,
+// takes pixels (16 k + tx, ty), k < GATE_PX, of its block's region, so every
+// load instruction reads 16 consecutive pixels of four rows and each thread
+// has GATE_PX pixels' loads in flight at once.  The shape also sets the work
+// list's order, which the stereo pass feels: a wave's appends are its 16 x 4
+// patches in turn, so the items of a patch are consecutive in the list and
+// a stereo wave's lanes search the lines of neighbouring pixels.  Measured
+// on the 640x480 update: 1, 2, 4 or 8 pixels per thread gave gate + stereo
+// of 43.2, 41.4, 42.1 and 44.5 us; rows of 32 or 64 threads instead of 16
+// (same pixel count) made the stereo pass 45 us instead of 32.
+constexpr int GATE_PX = 2;
+constexpr int GATE_TW = 16;                // threads per row of the block
+constexpr int GATE_TH = 256 / GATE_TW;     // rows of the block
+constexpr int GATE_W = GATE_TW * GATE_PX;  // block region width in pixels
+
+// Pixels outside [3, w-3) x [3, h-3) get LSDG_OBS_NONE.  Appends go to the
+// counter set `count`; the other set, `other`, is cleared for the next update
+// (the two alternate, so no clearing pass runs between updates).
 template <bool INL>
 __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, int4* __restrict__ work,
                                                       int* __restrict__ count, int* __restrict__ other, int segcap) {
   const int w = a.g.w, h = a.g.h;
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int y = blockIdx.y * GATE_TH + (int)(threadIdx.x / GATE_TW);
+  const int x0 = blockIdx.x * GATE_W + (int)(threadIdx.x % GATE_TW);
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2 * OBS_SEGS)
     other[threadIdx.x * OBS_COUNTER_STRIDE] = 0;
-  int kind = 0;  // 1 observeDepthCreate search, 2 observeDepthUpdate search
-  int idx = 0, ri = 0;
-  float epx = 0.0f, epy = 0.0f;
-  if (x < w && y < h && (x < 3 || x >= w - 3 || y < 3 || y >= h - 3)) {
-    a.obs[x + y * w] = LSDG_OBS_NONE;
-  } else if (x < w && y < h) {
-    idx = x + y * w;
-    // every per-pixel input is read up front (all in bounds: the pixel is
-    // interior), so the tests below wait on memory once, and once more for
-    // the reference's tracking mask
-    const bool has = a.cur.valid[idx] != 0;
-    const int bl = a.cur.bl[idx];
-    const float mg = a.kfMaxg[idx];
-    const float nsid = a.cur.nsid[idx];
-    const float* I = a.kfImg;
-    const float gx = I[idx + 1] - I[idx - 1];
-    const float gy = I[idx + w] - I[idx - w];
-    uint8_t code = LSDG_OBS_NONE;
-    if (has && mg < P.min_abs_grad_decrease) {
-      a.cur.valid[idx] = 0;
-      code = LSDG_OBS_LOWGRAD_INVALIDATE;
-    } else if (mg < P.min_abs_grad_create || bl < P.min_blacklist) {
-      code = LSDG_OBS_NONE;
-    } else {
-      bool go = true;
-      if (!has) {
-        ri = a.kfReactivated ? a.newestIdx : a.oldestIdx;
-      } else if (!a.kfReactivated) {
-        const int k = (int)nsid - a.refByIdOffset;
-        if (k >= a.nRefById) {
-          code = LSDG_OBS_SKIP_ALREADY_GOOD;
-          go = false;
-        } else {
-          ri = k < 0 ? a.oldestIdx : obs_by_id<INL>(a, k);
-        }
-      } else {
-        ri = a.newestIdx;
-      }
-      if (go) {
-        const StereoRef& ref = obs_ref<INL>(a, ri);
-        if (!was_good_gate(ref, x, y, w)) code = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
-        else if (!make_and_check_epl(a.g, P, x, y, gx, gy, ref, &epx, &epy)) code = LSDG_OBS_EPL_FAIL;
-        else kind = has ? 2 : 1;
-      }
-    }
-    if (kind == 0) a.obs[idx] = code;
+  const float* I = a.kfImg;
+  // every per-pixel input of the thread's pixels is read up front (interior
+  // pixels only, so every neighbour is in bounds): the tests below wait on
+  // memory once, and once more for the reference's tracking mask
+  bool has[GATE_PX], inner[GATE_PX];
+  int bl[GATE_PX];
+  float mg[GATE_PX], nsid[GATE_PX], gx[GATE_PX], gy[GATE_PX];
+#pragma unroll
+  for (int k = 0; k < GATE_PX; ++k) {
+    const int x = x0 + GATE_TW * k;
+    inner[k] = x >= 3 && x < w - 3 && y >= 3 && y < h - 3;
+    const int idx = inner[k] ? x + y * w : w + 1;  // (an interior pixel when not inner: loads stay in bounds)
+    has[k] = a.cur.valid[idx] != 0;
+    bl[k] = a.cur.bl[idx];
+    mg[k] = a.kfMaxg[idx];
+    nsid[k] = a.cur.nsid[idx];
+    gx[k] = I[idx + 1] - I[idx - 1];
+    gy[k] = I[idx + w] - I[idx - w];
   }
-  // block-wide ranks: per-wave ballots, then one atomic pair per block
+  // the tests up to the choice of reference frame, then every pixel's read
+  // of that frame's tracking mask together, then makeAndCheckEPL
+  int kind[GATE_PX];  // 0 done here, 1 observeDepthCreate search, 2 observeDepthUpdate search
+  int ri[GATE_PX];
+  uint8_t code[GATE_PX];
+  bool go[GATE_PX];
+#pragma unroll
+  for (int k = 0; k < GATE_PX; ++k) {
+    const int x = x0 + GATE_TW * k;
+    kind[k] = 0;
+    ri[k] = 0;
+    code[k] = LSDG_OBS_NONE;
+    go[k] = false;
+    if (!inner[k]) continue;  // (border pixel: LSDG_OBS_NONE; none outside the map)
+    if (has[k] && mg[k] < P.min_abs_grad_decrease) {
+      a.cur.valid[x + y * w] = 0;
+      code[k] = LSDG_OBS_LOWGRAD_INVALIDATE;
+    } else if (mg[k] < P.min_abs_grad_create || bl[k] < P.min_blacklist) {
+      code[k] = LSDG_OBS_NONE;
+    } else if (!has[k]) {
+      ri[k] = a.kfReactivated ? a.newestIdx : a.oldestIdx;
+      go[k] = true;
+    } else if (!a.kfReactivated) {
+      const int q = (int)nsid[k] - a.refByIdOffset;
+      if (q >= a.nRefById) {
+        code[k] = LSDG_OBS_SKIP_ALREADY_GOOD;
+      } else {
+        ri[k] = q < 0 ? a.oldestIdx : obs_by_id<INL>(a, q);
+        go[k] = true;
+      }
+    } else {
+      ri[k] = a.newestIdx;
+      go[k] = true;
+    }
+  }
+  bool good[GATE_PX];
+#pragma unroll
+  for (int k = 0; k < GATE_PX; ++k) good[k] = !go[k] || was_good_gate(obs_ref<INL>(a, ri[k]), x0 + GATE_TW * k, y, w);
+  float epx[GATE_PX], epy[GATE_PX];
+#pragma unroll
+  for (int k = 0; k < GATE_PX; ++k) {
+    const int x = x0 + GATE_TW * k;
+    epx[k] = 0.0f;
+    epy[k] = 0.0f;
+    if (go[k]) {
+      if (!good[k]) code[k] = LSDG_OBS_SKIP_NOT_GOOD_TRACKED;
+      else if (!make_and_check_epl(a.g, P, x, y, gx[k], gy[k], obs_ref<INL>(a, ri[k]), &epx[k], &epy[k]))
+        code[k] = LSDG_OBS_EPL_FAIL;
+      else kind[k] = has[k] ? 2 : 1;
+    }
+    if (kind[k] == 0 && x < w && y < h) a.obs[x + y * w] = code[k];
+  }
+  // block-wide ranks: per-wave ballots, then one atomic pair per block; an
+  // item's slot counts the block's earlier waves, then the wave's earlier
+  // pixel columns k, then the lanes before it
   __shared__ int wcount[4][2], base[2];
   const int wave = threadIdx.x >> 6;
-  const unsigned long long mc = __ballot(kind == 1), mu = __ballot(kind == 2);
+  unsigned long long mc[GATE_PX], mu[GATE_PX];
+  int tc = 0, tu = 0;
+#pragma unroll
+  for (int k = 0; k < GATE_PX; ++k) {
+    mc[k] = __ballot(kind[k] == 1);
+    mu[k] = __ballot(kind[k] == 2);
+    tc += __popcll(mc[k]);
+    tu += __popcll(mu[k]);
+  }
   if (__lane_id() == 0) {
-    wcount[wave][0] = __popcll(mc);
-    wcount[wave][1] = __popcll(mu);
+    wcount[wave][0] = tc;
+    wcount[wave][1] = tu;
   }
   __syncthreads();
   const int seg = (blockIdx.x + blockIdx.y * gridDim.x) % OBS_SEGS;
@@ -708,13 +756,21 @@ __global__ __launch_bounds__(256) void k_observe_gate(ObsArgs a, lsdg_params P, 
     base[k] = tot ? atomicAdd(&count[(2 * seg + k) * OBS_COUNTER_STRIDE], tot) : 0;
   }
   __syncthreads();
-  if (kind == 0) return;
-  int r = kind == 1 ? lane_rank(mc) : lane_rank(mu);
-  for (int v = 0; v < wave; ++v) r += wcount[v][kind - 1];
-  const int4 item = make_int4(idx, ri, __float_as_int(epx), __float_as_int(epy));
+  int rc = base[0], ru = base[1];
+  for (int v = 0; v < wave; ++v) {
+    rc += wcount[v][0];
+    ru += wcount[v][1];
+  }
   int4* segw = work + (size_t)seg * segcap;
-  if (kind == 1) segw[base[0] + r] = item;
-  else segw[segcap - 1 - (base[1] + r)] = item;
+#pragma unroll
+  for (int k = 0; k < GATE_PX; ++k) {
+    const int x = x0 + GATE_TW * k;
+    const int4 item = make_int4(x + y * w, ri[k], __float_as_int(epx[k]), __float_as_int(epy[k]));
+    if (kind[k] == 1) segw[rc + lane_rank(mc[k])] = item;
+    else if (kind[k] == 2) segw[segcap - 1 - (ru + lane_rank(mu[k]))] = item;
+    rc += __popcll(mc[k]);
+    ru += __popcll(mu[k]);
+  }
 }
 
 #ifndef LSDG_STEREO_MIN_BLOCKS
@@ -1507,8 +1563,8 @@ static inline dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 
 static_assert(OBS_COUNT_BYTES >= 2 * 2 * OBS_SEGS * OBS_COUNTER_STRIDE * sizeof(int), "observe counter sets");
 
 size_t obs_work_slots(int w, int h) {
-  const size_t bx = (size_t)(w + 15) / 16 + 1, by = (size_t)(h + 15) / 16 + 1;
-  return ((bx * by + OBS_SEGS - 1) / OBS_SEGS) * OBS_SEGS * 256;
+  const size_t bx = (size_t)(w + GATE_W - 1) / GATE_W, by = (size_t)(h + GATE_TH - 1) / GATE_TH;
+  return ((bx * by + OBS_SEGS - 1) / OBS_SEGS) * OBS_SEGS * 256 * GATE_PX;
 }
 
 hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRef* h_refs, int nRefs,
@@ -1544,10 +1600,10 @@ hipError_t launch_observe(lsdg_depthmap dm, const lsdg_params& P, const StereoRe
     a.refs = dm->d_refs;
     a.refById = dm->d_refById;
   }
-  const dim3 gg = grid16(dm->g.w, dm->g.h);
-  // a segment takes every OBS_SEGS-th gate block: 256 slots per block
+  const dim3 gg((unsigned)((dm->g.w + GATE_W - 1) / GATE_W), (unsigned)((dm->g.h + GATE_TH - 1) / GATE_TH));
+  // a segment takes every OBS_SEGS-th gate block: one slot per pixel
   const int segBlocks = (int)((gg.x * gg.y + OBS_SEGS - 1) / OBS_SEGS);
-  const int segcap = segBlocks * 256;
+  const int segcap = segBlocks * 256 * GATE_PX;
   if ((size_t)segcap * OBS_SEGS > obs_work_slots(dm->g.w, dm->g.h)) return hipErrorInvalidValue;
   const int set = 2 * OBS_SEGS * OBS_COUNTER_STRIDE;
   int* count = dm->d_work_count + dm->obsParity * set;
