@@ -1168,6 +1168,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_regularize(DepthGeom g, HypBuf
 // stores the filled map (`filled`: the reference's other map afterwards) and
 // the regularized one (`out`), and, when kfIdepth is set, Frame::setDepth's
 // level-0 idepth / variance of the regularized map.
+// The tile a block of a gx x gy grid takes so that each XCD holds one band
+// of consecutive tiles in raster order.  Blocks are dealt to the XCDs in
+// turn (block b to XCD b % 8), and a tile kernel's halo is its neighbours'
+// pixels: with the tiles in raster order the neighbours sit on other XCDs
+// and every halo is fetched again into another L2.
+__device__ inline int2 xcd_tile(int gx, int gy) {
+  constexpr int XCDS = 8;
+  const int n = gx * gy, b = (int)(blockIdx.x + blockIdx.y * gx);
+  const int q = n / XCDS, r = n % XCDS, x = b % XCDS, slot = b / XCDS;
+  const int t = x * q + min(x, r) + slot;  // XCD x takes q + (x < r) tiles
+  return make_int2(t % gx, t / gx);
+}
+
 static_assert(PYR == 5, "a 16x16 tile holds whole 2^(PYR-1) blocks of level-0 pixels");
 __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, HypBuf in, HypBuf filled, HypBuf out,
                                                                   const float* __restrict__ maxg, lsdg_params P,
@@ -1177,7 +1190,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, H
   constexpr int AITER = (24 * 24 + TILE_THREADS - 1) / TILE_THREADS;
   constexpr int FITER = (20 * 20 + TILE_THREADS - 1) / TILE_THREADS;
   const int w = g.w, h = g.h;
-  const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+  const int2 tile = xcd_tile(gridDim.x, gridDim.y);
+  const int bx = tile.x * 16, by = tile.y * 16;
   const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
   const int x = bx + lx, y = by + ly;
   const bool inside = x < w && y < h;
@@ -1265,7 +1279,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_regularize(DepthGeom g, H
     const int tx = threadIdx.x % n, ty = threadIdx.x / n;
     float oi = -1.0f, ov = -1.0f;
     if ((int)threadIdx.x < n * n) {
-      const int px = blockIdx.x * n + tx, py = blockIdx.y * n + ty;
+      const int px = tile.x * n + tx, py = tile.y * n + ty;
       if (px < kf.w[l] && py < kf.h[l]) {
         const float ia[4] = {pid[2 * ty][2 * tx], pid[2 * ty][2 * tx + 1], pid[2 * ty + 1][2 * tx],
                              pid[2 * ty + 1][2 * tx + 1]};
