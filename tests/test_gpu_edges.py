@@ -13,7 +13,10 @@ GPU path:
   plain alignment; one job more than a batch holds is refused;
 * a depth update whose reference frame is the keyframe itself (zero
   baseline: every epipolar line is degenerate), bit-exact against the FP32
-  restatement."""
+  restatement;
+* a depth update on a ragged 200x150 map (partial gate blocks and fill
+  tiles on the right and bottom edges, a tile count that does not divide
+  over the XCDs), bit-exact against the FP32 restatement."""
 import ctypes as C
 
 import numpy as np
@@ -139,12 +142,15 @@ def _depth_run(sys_, d, same_frame=False):
     return dm.read()
 
 
-@pytest.mark.parametrize("case", ["smallest-frame", "zero-baseline"])
+DEPTH_EDGE_CAMS = {"smallest-frame": (1221, 64, 64), "zero-baseline": (1222, 160, 120), "ragged": (1223, 200, 150)}
+
+
+@pytest.mark.parametrize("case", list(DEPTH_EDGE_CAMS))
 def test_depth_update_edges_bit_exact(ctx, case):
     p = pyoracle.default_params()
     p.max_epl_length_crop = 100.0
-    d = synth.depth_instance(1221, cam=synth.camera(64, 64)) if case == "smallest-frame" else \
-        synth.depth_instance(1222, cam=synth.camera(160, 120))
+    seed, w, h = DEPTH_EDGE_CAMS[case]
+    d = synth.depth_instance(seed, cam=synth.camera(w, h))
     same = case == "zero-baseline"
     ctx.set_params(p)
     try:
