@@ -890,11 +890,11 @@ __device__ inline void observe_update(const ObsArgs& a, const lsdg_params& P, co
 // The grid is what the device holds at once, so every wave starts at launch.
 // Blocks are dealt to the CUs in turn (block b to the same CU as block
 // b mod (simds / 4)), so the waves fall into rounds of `simds` waves, one per
-// SIMD.  A create wave runs about three times as long as an update wave and
-// the pass ends with its busiest SIMD: round 0 takes the first create chunks,
-// round 1 the rest of the creates and then updates, and round 2 updates only
-// on the CUs whose round-1 block took none of the creates.  Later chunks go
-// to every wave in turn.
+// SIMD.  A create wave (lines of about 100 steps) outlasts an update wave
+// (about 30) and the pass ends with its busiest SIMD: round 0 takes the
+// first create chunks, round 1 the rest of the creates and then updates, and
+// round 2 updates only on the CUs whose round-1 block took none of the
+// creates.  Later chunks go to every wave in turn.
 template <bool INL>
 __global__ __launch_bounds__(256, STEREO_MIN_BLOCKS) void k_observe_stereo(ObsArgs a, lsdg_params P, const int4* __restrict__ work,
                                                         const int* __restrict__ count, int segcap, int simds) {
